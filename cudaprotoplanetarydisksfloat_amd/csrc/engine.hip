@@ -1,0 +1,721 @@
+// engine.hip -- device state of a batch of disks and the AlgoGas driver.
+//
+// A batch holds N >= 1 disks of possibly different sizes in one set of
+// device arrays (one array per field, disks back to back).  One call of
+// AlgoGas (SourceEuler.cu:208-301) for the batch is a loop of substeps; each
+// substep is a fixed sequence of kernel launches that each cover every disk
+// still inside its DT.  dt, the CFL count, planets and time live on the
+// device, so the host only learns the substep count of the batch: it reads a
+// two-int summary every few substeps instead of synchronising per kernel as
+// the reference does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fgx_internal.h"
+#include "kernels.hpp"
+
+namespace fgx {
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return set_error(FGX_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                                 \
+  } while (0)
+
+struct BatchBase {
+  virtual ~BatchBase() {}
+  virtual int step(int n_dt) = 0;
+  virtual int read_field(int disk, int field, int dtype, void* host, size_t count) = 0;
+  virtual int write_field(int disk, int field, int dtype, const void* host, size_t count) = 0;
+  virtual int read_ring(int disk, int ring, int dtype, void* host, size_t count) = 0;
+  virtual int info(int disk, fgx_disk_info* out) = 0;
+  virtual int planets(int disk, fgx_planets* out) = 0;
+  virtual int set_planets(int disk, const fgx_planets* in) = 0;
+  virtual int force(int disk, double x, double y, double sm, double m, double* out44) = 0;
+  virtual int mdcp(int disk, double* out) = 0;
+  virtual int profile_enable(int on) = 0;
+  virtual int profile_get(int k, const char** name, double* ms, long long* n, double* bytes) = 0;
+  virtual int profile_reset() = 0;
+  virtual const HostDisk& host(int d) const = 0;
+  virtual int size() const = 0;
+};
+
+template <class real>
+struct Batch : BatchBase {
+  int nd = 0;
+  std::vector<HostDisk> hd;
+  std::vector<DiskConst> dch;
+  hipStream_t stream = nullptr;
+  Dev<real> D{};
+  std::vector<void*> allocs;
+  int n_ctiles = 0, n_rtiles = 0, max_nsec = 0, max_nrad = 0;
+  size_t F = 0, R = 0, S = 0;
+  int* h_summary = nullptr;  // pinned
+  int visc_dynamic = 0, alpha = 0, need_force = 1;
+  size_t theta_shmem = 0;
+  // profiling
+  struct Prof { std::string name; double ms = 0; long long n = 0; double bytes = 0; };
+  bool prof_on = false;
+  std::vector<Prof> prof;
+  struct Pending { int k; hipEvent_t a, b; double bytes; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+
+  ~Batch() override {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto e : free_events) hipEventDestroy(e);
+    for (void* p : allocs) hipFree(p);
+    if (h_summary) hipHostFree(h_summary);
+    if (stream) hipStreamDestroy(stream);
+  }
+  int size() const override { return nd; }
+  const HostDisk& host(int d) const override { return hd[d]; }
+
+  template <class T>
+  int dalloc(T** p, size_t n) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return set_error(FGX_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    allocs.push_back(q);
+    HIPCHK(hipMemsetAsync(q, 0, bytes, stream));
+    *p = (T*)q;
+    return FGX_OK;
+  }
+
+  // ------------------------------------------------------------- profiling
+  int prof_index(const char* name) {
+    for (size_t k = 0; k < prof.size(); ++k)
+      if (prof[k].name == name) return (int)k;
+    prof.push_back(Prof{name});
+    return (int)prof.size() - 1;
+  }
+  hipEvent_t get_event() {
+    if (!free_events.empty()) { hipEvent_t e = free_events.back(); free_events.pop_back(); return e; }
+    hipEvent_t e; hipEventCreate(&e); return e;
+  }
+  void prof_collect() {
+    for (auto& p : pending) {
+      float ms = 0;
+      hipEventSynchronize(p.b);
+      hipEventElapsedTime(&ms, p.a, p.b);
+      prof[p.k].ms += ms; prof[p.k].n += 1; prof[p.k].bytes += p.bytes;
+      free_events.push_back(p.a); free_events.push_back(p.b);
+    }
+    pending.clear();
+  }
+
+  // algorithmic bytes of one launch: bytes per cell of the kernel's distinct
+  // full-field reads and writes, times the cells of the batch
+  double cells() const {
+    double c = 0;
+    for (auto& h : hd) c += (double)h.nrad * h.nsec;
+    return c;
+  }
+  template <class K, class... A>
+  int launch(const char* name, double bytes_per_cell, K kern, int grid, int block, size_t shmem, A... args) {
+    if (grid <= 0) return FGX_OK;
+    Pending pd{};
+    if (prof_on) {
+      pd.k = prof_index(name); pd.a = get_event(); pd.b = get_event();
+      pd.bytes = bytes_per_cell * sizeof(real) * cells();
+      hipEventRecord(pd.a, stream);
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, stream, args...);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(FGX_EDEVICE, "launch of %s failed: %s", name, hipGetErrorString(e));
+    if (prof_on) { hipEventRecord(pd.b, stream); pending.push_back(pd); }
+    return FGX_OK;
+  }
+#define LCHK(x) do { int r_ = (x); if (r_) return r_; } while (0)
+
+  // ------------------------------------------------------------------ create
+  int create(const std::vector<fgx_params>& ps, const std::vector<fgx_planets>& pls) {
+    nd = (int)ps.size();
+    if (nd < 1) return set_error(FGX_EPARAM, "empty batch");
+    const fgx_params& P0 = ps[0];
+    for (int d = 0; d < nd; ++d) {
+      const fgx_params& p = ps[d];
+      LCHK(params_validate(&p));
+      if (p.adiabatic != P0.adiabatic || p.open_inner != P0.open_inner || p.nonreflecting != P0.nonreflecting ||
+          p.evanescent != P0.evanescent || p.fast_transport != P0.fast_transport ||
+          p.self_gravity != P0.self_gravity || p.cooling != P0.cooling || p.indirect_term != P0.indirect_term ||
+          p.exclude_hill != P0.exclude_hill || p.sloppy_cfl != P0.sloppy_cfl || p.corotating != P0.corotating ||
+          p.guiding_center != P0.guiding_center || p.forced_circular != P0.forced_circular ||
+          p.roche_smoothing != P0.roche_smoothing || p.viscosity_alpha != P0.viscosity_alpha ||
+          p.ext_fix_taurp_corner != P0.ext_fix_taurp_corner || p.ext_fix_pp_distance != P0.ext_fix_pp_distance ||
+          p.is_disk != P0.is_disk)
+        return set_error(FGX_EPARAM, "disk %d of the batch differs from disk 0 in a mode flag", d);
+      if (p.self_gravity) return set_error(FGX_EPARAM, "self-gravity is not available in this build");
+      if (p.corotating && pls[d].nb == 0) return set_error(FGX_EPARAM, "a corotating frame needs a planet");
+      if (!p.is_disk) return set_error(FGX_EPARAM, "DISK NO (planets only) is not supported");
+    }
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void**)&h_summary, 4 * sizeof(int)));
+    hd.resize(nd);
+    for (int d = 0; d < nd; ++d) LCHK(setup_grid_and_density(ps[d], pls[d], hd[d]));
+    for (int d = 0; d < nd; ++d) setup_velocities(hd[d], {});
+    // layout
+    dch.resize(nd);
+    std::vector<CellTile> ct;
+    std::vector<RingTile> rt;
+    for (int d = 0; d < nd; ++d) {
+      const HostDisk& h = hd[d];
+      DiskConst& c = dch[d];
+      std::memset(&c, 0, sizeof c);
+      c.nrad = h.nrad; c.nsec = h.nsec;
+      c.fo = (long long)F; c.ro = (int)R; c.so = (int)S;
+      c.rt0 = (int)rt.size();
+      F += (size_t)(h.nrad + 1) * h.nsec;
+      R += (size_t)h.nrad + 1;
+      S += (size_t)h.nsec;
+      max_nsec = std::max(max_nsec, h.nsec);
+      max_nrad = std::max(max_nrad, h.nrad);
+      for (int i = 0; i < h.nrad; ++i) {
+        rt.push_back(RingTile{d, i});
+        for (int jb = 0; jb * FGX_TPB < h.nsec; ++jb) ct.push_back(CellTile{d, i, jb, 0});
+      }
+      const fgx_params& p = h.P;
+      c.nb = h.sys.nb;
+      for (int k = 0; k < c.nb; ++k) {
+        c.feel_disk[k] = h.sys.feel_disk[k]; c.feel_others[k] = h.sys.feel_others[k]; c.mass[k] = h.sys.mass[k];
+      }
+      c.DT = p.dt; c.masstaper = p.masstaper; c.sigmaslope = p.sigmaslope; c.flaringindex = p.flaringindex;
+      c.aspectratio = p.aspectratio; c.adiabaticindex = p.adiabaticindex;
+      c.thicknesssmoothing = p.thicknesssmoothing; c.rochesmoothing = p.rochesmoothing;
+      c.releasedate = p.releasedate; c.releaseradius = p.releaseradius;
+      c.transitionradius = p.transitionradius; c.transitionratio = p.transitionratio;
+      c.transitionwidth = p.transitionwidth; c.lambdadoubling = p.lambdadoubling;
+      c.cavityradius = p.cavityradius; c.cavityratio = p.cavityratio; c.cavitywidth = p.cavitywidth;
+      c.viscosity = p.viscosity; c.alphaviscosity = p.alphaviscosity;
+      c.hill_radius = h.hill_radius; c.sigma0 = p.sigma0; c.eccentricity = p.eccentricity;
+      c.substeps_forced = p.ext_substeps;
+      c.visc_dynamic = (p.viscosity_alpha || p.lambdadoubling != 0.0f) ? 1 : 0;
+      visc_dynamic |= c.visc_dynamic;
+    }
+    alpha = P0.viscosity_alpha;
+    while (ct.size() % 8) ct.push_back(CellTile{-1, 0, 0, 0});
+    while (rt.size() % 8) rt.push_back(RingTile{-1, 0});
+    n_ctiles = (int)ct.size();
+    n_rtiles = (int)rt.size();
+    // device arrays
+    const int ad = P0.adiabatic;
+    auto f = [&](real** p, bool need) -> int { return dalloc(p, need ? F : 1); };
+    LCHK(f(&D.dens, true)); LCHK(f(&D.vrad, true)); LCHK(f(&D.vtheta, true)); LCHK(f(&D.energy, ad));
+    LCHK(f(&D.cs, ad)); LCHK(f(&D.vradint, true)); LCHK(f(&D.vthetaint, true));
+    LCHK(f(&D.taurr, true)); LCHK(f(&D.taupp, true)); LCHK(f(&D.taurp, true));
+    LCHK(f(&D.divv, ad)); LCHK(f(&D.qplus, ad)); LCHK(f(&D.energyint, ad)); LCHK(f(&D.pressure, true));
+    LCHK(f(&D.t_dens, true)); LCHK(f(&D.t_rmp, true)); LCHK(f(&D.t_rmm, true));
+    LCHK(f(&D.t_tmp, true)); LCHK(f(&D.t_tmm, true)); LCHK(f(&D.t_e, ad)); LCHK(f(&D.work, true));
+    LCHK(f(&D.sg_accr, false)); LCHK(f(&D.sg_acct, false));
+    real** rings[] = {&D.rinf, &D.rsup, &D.rmed, &D.surf, &D.invsurf, &D.invrmed, &D.invrinf, &D.invdiffrmed,
+                      &D.invdiffrsup, &D.dxtheta, &D.supp_torque, &D.sigmamed, &D.energymed, &D.coolingtime,
+                      &D.qplusmed, &D.visc, &D.cs_ring, &D.csmean, &D.vmed, &D.newdt};
+    for (real** r : rings) LCHK(dalloc(r, R));
+    LCHK(dalloc(&D.nshift, R));
+    LCHK(dalloc(&D.cosj, S)); LCHK(dalloc(&D.sinj, S)); LCHK(dalloc(&D.lost, S));
+    DiskConst* dcd; DiskState* std_;
+    LCHK(dalloc(&dcd, nd)); LCHK(dalloc(&std_, nd));
+    D.dc = dcd; D.st = std_;
+    LCHK(dalloc(&D.fpart, (size_t)n_rtiles * 2 * FGX_NFORCE_T));
+    LCHK(dalloc(&D.rpart, (size_t)std::max(max_nrad, 1) * 44));
+    LCHK(dalloc(&D.summary, 4));
+    CellTile* ctd; RingTile* rtd;
+    LCHK(dalloc(&ctd, ct.size())); LCHK(dalloc(&rtd, rt.size()));
+    D.ctiles = ctd; D.rtiles = rtd;
+    D.max_nsec = max_nsec; D.ndisk = nd;
+    D.adiabatic = ad; D.fast_transport = P0.fast_transport; D.open_inner = P0.open_inner;
+    D.nonreflecting = P0.nonreflecting; D.evanescent = P0.evanescent; D.self_gravity = P0.self_gravity;
+    D.cooling = P0.cooling; D.indirect = P0.indirect_term; D.exclude_hill = P0.exclude_hill;
+    D.fix_taurp = P0.ext_fix_taurp_corner; D.fix_pp = P0.ext_fix_pp_distance; D.fix_sgwrap = P0.ext_fix_sg_wrap;
+    D.sloppy = P0.sloppy_cfl; D.corotating = P0.corotating; D.guiding = P0.guiding_center;
+    D.forced_circular = P0.forced_circular; D.roche = P0.roche_smoothing;
+    D.write_pressure = (!ad && P0.write_temperature) ? 1 : 0;
+    D.ignore_active = 0;
+    int viscous = 0;
+    for (int d = 0; d < nd; ++d)
+      for (int i = 0; i < hd[d].nrad; ++i) viscous |= (hd[d].visc[i] != 0.0);
+    D.viscous = viscous || visc_dynamic;
+    // uploads
+    HIPCHK(hipMemcpyAsync((void*)ctd, ct.data(), ct.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync((void*)rtd, rt.data(), rt.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync((void*)dcd, dch.data(), nd * sizeof(DiskConst), hipMemcpyHostToDevice, stream));
+    std::vector<DiskState> sth(nd);
+    for (int d = 0; d < nd; ++d) {
+      DiskState& s = sth[d];
+      std::memset(&s, 0, sizeof s);
+      const HostDisk& h = hd[d];
+      for (int k = 0; k < h.sys.nb; ++k) {
+        s.x[k] = h.sys.x[k]; s.y[k] = h.sys.y[k]; s.vx[k] = h.sys.vx[k]; s.vy[k] = h.sys.vy[k];
+      }
+      s.omega_frame = h.omega_frame;
+      s.active = 0;
+    }
+    HIPCHK(hipMemcpyAsync((void*)std_, sth.data(), nd * sizeof(DiskState), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    LCHK(upload_rings());
+    for (int d = 0; d < nd; ++d) {
+      LCHK(upload_field(d, D.dens, hd[d].dens));
+      LCHK(upload_field(d, D.vrad, hd[d].vrad));
+      LCHK(upload_field(d, D.vtheta, hd[d].vtheta));
+      if (ad) LCHK(upload_field(d, D.energy, hd[d].energy));
+    }
+    // initial pressure (InitEuler's ComputePressureField), used by the
+    // isothermal temperature output
+    D.ignore_active = 1;
+    LCHK(launch("pressure_init", 2, k_pressure_init<real>, n_ctiles, FGX_TPB, 0, D));
+    if (ad) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D));
+    D.ignore_active = 0;
+    // LDS of the azimuthal transport kernel
+    theta_shmem = (size_t)FGX_THETA_ARRAYS * max_nsec * sizeof(real);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    size_t lds_max = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : prop.sharedMemPerBlock;
+    if (theta_shmem + 1024 > lds_max)
+      return set_error(FGX_EPARAM, "NSEC=%d needs %zu B of LDS per ring in this precision; the device has %zu", max_nsec,
+                       theta_shmem, lds_max);
+    if (theta_shmem > 48 * 1024)
+      HIPCHK(hipFuncSetAttribute((const void*)k_trans_theta<real>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)theta_shmem));
+    HIPCHK(hipStreamSynchronize(stream));
+    return FGX_OK;
+  }
+
+  template <class V>
+  int upload_vec(real* dst, const V& src, size_t n) {
+    std::vector<real> tmp(n);
+    for (size_t k = 0; k < n; ++k) tmp[k] = (real)src[k];
+    HIPCHK(hipMemcpy(dst, tmp.data(), n * sizeof(real), hipMemcpyHostToDevice));
+    return FGX_OK;
+  }
+  int upload_field(int d, real* base, const std::vector<double>& v) {
+    return upload_vec(base + dch[d].fo, v, v.size());
+  }
+  int upload_rings() {
+    for (int d = 0; d < nd; ++d) {
+      const HostDisk& h = hd[d];
+      const int o = dch[d].ro;
+      const size_t n = (size_t)h.nrad + 1;
+      LCHK(upload_vec(D.rinf + o, h.rinf, n)); LCHK(upload_vec(D.rsup + o, h.rsup, n));
+      LCHK(upload_vec(D.rmed + o, h.rmed, n)); LCHK(upload_vec(D.surf + o, h.surf, n));
+      LCHK(upload_vec(D.invsurf + o, h.invsurf, n)); LCHK(upload_vec(D.invrmed + o, h.invrmed, n));
+      LCHK(upload_vec(D.invrinf + o, h.invrinf, n)); LCHK(upload_vec(D.invdiffrmed + o, h.invdiffrmed, n));
+      LCHK(upload_vec(D.invdiffrsup + o, h.invdiffrsup, n)); LCHK(upload_vec(D.dxtheta + o, h.dxtheta, n));
+      LCHK(upload_vec(D.supp_torque + o, h.supp_torque, n)); LCHK(upload_vec(D.sigmamed + o, h.sigmamed, n));
+      LCHK(upload_vec(D.energymed + o, h.energymed, n)); LCHK(upload_vec(D.coolingtime + o, h.coolingtime, n));
+      LCHK(upload_vec(D.qplusmed + o, h.qplusmed, n)); LCHK(upload_vec(D.visc + o, h.visc, n));
+      LCHK(upload_vec(D.cs_ring + o, h.cs_ring, n));
+      // ring-mean sound speed as Make1Dprofile(1) leaves it after set-up
+      std::vector<double> csm(n, 0.0);
+      if (!h.P.adiabatic) csm = h.cs_ring;
+      else
+        for (int i = 0; i < h.nrad; ++i) {
+          double s = 0;
+          for (int j = 0; j < h.nsec; ++j) {
+            size_t k = (size_t)i * h.nsec + j;
+            s += (double)(real)std::sqrt((double)h.P.adiabaticindex * (h.P.adiabaticindex - 1.0) *
+                                         (double)(real)h.energy[k] / (double)(real)h.dens[k]);
+          }
+          csm[i] = s / h.nsec;
+        }
+      LCHK(upload_vec(D.csmean + o, csm, n));
+      LCHK(upload_vec(D.cosj + dch[d].so, h.cosj, h.nsec));
+      LCHK(upload_vec(D.sinj + dch[d].so, h.sinj, h.nsec));
+    }
+    return FGX_OK;
+  }
+
+  // ------------------------------------------------------------------ step
+  int boundary() {
+    if (D.open_inner) LCHK(launch("open_boundary", 0, k_open_boundary<real>, nd, FGX_TPB, 0, D));
+    if (D.nonreflecting) {
+      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D));
+      LCHK(launch("nonreflecting", 0, k_nonreflecting<real>, nd, FGX_TPB, 0, D));
+    }
+    if (D.evanescent) LCHK(launch("evanescent", D.adiabatic ? 8 : 6, k_evanescent<real>, n_ctiles, FGX_TPB, 0, D));
+    return FGX_OK;
+  }
+  int substep() {
+    const int ad = D.adiabatic;
+    LCHK(launch("cfl", 3, k_cfl_ring<real>, n_rtiles, FGX_TPB, 0, D));
+    LCHK(launch("substep_begin", 0, k_substep_begin<real>, nd, FGX_TPB, 0, D));
+    LCHK(launch("force", 1, k_force_ring<real>, n_rtiles, FGX_TPB, 0, D));
+    LCHK(launch("planets", 0, k_planets<real>, nd, FGX_TPB, 0, D));
+    if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_ctiles, FGX_TPB, 0, D));
+    if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
+    LCHK(boundary());
+    LCHK(launch("substep1", ad ? 6 : 5, k_substep1<real>, n_ctiles, FGX_TPB, 0, D));
+    if (D.viscous) {
+      if (visc_dynamic) LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
+      LCHK(launch("visc_tau", 6, k_visc_tau<real>, n_ctiles, FGX_TPB, 0, D, (const real*)D.vradint,
+                  (const real*)D.vthetaint, 0));
+    }
+    LCHK(launch("visc_update", 7, k_visc_update<real>, n_ctiles, FGX_TPB, 0, D));
+    LCHK(launch("substep2", ad ? 7 : 5, k_substep2<real>, n_ctiles, FGX_TPB, 0, D));
+    LCHK(boundary());
+    if (ad) {
+      if (visc_dynamic) LCHK(launch("visc_profile", 1, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
+      LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_ctiles, FGX_TPB, 0, D, (const real*)D.vrad, (const real*)D.vtheta, 1));
+      LCHK(launch("substep3", 9, k_substep3<real>, n_ctiles, FGX_TPB, 0, D));
+    }
+    LCHK(launch("trans_rad", ad ? 10 : 8, k_trans_rad<real>, n_ctiles, FGX_TPB, 0, D));
+    LCHK(launch("trans_theta", ad ? 11 : 9, k_trans_theta<real>, n_rtiles, FGX_TPB, theta_shmem, D));
+    LCHK(launch("trans_vr", 4, k_trans_vr<real>, n_ctiles, FGX_TPB, 0, D));
+    LCHK(boundary());
+    LCHK(launch("end_substep", 0, k_end_substep<real>, nd, 64, 0, D));
+    return FGX_OK;
+  }
+  int step(int n_dt) override {
+    for (int k = 0; k < n_dt; ++k) {
+      LCHK(launch("begin_dt", 0, k_begin_dt<real>, (nd + 63) / 64, 64, 0, D));
+      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D));
+      int plan = 0;
+      for (int guard = 0;; ++guard) {
+        if (guard > 10000000) return set_error(FGX_ENUMERIC, "substep loop did not terminate");
+        LCHK(substep());
+        if (plan > 1) { --plan; continue; }
+        LCHK(launch("summary", 0, k_summary<real>, 1, FGX_TPB, 0, D));
+        HIPCHK(hipMemcpyAsync(h_summary, D.summary, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (prof_on) prof_collect();
+        if (h_summary[0] == 0) break;
+        plan = std::max(1, h_summary[1]);
+      }
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    if (prof_on) prof_collect();
+    return FGX_OK;
+  }
+
+  // ------------------------------------------------------------- accessors
+  real* field_ptr(int field) {
+    switch (field) {
+      case FGX_DENS: return D.dens;
+      case FGX_VRAD: return D.vrad;
+      case FGX_VTHETA: return D.vtheta;
+      case FGX_ENERGY: return D.adiabatic ? D.energy : nullptr;
+      case FGX_SOUNDSPEED: return D.adiabatic ? D.cs : nullptr;
+      case FGX_DIVV: return D.adiabatic ? D.divv : nullptr;
+      case FGX_QPLUS: return D.adiabatic ? D.qplus : nullptr;
+      default: return nullptr;
+    }
+  }
+  int read_field(int d, int field, int dtype, void* host, size_t count) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    const HostDisk& h = hd[d];
+    const size_t nfull = (size_t)(h.nrad + 1) * h.nsec, ncell = (size_t)h.nrad * h.nsec;
+    if (count < ncell) return set_error(FGX_ERANGE, "buffer of %zu values < nrad*nsec = %zu", count, ncell);
+    const size_t n = std::min(count, nfull);
+    std::vector<real> tmp(nfull, (real)0);
+    real* src = field_ptr(field);
+    if (field == FGX_TEMPERATURE || field == FGX_POTENTIAL) {
+      D.ignore_active = 1;
+      int r = (field == FGX_TEMPERATURE) ? launch("temperature", 0, k_temperature<real>, n_ctiles, FGX_TPB, 0, D)
+                                         : launch("potential", 0, k_potential_field<real>, n_ctiles, FGX_TPB, 0, D);
+      D.ignore_active = 0;
+      LCHK(r);
+      src = D.work;
+    } else if (field == FGX_LABEL) {
+      for (size_t k = 0; k < n; ++k) tmp[k] = (real)h.label[k];
+      src = nullptr;
+    } else if (field == FGX_SOUNDSPEED && !D.adiabatic) {
+      for (int i = 0; i < h.nrad; ++i)
+        for (int j = 0; j < h.nsec; ++j) tmp[(size_t)i * h.nsec + j] = (real)h.cs_ring[i];
+    } else if (!src && field != FGX_ENERGY && field != FGX_DIVV && field != FGX_QPLUS) {
+      return set_error(FGX_ERANGE, "unknown field %d", field);
+    }
+    if (src) {
+      HIPCHK(hipMemcpyAsync(tmp.data(), src + dch[d].fo, nfull * sizeof(real), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    if (dtype == FGX_F32) for (size_t k = 0; k < n; ++k) ((float*)host)[k] = (float)tmp[k];
+    else for (size_t k = 0; k < n; ++k) ((double*)host)[k] = (double)tmp[k];
+    return FGX_OK;
+  }
+  int write_field(int d, int field, int dtype, const void* host, size_t count) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    const HostDisk& h = hd[d];
+    const size_t nfull = (size_t)(h.nrad + 1) * h.nsec, ncell = (size_t)h.nrad * h.nsec;
+    if (count < ncell) return set_error(FGX_ERANGE, "buffer of %zu values < nrad*nsec = %zu", count, ncell);
+    real* dst = field_ptr(field);
+    if (!dst || field == FGX_SOUNDSPEED || field == FGX_DIVV || field == FGX_QPLUS)
+      return set_error(FGX_ERANGE, "field %d is not writable", field);
+    const size_t n = std::min(count, nfull);
+    std::vector<real> tmp(n);
+    if (dtype == FGX_F32) for (size_t k = 0; k < n; ++k) tmp[k] = (real)((const float*)host)[k];
+    else for (size_t k = 0; k < n; ++k) tmp[k] = (real)((const double*)host)[k];
+    HIPCHK(hipMemcpyAsync(dst + dch[d].fo, tmp.data(), n * sizeof(real), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (field == FGX_DENS || field == FGX_ENERGY) {
+      D.ignore_active = 1;
+      int r = launch("pressure_init", 2, k_pressure_init<real>, n_ctiles, FGX_TPB, 0, D);
+      if (!r && D.adiabatic) r = launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D);
+      D.ignore_active = 0;
+      LCHK(r);
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    return FGX_OK;
+  }
+  int read_ring(int d, int ring, int dtype, void* host, size_t count) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    const HostDisk& h = hd[d];
+    const size_t n = std::min(count, (size_t)h.nrad + 1);
+    std::vector<double> out(n, 0.0);
+    real* src = nullptr;
+    switch (ring) {
+      case FGX_R_RADII: for (size_t i = 0; i < n; ++i) out[i] = (real)h.radii[i]; break;
+      case FGX_R_RMED: src = D.rmed; break;
+      case FGX_R_SURF: src = D.surf; break;
+      case FGX_R_SIGMAMED: src = D.sigmamed; break;
+      case FGX_R_ENERGYMED: src = D.energymed; break;
+      case FGX_R_VISCOSITY: src = D.visc; break;
+      case FGX_R_VMED: src = D.vmed; break;
+      case FGX_R_CS: src = D.cs_ring; break;
+      case FGX_R_NEWDT: src = D.newdt; break;
+      case FGX_R_NSHIFT: {
+        std::vector<int> tmp(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), D.nshift + dch[d].ro, n * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < n; ++i) out[i] = tmp[i];
+        break;
+      }
+      default: return set_error(FGX_ERANGE, "unknown ring profile %d", ring);
+    }
+    if (src) {
+      std::vector<real> tmp(n);
+      HIPCHK(hipMemcpyAsync(tmp.data(), src + dch[d].ro, n * sizeof(real), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (size_t i = 0; i < n; ++i) out[i] = (double)tmp[i];
+    }
+    if (dtype == FGX_F32) for (size_t i = 0; i < n; ++i) ((float*)host)[i] = (float)out[i];
+    else for (size_t i = 0; i < n; ++i) ((double*)host)[i] = out[i];
+    return FGX_OK;
+  }
+  int get_state(int d, DiskState& s) {
+    HIPCHK(hipMemcpyAsync(&s, D.st + d, sizeof(DiskState), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return FGX_OK;
+  }
+  int info(int d, fgx_disk_info* o) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    DiskState s;
+    LCHK(get_state(d, s));
+    std::memset(o, 0, sizeof *o);
+    o->nrad = hd[d].nrad; o->nsec = hd[d].nsec; o->precision = sizeof(real) == 8 ? FGX_F64 : FGX_F32;
+    o->physical_time = s.time; o->omega_frame = s.omega_frame; o->lost_mass = s.lost_mass;
+    o->last_dt = s.dt; o->last_ncfl = s.ncfl; o->substeps = s.substeps; o->dt_steps = s.dt_steps;
+    o->health = s.health;
+    double m = 0;
+    if (s.substeps > 0) LCHK(mdcp(d, &m));
+    o->mdcp1 = m; o->exces_mdcp = m;  // mdcp is never set by the reference (Main.cu:341)
+    return FGX_OK;
+  }
+  int planets(int d, fgx_planets* out) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    DiskState s;
+    LCHK(get_state(d, s));
+    *out = hd[d].sys;
+    for (int k = 0; k < out->nb; ++k) { out->x[k] = s.x[k]; out->y[k] = s.y[k]; out->vx[k] = s.vx[k]; out->vy[k] = s.vy[k]; }
+    return FGX_OK;
+  }
+  int set_planets(int d, const fgx_planets* in) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    if (in->nb != hd[d].sys.nb) return set_error(FGX_EPARAM, "planet count differs from the disk's");
+    DiskState s;
+    LCHK(get_state(d, s));
+    for (int k = 0; k < in->nb; ++k) { s.x[k] = in->x[k]; s.y[k] = in->y[k]; s.vx[k] = in->vx[k]; s.vy[k] = in->vy[k]; }
+    HIPCHK(hipMemcpyAsync(D.st + d, &s, sizeof(DiskState), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return FGX_OK;
+  }
+  int force(int d, double x, double y, double sm, double m, double* out44) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    const int nr = hd[d].nrad;
+    LCHK(launch("force_full", 1, k_force_full<real>, nr, FGX_TPB, 0, D, d, x, y, sm, m));
+    std::vector<double> part((size_t)nr * 44);
+    HIPCHK(hipMemcpyAsync(part.data(), D.rpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int q = 0; q < 44; ++q) {
+      double s = 0;
+      for (int i = 0; i < nr; ++i) s += part[(size_t)i * 44 + q];
+      out44[q] = s;
+    }
+    return FGX_OK;
+  }
+  int mdcp(int d, double* out) override {
+    if (hd[d].sys.nb == 0) { *out = 0; return FGX_OK; }
+    DiskState s;
+    LCHK(get_state(d, s));
+    const int nr = hd[d].nrad;
+    LCHK(launch("mdcp", 1, k_mdcp<real>, nr, FGX_TPB, 0, D, d, (double)(float)s.x[0], (double)(float)s.y[0],
+                hd[d].hill_radius));
+    std::vector<double> part(nr);
+    HIPCHK(hipMemcpyAsync(part.data(), D.rpart, nr * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double a = 0;
+    for (double v : part) a += v;
+    *out = a;
+    return FGX_OK;
+  }
+  int profile_enable(int on) override {
+    if (!on && prof_on) { hipStreamSynchronize(stream); prof_collect(); }
+    prof_on = on != 0;
+    return FGX_OK;
+  }
+  int profile_get(int k, const char** name, double* ms, long long* n, double* bytes) override {
+    if (k < 0 || k >= (int)prof.size()) { if (name) *name = nullptr; return FGX_ERANGE; }
+    if (name) *name = prof[k].name.c_str();
+    if (ms) *ms = prof[k].ms;
+    if (n) *n = prof[k].n;
+    if (bytes) *bytes = prof[k].n ? prof[k].bytes / prof[k].n : 0;
+    return FGX_OK;
+  }
+  int profile_reset() override { prof.clear(); return FGX_OK; }
+};
+
+}  // namespace fgx
+
+// ============================================================== C ABI
+struct fgx_batch {
+  std::unique_ptr<fgx::BatchBase> impl;
+};
+
+using fgx::set_error;
+
+static int make_batch(const std::vector<fgx_params>& ps, const std::vector<fgx_planets>& pls, int precision,
+                      fgx_batch** out) {
+  if (!out) return set_error(FGX_EPARAM, "null output handle");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(FGX_EDEVICE, "no HIP device available");
+  std::unique_ptr<fgx_batch> b(new fgx_batch());
+  int r;
+  if (precision == FGX_F64) {
+    auto* impl = new fgx::Batch<double>();
+    b->impl.reset(impl);
+    r = impl->create(ps, pls);
+  } else if (precision == FGX_F32) {
+    auto* impl = new fgx::Batch<float>();
+    b->impl.reset(impl);
+    r = impl->create(ps, pls);
+  } else {
+    return set_error(FGX_EPARAM, "precision must be FGX_F32 or FGX_F64");
+  }
+  if (r) return r;
+  *out = b.release();
+  return FGX_OK;
+}
+
+extern "C" {
+
+int fgx_disk_create(const fgx_params* p, const fgx_planets* pl, int precision, fgx_disk** out) {
+  if (!p || !pl) return set_error(FGX_EPARAM, "null parameters");
+  return make_batch({*p}, {*pl}, precision, out);
+}
+int fgx_batch_create(int n, const fgx_params* p, const fgx_planets* pl, int precision, fgx_batch** out) {
+  if (n < 1 || !p || !pl) return set_error(FGX_EPARAM, "bad batch arguments");
+  return make_batch(std::vector<fgx_params>(p, p + n), std::vector<fgx_planets>(pl, pl + n), precision, out);
+}
+int fgx_disk_step(fgx_disk* d, int n) {
+  if (!d) return set_error(FGX_EPARAM, "null handle");
+  if (n < 0) return set_error(FGX_EPARAM, "negative step count");
+  return d->impl->step(n);
+}
+int fgx_batch_step(fgx_batch* b, int n) { return fgx_disk_step(b, n); }
+int fgx_disk_destroy(fgx_disk* d) { delete d; return FGX_OK; }
+int fgx_batch_destroy(fgx_batch* b) { delete b; return FGX_OK; }
+int fgx_batch_size(fgx_batch* b, int* n) {
+  if (!b || !n) return set_error(FGX_EPARAM, "null argument");
+  *n = b->impl->size();
+  return FGX_OK;
+}
+int fgx_disk_info_get(fgx_disk* d, fgx_disk_info* info) {
+  if (!d || !info) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->info(0, info);
+}
+int fgx_batch_info(fgx_batch* b, int disk, fgx_disk_info* info) {
+  if (!b || !info) return set_error(FGX_EPARAM, "null argument");
+  return b->impl->info(disk, info);
+}
+int fgx_disk_read_field(fgx_disk* d, int field, int dtype, void* host, size_t count) {
+  if (!d || !host) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->read_field(0, field, dtype, host, count);
+}
+int fgx_batch_read_field(fgx_batch* b, int disk, int field, int dtype, void* host, size_t count) {
+  if (!b || !host) return set_error(FGX_EPARAM, "null argument");
+  return b->impl->read_field(disk, field, dtype, host, count);
+}
+int fgx_disk_write_field(fgx_disk* d, int field, int dtype, const void* host, size_t count) {
+  if (!d || !host) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->write_field(0, field, dtype, host, count);
+}
+int fgx_batch_write_field(fgx_batch* b, int disk, int field, int dtype, const void* host, size_t count) {
+  if (!b || !host) return set_error(FGX_EPARAM, "null argument");
+  return b->impl->write_field(disk, field, dtype, host, count);
+}
+int fgx_disk_read_ring(fgx_disk* d, int ring, int dtype, void* host, size_t count) {
+  if (!d || !host) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->read_ring(0, ring, dtype, host, count);
+}
+int fgx_disk_planets(fgx_disk* d, fgx_planets* out) {
+  if (!d || !out) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->planets(0, out);
+}
+int fgx_disk_set_planets(fgx_disk* d, const fgx_planets* pl) {
+  if (!d || !pl) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->set_planets(0, pl);
+}
+int fgx_disk_force(fgx_disk* d, double x, double y, double sm, double m, double out[44]) {
+  if (!d || !out) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->force(0, x, y, sm, m, out);
+}
+int fgx_profile_enable(fgx_batch* b, int enable) {
+  if (!b) return set_error(FGX_EPARAM, "null handle");
+  return b->impl->profile_enable(enable);
+}
+int fgx_profile_get(fgx_batch* b, int k, const char** name, double* total_ms, long long* launches,
+                    double* bytes_per_launch) {
+  if (!b) return set_error(FGX_EPARAM, "null handle");
+  return b->impl->profile_get(k, name, total_ms, launches, bytes_per_launch);
+}
+int fgx_profile_reset(fgx_batch* b) {
+  if (!b) return set_error(FGX_EPARAM, "null handle");
+  return b->impl->profile_reset();
+}
+int fgx_device_count(int* n) {
+  if (!n) return set_error(FGX_EPARAM, "null argument");
+  *n = 0;
+  if (hipGetDeviceCount(n) != hipSuccess) *n = 0;
+  return FGX_OK;
+}
+int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes) {
+  if (!p || !bytes) return set_error(FGX_EPARAM, "null argument");
+  const size_t F = (size_t)(p->nrad + 1) * p->nsec, R = (size_t)p->nrad + 1;
+  const size_t rs = precision == FGX_F64 ? 8 : 4;
+  const int nfields = p->adiabatic ? 23 : 17;
+  *bytes = F * rs * nfields + R * rs * 21 + (size_t)p->nsec * rs * 3 + R * 8 * 2 * FGX_NFORCE_T + 4096;
+  return FGX_OK;
+}
+
+}  // extern "C"
+
+// internal accessors for the output writers (output.cpp)
+namespace fgx {
+const HostDisk* batch_host(fgx_batch* b, int d) { return &b->impl->host(d); }
+int batch_mdcp(fgx_batch* b, int d, double* out) { return b->impl->mdcp(d, out); }
+}  // namespace fgx

@@ -1,0 +1,113 @@
+// fgx_internal.h -- declarations shared by the host set-up, the HIP engine
+// and the C ABI.  Not installed; the public surface is include/fgx.h.
+#pragma once
+#include "../../include/fgx.h"
+
+#include <string>
+#include <vector>
+
+namespace fgx {
+
+// thread-local last error; returns `code`
+int set_error(int code, const char* fmt, ...);
+const char* last_error();
+
+// ---- parameters / planets (params.cpp) ------------------------------------
+void params_defaults(fgx_params* p);
+int params_parse_text(const char* text, fgx_params* p);
+int params_set_key(fgx_params* p, const char* key, const char* value);
+int params_get_key(const fgx_params* p, const char* key, char* out, size_t n);
+int params_validate(const fgx_params* p);
+int planets_parse_text(const char* text, const fgx_params* p, const double* rmed,
+                       const double* radii, fgx_planets* out);
+bool read_file(const char* path, std::string& out);
+
+constexpr double kG = 1.0;
+constexpr double kPI = 3.14159265358979323844;   // include/Fondam.cuh:3
+constexpr double kCFLSECURITY = 0.5;             // include/Fondam.cuh:7
+constexpr double kCVNR = 1.41;                   // include/Fondam.cuh:10
+
+// ---- host-side set-up of one disk, FP64 (setup.cpp) --------------------------
+// Everything main() computes before its loop that does not need the device:
+// ring geometry, the analytic profiles, the initial fields.
+struct HostDisk {
+  fgx_params P;
+  fgx_planets sys;
+  int nrad = 0, nsec = 0;
+  // ring tables, nrad+1 entries (FillPolar1DArrays, SourceEuler.cu:76-157)
+  std::vector<double> radii, rinf, rsup, rmed, surf, invrmed, invsurf, invdiffrsup,
+      invdiffrmed, invrinf, sigmamed, sigmainf, energymed, coolingtime, qplusmed,
+      supp_torque, dxtheta, visc, cs_ring;
+  std::vector<double> cosj, sinj;  // per sector
+  // initial fields, (nrad+1)*nsec
+  std::vector<double> dens, vrad, vtheta, energy, label;
+  double omega_frame = 0, hill_radius = 0;
+};
+
+// grid, profiles, density (with noise) and energy; velocities are filled by
+// setup_velocities once the self-gravity field (if any) is known
+int setup_grid_and_density(const fgx_params& p, const fgx_planets& pl, HostDisk& h);
+// InitGasVelocities / InitVelocities (Pframeforce.cu:194-272); sg_axi is the
+// ring mean of the radial SG acceleration, or empty without self-gravity
+void setup_velocities(HostDisk& h, const std::vector<double>& sg_axi);
+// FViscosity at time t (Viscosity.cu:19-37); csmean = ring-mean sound speed
+double fviscosity(const fgx_params& P, double r, double csmean, double t);
+double aspect_ratio(const fgx_params& P, double r, double t);
+// GetPsysInfo(FREQUENCY) (Psys.cu:162-232)
+double psys_frequency(const fgx_params& P, const fgx_planets& s);
+
+// engine accessors used by the output writers (engine.hip)
+const HostDisk* batch_host(fgx_batch* b, int d);
+int batch_mdcp(fgx_batch* b, int d, double* out);
+
+}  // namespace fgx
+
+// ---- device-visible per-disk records ------------------------------------
+#define FGX_NFORCE_T (1 + FGX_MAX_PLANETS)  // star + planets
+
+// constant description of one disk of a batch
+struct DiskConst {
+  int nrad, nsec;
+  long long fo;       // element offset of this disk in every field array
+  int ro;             // offset in the ring arrays (nrad+1 entries per disk)
+  int so;             // offset in the sector arrays (nsec entries per disk)
+  int rt0;            // index of this disk's first ring tile
+  int nb;             // planets
+  int feel_disk[FGX_MAX_PLANETS], feel_others[FGX_MAX_PLANETS];
+  double mass[FGX_MAX_PLANETS];
+  double DT, masstaper, sigmaslope, flaringindex, aspectratio, adiabaticindex;
+  double thicknesssmoothing, rochesmoothing, releasedate, releaseradius;
+  double transitionradius, transitionratio, transitionwidth, lambdadoubling;
+  double cavityradius, cavityratio, cavitywidth, viscosity, alphaviscosity;
+  double hill_radius, sigma0, eccentricity;
+  int substeps_forced;     // SUBSTEPS key, 0 = CFL
+  int visc_dynamic;        // recompute the viscosity profile each substep
+  double sg_eps, sg_rstep, sg_tstep;
+};
+
+// mutable FP64 state of one disk (planets, time, frame, per-substep scalars)
+struct DiskState {
+  double time, time0, dtemp, dt, omega_frame, masstaper;
+  double x[FGX_MAX_PLANETS], y[FGX_MAX_PLANETS], vx[FGX_MAX_PLANETS], vy[FGX_MAX_PLANETS];
+  // potential of this substep (FillForcesArrays, Pframeforce.cu:37-66)
+  double pot_x[FGX_MAX_PLANETS], pot_y[FGX_MAX_PLANETS], pot_m[FGX_MAX_PLANETS],
+      pot_sm2[FGX_MAX_PLANETS], pot_inv3[FGX_MAX_PLANETS];
+  double ind_x, ind_y;                  // IndirectTerm
+  // force targets of this substep: star + feel-disk planets
+  double tx[FGX_NFORCE_T], ty[FGX_NFORCE_T], tsm2[FGX_NFORCE_T], trh2[FGX_NFORCE_T];
+  int tk[FGX_NFORCE_T];                 // Hill-cut variant (0 or 10)
+  int ntarget;
+  double mark_x, mark_y;                // GetPsysInfo(MARK) statics
+  double domega;                        // corotation correction of this substep
+  double vkep_in, vkep_out;             // ApplySubKeplerianBoundary values
+  double lost_mass, mdcp1, exces_mdcp;
+  double ev_tin, ev_tout, ev_drmin, ev_drmax;  // EvanescentBoundary constants
+  long long substeps, dt_steps;
+  int ncfl, active, sub_in_dt, health, rem_est;
+  int nr_iangle, nr_iangle2;            // non-reflecting shifts of this call
+};
+
+// a field tile: ring i of disk d, sectors [jb*256, jb*256+256)
+struct CellTile { int d, i, jb, pad; };
+// a ring tile: ring i of disk d
+struct RingTile { int d, i; };

@@ -1,0 +1,1239 @@
+// kernels.hpp -- HIP kernels of the AlgoGas step for gfx950 (CDNA4).
+//
+// Every kernel serves a whole batch of disks: the launch grid walks a tile
+// table (CellTile = one ring x 256 sectors, RingTile = one ring) built on the
+// host; the block -> tile mapping keeps consecutive tiles on one XCD so ring
+// neighbours share that XCD's L2.  Fields are ring-major with sectors
+// contiguous, so a wave64 reads 64 consecutive sectors of a ring: all global
+// traffic is coalesced.  `real` is float (the reference's precision) or
+// double (verification mode).  Per-disk scalars and planets are FP64.
+//
+// Each kernel names the reference kernels whose arithmetic it carries.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fgx_internal.h"
+
+#define FGX_TPB 256
+#define FGX_THETA_ARRAYS 8
+
+namespace fgx {
+
+template <class real>
+struct Dev {
+  // state and work fields, (nrad+1)*nsec per disk
+  real *dens, *vrad, *vtheta, *energy, *cs;
+  real *vradint, *vthetaint, *taurr, *taupp, *taurp, *divv, *qplus, *energyint, *pressure;
+  real *t_dens, *t_rmp, *t_rmm, *t_tmp, *t_tmm, *t_e;
+  real *work, *sg_accr, *sg_acct;
+  // ring tables, nrad+1 per disk
+  real *rinf, *rsup, *rmed, *surf, *invsurf, *invrmed, *invrinf, *invdiffrmed, *invdiffrsup;
+  real *dxtheta, *supp_torque, *sigmamed, *energymed, *coolingtime, *qplusmed, *visc;
+  real *cs_ring, *csmean, *vmed, *newdt;
+  int* nshift;
+  // sector tables, nsec per disk
+  real *cosj, *sinj, *lost;
+  const DiskConst* dc;
+  DiskState* st;
+  double* fpart;       // per ring tile: 2*FGX_NFORCE_T force partial sums
+  double* rpart;       // per ring tile: generic partial sums (44 max)
+  int* summary;        // [0] active disks, [1] max ncfl
+  const CellTile* ctiles;
+  const RingTile* rtiles;
+  int max_nsec, ndisk;
+  // batch-wide mode flags
+  int adiabatic, fast_transport, open_inner, nonreflecting, evanescent, self_gravity;
+  int cooling, indirect, exclude_hill, fix_taurp, fix_pp, fix_sgwrap, sloppy, corotating;
+  int guiding, forced_circular, roche, write_pressure, ignore_active, viscous;
+};
+
+// ---- small helpers ------------------------------------------------------
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  // n is a multiple of 8; hardware hands blocks to the 8 XCDs round-robin
+  const int per = n >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+// block of FGX_TPB threads; every thread gets the result
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  T r = 0;
+#pragma unroll
+  for (int k = 0; k < FGX_TPB / 64; ++k) r += red[k];
+  return r;
+}
+template <class T>
+__device__ __forceinline__ T block_min(T v, T* red) {
+  v = wave_min(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  T r = red[0];
+#pragma unroll
+  for (int k = 1; k < FGX_TPB / 64; ++k) r = red[k] < r ? red[k] : r;
+  return r;
+}
+__device__ __forceinline__ double aspect_ratio_dev(const DiskConst& c, double r, double t) {
+  double h = c.aspectratio;
+  double rmin = c.transitionradius - c.transitionwidth * c.aspectratio;
+  double rmax = c.transitionradius + c.transitionwidth * c.aspectratio;
+  double scale = 1.0 + t * c.lambdadoubling;
+  rmin *= scale; rmax *= scale;
+  if (r < rmin) h *= c.transitionratio;
+  if (r >= rmin && r <= rmax) h *= exp((rmax - r) / (rmax - rmin) * log(c.transitionratio));
+  return h;
+}
+__device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, double r, double csm, double t) {
+  double nu = c.viscosity;
+  if (alpha) nu = c.alphaviscosity * csm * csm * pow(r, 1.5);
+  double rmin = c.cavityradius - c.cavitywidth * c.aspectratio;
+  double rmax = c.cavityradius + c.cavitywidth * c.aspectratio;
+  double scale = 1.0 + t * c.lambdadoubling;
+  rmin *= scale; rmax *= scale;
+  if (r < rmin) nu *= c.cavityratio;
+  if (r >= rmin && r <= rmax) nu *= exp((rmax - r) / (rmax - rmin) * log(c.cavityratio));
+  return nu;
+}
+
+#define FGX_CELL_PROLOGUE                                                   \
+  const CellTile t = D.ctiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
+  if (t.d < 0) return;                                                      \
+  const DiskConst& c = D.dc[t.d];                                           \
+  DiskState& s = D.st[t.d];                                                 \
+  if (!D.ignore_active && !s.active) return;                                \
+  const int ns = c.nsec, nr = c.nrad;                                       \
+  const int i = t.i;                                                        \
+  const int j = t.jb * FGX_TPB + (int)threadIdx.x;                          \
+  if (j >= ns) return;                                                      \
+  const long long fb = c.fo;                                                \
+  const int rb = c.ro;                                                      \
+  (void)nr; (void)rb; (void)s;
+
+#define FGX_RING_PROLOGUE                                                   \
+  const RingTile t = D.rtiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
+  if (t.d < 0) return;                                                      \
+  const DiskConst& c = D.dc[t.d];                                           \
+  DiskState& s = D.st[t.d];                                                 \
+  if (!D.ignore_active && !s.active) return;                                \
+  const int ns = c.nsec, nr = c.nrad;                                       \
+  const int i = t.i;                                                        \
+  const long long fb = c.fo;                                                \
+  const int rb = c.ro;                                                      \
+  (void)nr; (void)rb; (void)fb;
+
+#define AT(a, ii, jj) a[fb + (long long)(ii) * ns + (jj)]
+
+// potential of star + planets at cell (i,j): FillForcesArraysKernel
+// (Kernels.cu:1322-1349) for all planets in one evaluation
+template <class real>
+__device__ __forceinline__ real potential_at(const Dev<real>& D, const DiskConst& c, const DiskState& s,
+                                             real rmed, int j) {
+  const real x = rmed * D.cosj[c.so + j], y = rmed * D.sinj[c.so + j];
+  real pot = 0;
+  for (int k = 0; k < c.nb; ++k) {
+    const real xp = (real)s.pot_x[k], yp = (real)s.pot_y[k], mp = (real)s.pot_m[k];
+    const real dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
+    real p = -mp / sqrt(dist + (real)s.pot_sm2[k]);
+    if (D.indirect) p += mp * (real)s.pot_inv3[k] * (x * xp + y * yp);
+    pot += p;
+  }
+  pot += -(real)1 / rmed - ((real)s.ind_x * x + (real)s.ind_y * y);
+  return pot;
+}
+
+// ===================================================================== CFL
+// ConditionCFLKernel1D + ConditionCFLKernel2D1 + per-ring min of
+// ConditionCFLKernel2D2 (Kernels.cu:1363-1422): one block per ring
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
+  FGX_RING_PROLOGUE
+  if (c.substeps_forced > 0 || (D.sloppy && s.sub_in_dt > 0)) return;
+  __shared__ double redd[FGX_TPB / 64];
+  __shared__ real redr[FGX_TPB / 64];
+  const long long row = fb + (long long)i * ns;
+  double acc = 0;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)D.vtheta[row + j];
+  acc = block_sum(acc, redd);
+  const real vmoy = (real)(acc / (double)ns);
+  if (i == 0) {
+    if (threadIdx.x == 0) D.vmed[rb + i] = vmoy;
+    return;
+  }
+  const real dxrad = D.rsup[rb + i] - D.rinf[rb + i];
+  const real dxt = D.dxtheta[rb + i];
+  const real mind = dxrad < dxt ? dxrad : dxt;
+  real mn = (real)1e30;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    const real vt = D.vtheta[row + j];
+    const real vr = D.vrad[row + j];
+    const real cs = D.adiabatic ? D.cs[row + j] : D.cs_ring[rb + i];
+    const real vres = D.fast_transport ? vt - vmoy : vt;
+    const real i1 = cs / mind;
+    const real i2 = fabs(vr) / dxrad;
+    const real i3 = fabs(vres) / dxt;
+    real dvr = D.vrad[row + ns + j] - vr;
+    real dvt = D.vtheta[row + (j + 1 == ns ? 0 : j + 1)] - vt;
+    dvr = (dvr >= 0) ? (real)1e-10 : -dvr;
+    dvt = (dvt >= 0) ? (real)1e-10 : -dvt;
+    real i4 = dvr / dxrad;
+    const real i4b = dvt / dxt;
+    i4 = i4b > i4 ? i4b : i4;
+    i4 *= (real)(4.0 * kCVNR * kCVNR);
+    const real dt2 = (real)kCFLSECURITY / sqrt(i1 * i1 + i2 * i2 + i3 * i3 + i4 * i4);
+    mn = dt2 < mn ? dt2 : mn;
+  }
+  mn = block_min(mn, redr);
+  if (threadIdx.x == 0) { D.vmed[rb + i] = vmoy; D.newdt[rb + i] = mn; }
+}
+
+// ============================================================ substep begin
+// MassTaper, the CFL count of ConditionCFL (SourceEuler.cu:494-536), dt,
+// GetPsysInfo(MARK) and the force targets of the substep: one block per disk
+template <class real>
+__device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, DiskState& s, int action);
+
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
+  const int d = blockIdx.x;
+  const DiskConst& c = D.dc[d];
+  DiskState& s = D.st[d];
+  if (!s.active) return;
+  const int nr = c.nrad, rb = c.ro;
+  __shared__ real redr[FGX_TPB / 64];
+  const bool need_cfl = !(c.substeps_forced > 0) && !(D.sloppy && s.sub_in_dt > 0);
+  real newdt = (real)1e30;
+  if (need_cfl) {
+    real mn = (real)1e30;
+    for (int i = 1 + threadIdx.x; i < nr; i += FGX_TPB) {
+      const real v = D.newdt[rb + i];
+      mn = v < mn ? v : mn;
+    }
+    for (int i = threadIdx.x; i < nr - 1; i += FGX_TPB) {
+      const real a = D.vmed[rb + i] * D.invrmed[rb + i] - D.vmed[rb + i + 1] * D.invrmed[rb + i + 1];
+      const real v = (real)(2.0 * kPI * kCFLSECURITY / (double)c.nsec / fabs((double)a));
+      mn = v < mn ? v : mn;
+    }
+    newdt = block_min(mn, redr);
+  }
+  if (threadIdx.x != 0) return;
+  // MassTaper (SourceEuler.cu:231-232)
+  double mt = s.time / (c.masstaper * 2.0 * M_PI);
+  s.masstaper = (mt > 1.0) ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0);
+  const double DT = c.DT;
+  int n;
+  if (c.substeps_forced > 0) {
+    n = D.sloppy ? c.substeps_forced : c.substeps_forced - s.sub_in_dt;
+    if (n < 1) n = 1;
+  } else if (need_cfl) {
+    const real dT = (real)(DT - s.dtemp);
+    real nd = newdt;
+    if (dT < nd) nd = dT;
+    n = (int)ceil(dT / nd);
+  } else n = s.ncfl;
+  // SloppyCFL keeps the dt of the first substep (SourceEuler.cu:224-229)
+  if (!(D.sloppy && s.sub_in_dt > 0)) s.dt = D.sloppy ? DT / (double)n : (DT - s.dtemp) / (double)n;
+  s.ncfl = n;
+  s.dtemp += s.dt;
+  if (D.corotating) get_psys_info_dev(D, c, s, 1);
+  // force targets: the star (Pframeforce.cu via SourceEuler.cu:247) and the
+  // planets that feel the disk (Pframeforce.cu:68-92)
+  int nt = 0;
+  s.tx[0] = 0; s.ty[0] = 0; s.tsm2[0] = 0; s.trh2[0] = 1e-30; s.tk[0] = 0;
+  nt = 1;
+  const double tt = s.time - s.time0;
+  for (int k = 0; k < c.nb; ++k) {
+    if (!c.feel_disk[k]) continue;
+    const double m = c.mass[k], x = s.x[k], y = s.y[k];
+    const double r = sqrt(x * x + y * y);
+    const double sm = D.roche ? r * pow(m / 3., 1. / 3.) * c.rochesmoothing
+                              : c.thicknesssmoothing * aspect_ratio_dev(c, r, tt) * pow(r, 1.0 + c.flaringindex);
+    const double rh = pow(m / 3., 1. / 3.) * r + 1e-15;
+    s.tx[nt] = x; s.ty[nt] = y; s.tsm2[nt] = sm * sm; s.trh2[nt] = rh * rh;
+    s.tk[nt] = D.exclude_hill ? 10 : 0;
+    nt++;
+  }
+  s.ntarget = nt;
+  // ApplySubKeplerianBoundary values (SideEuler.cu:250-268), FP64
+  const double r0 = D.rmed[rb], rn = D.rmed[rb + nr - 1];
+  const double fac = 1.0 + c.sigmaslope - 2.0 * c.flaringindex;
+  const double h0 = aspect_ratio_dev(c, r0, tt), hn = aspect_ratio_dev(c, rn, tt);
+  s.vkep_in = 1.0 / r0 * (1.0 - fac * h0 * h0 * pow(r0, 2.0 * c.flaringindex));
+  s.vkep_out = 1.0 / rn * (1.0 - fac * hn * hn * pow(rn, 2.0 * c.flaringindex));
+  // EvanescentBoundary constants (SideEuler.cu:227-233)
+  s.ev_tin = 2.0 * kPI * pow(r0, 1.5);
+  s.ev_tout = 2.0 * kPI * pow(rn, 1.5);
+  s.ev_drmin = r0 * 1.25;
+  s.ev_drmax = rn * 0.84;
+}
+
+// ================================================================== forces
+// ComputeForceKernel (Kernels.cu:804-839) for the substep's targets, summed
+// per ring in FP64 (k = 0 or k = 10 Hill cut): one block per ring
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_force_ring(Dev<real> D) {
+  FGX_RING_PROLOGUE
+  __shared__ double redd[FGX_TPB / 64];
+  const long long row = fb + (long long)i * ns;
+  const real rmed = D.rmed[rb + i], surf = D.surf[rb + i];
+  double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
+  const int nt = s.ntarget;
+  for (int q = 0; q < nt; ++q) {
+    const real tx = (real)s.tx[q], ty = (real)s.ty[q], sm2 = (real)s.tsm2[q], rh2 = (real)s.trh2[q];
+    const int kk = s.tk[q];
+    double fx = 0, fy = 0;
+    for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+      const real cm = surf * D.dens[row + j];
+      const real dx = rmed * D.cosj[c.so + j] - tx, dy = rmed * D.sinj[c.so + j] - ty;
+      const real d2 = dx * dx + dy * dy;
+      const real dist2 = d2 + sm2;
+      const real inv3 = (real)1 / dist2 / sqrt(dist2);
+      const real cut = kk ? (real)1 - exp(-d2 / rh2) : (real)1;
+      fx += (double)(cm * dx * inv3 * cut);
+      fy += (double)(cm * dy * inv3 * cut);
+    }
+    fx = block_sum(fx, redd);
+    fy = block_sum(fy, redd);
+    if (threadIdx.x == 0) { out[2 * q] = fx; out[2 * q + 1] = fy; }
+  }
+}
+
+// ================================================================= planets
+template <class real>
+__device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, DiskState& s, int action) {
+  // GetPsysInfo (Psys.cu:162-232)
+  double x = s.x[0], y = s.y[0], vx = s.vx[0], vy = s.vy[0];
+  double xc = x, yc = y;
+  const double m = c.mass[0] + 1.;
+  const double h = x * vy - y * vx;
+  const double dd = sqrt(x * x + y * y);
+  const double Ax = x * vy * vy - y * vx * vy - m * x / dd;
+  const double Ay = y * vx * vx - x * vx * vy - m * y / dd;
+  const double e = sqrt(Ax * Ax + Ay * Ay) / m;
+  const double a = h * h / m / (1. - e * e);
+  const double arg = (e == 0.0) ? 1.0 : (1.0 - dd / a) / e;
+  double E;
+  if (fabs(arg) >= 1.0) E = kPI * (1. - arg / fabs(arg)) / 2.;
+  else E = acos((1.0 - dd / a) / e);
+  if ((x * y * (vy * vy - vx * vx) + vx * vy * (x * x - y * y)) < 0) E = -E;
+  const double M = E - e * sin(E);
+  const double omega = sqrt(m / a / a / a);
+  const double pa = atan2(Ay, Ax);
+  if (D.guiding) { xc = a * cos(M + pa); yc = a * sin(M + pa); }
+  if (e < 1e-8) { xc = x; yc = y; }
+  if (action == 1) { s.mark_x = xc; s.mark_y = yc; return 0.; }
+  if (action == 0) {
+    const double d2 = sqrt(xc * xc + yc * yc);
+    const double d1 = sqrt(s.mark_x * s.mark_x + s.mark_y * s.mark_y);
+    const double cross = s.mark_x * yc - xc * s.mark_y;
+    s.mark_x = xc; s.mark_y = yc;
+    return asin(cross / (d1 * d2));
+  }
+  return omega;
+}
+
+// DerivMotionRK5 (RungeKutta.cu:9-46)
+template <class real>
+__device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const double* q, double* der, int nb, double dt) {
+  const double *x = q, *y = q + nb, *vx = q + 2 * nb, *vy = q + 3 * nb;
+  double dist[FGX_MAX_PLANETS];
+  for (int i = 0; i < nb; ++i) dist[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
+  for (int i = 0; i < nb; ++i) {
+    der[i] = vx[i];
+    der[i + nb] = vy[i];
+    double ax = -1.0 / dist[i] / dist[i] / dist[i] * x[i];
+    double ay = -1.0 / dist[i] / dist[i] / dist[i] * y[i];
+    for (int j = 0; j < nb; ++j) {
+      if (D.indirect) {
+        ax -= c.mass[j] / dist[j] / dist[j] / dist[j] * x[j];
+        ay -= c.mass[j] / dist[j] / dist[j] / dist[j] * y[j];
+      }
+      if (j != i && c.feel_others[i]) {
+        double d3;
+        if (D.fix_pp) {
+          const double dd = sqrt((x[i] - x[j]) * (x[i] - x[j]) + (y[i] - y[j]) * (y[i] - y[j]));
+          d3 = dd * dd * dd;
+        } else d3 = dist[j] * dist[j] * dist[j];
+        ax += c.mass[j] / d3 * (x[j] - x[i]);
+        ay += c.mass[j] / d3 * (y[j] - y[i]);
+      }
+    }
+    der[i + 2 * nb] = ax;
+    der[i + 3 * nb] = ay;
+  }
+  for (int i = 0; i < 4 * nb; ++i) der[i] *= dt;
+}
+
+// reduction of the ring partial forces, FillForcesArrays' potential
+// parameters, AdvanceSystemFromDisk, AdvanceSystemRK5 (Cash-Karp,
+// RungeKutta.cu:55-74), corotation and RotatePsys: one block per disk
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_planets(Dev<real> D) {
+  const int d = blockIdx.x;
+  const DiskConst& c = D.dc[d];
+  DiskState& s = D.st[d];
+  if (!s.active) return;
+  __shared__ double redd[FGX_TPB / 64];
+  __shared__ double tot[2 * FGX_NFORCE_T];
+  const int nt = s.ntarget;
+  for (int q = 0; q < 2 * nt; ++q) {
+    double a = 0;
+    for (int i = threadIdx.x; i < c.nrad; i += FGX_TPB) a += D.fpart[(size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T) + q];
+    a = block_sum(a, redd);
+    if (threadIdx.x == 0) tot[q] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double dt = s.dt;
+  const double tt = s.time - s.time0;
+  // ComputeIndirectTerm (Pframeforce.cu:27-35)
+  s.ind_x = D.indirect ? -tot[0] : 0.0;
+  s.ind_y = D.indirect ? -tot[1] : 0.0;
+  const int nb = c.nb;
+  // FillForcesArrays parameters (Pframeforce.cu:49-61) at the current positions
+  for (int k = 0; k < nb; ++k) {
+    const double xp = s.x[k], yp = s.y[k], mp = c.mass[k] * s.masstaper;
+    const double pd = sqrt(xp * xp + yp * yp);
+    const double rroche = pd * pow((1.0 / 3.0 * mp), 1.0 / 3.0);
+    const double sm = D.roche ? rroche * c.rochesmoothing
+                              : c.thicknesssmoothing * aspect_ratio_dev(c, pd, tt) * pow(pd, 1.0 + c.flaringindex);
+    s.pot_x[k] = xp; s.pot_y[k] = yp; s.pot_m[k] = mp;
+    s.pot_inv3[k] = 1.0 / pd / pd / pd;
+    s.pot_sm2[k] = sm * sm;
+  }
+  // AdvanceSystemFromDisk (Pframeforce.cu:68-92)
+  int q = 1;
+  for (int k = 0; k < nb; ++k) {
+    if (!c.feel_disk[k]) continue;
+    s.vx[k] += dt * tot[2 * q]; s.vy[k] += dt * tot[2 * q + 1];
+    s.vx[k] += dt * s.ind_x; s.vy[k] += dt * s.ind_y;
+    q++;
+  }
+  // AdvanceSystemRK5 (Pframeforce.cu:94-157)
+  if (nb > 0) {
+    double q0[4 * FGX_MAX_PLANETS], q1[4 * FGX_MAX_PLANETS];
+    if (!D.forced_circular) {
+      double k1[4 * FGX_MAX_PLANETS], k2[4 * FGX_MAX_PLANETS], k3[4 * FGX_MAX_PLANETS],
+          k4[4 * FGX_MAX_PLANETS], k5[4 * FGX_MAX_PLANETS], k6[4 * FGX_MAX_PLANETS];
+      for (int k = 0; k < nb; ++k) {
+        q0[k] = s.x[k]; q0[k + nb] = s.y[k]; q0[k + 2 * nb] = s.vx[k]; q0[k + 3 * nb] = s.vy[k];
+      }
+      const int n4 = 4 * nb;
+      deriv_rk5_dev(D, c, q0, k1, nb, dt);
+      for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.2 * k1[m];
+      deriv_rk5_dev(D, c, q1, k2, nb, dt);
+      for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.075 * k1[m] + 0.225 * k2[m];
+      deriv_rk5_dev(D, c, q1, k3, nb, dt);
+      for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.3 * k1[m] - 0.9 * k2[m] + 1.2 * k3[m];
+      deriv_rk5_dev(D, c, q1, k4, nb, dt);
+      for (int m = 0; m < n4; ++m)
+        q1[m] = q0[m] - 11.0 / 54.0 * k1[m] + 2.5 * k2[m] - 70.0 / 27.0 * k3[m] + 35.0 / 27.0 * k4[m];
+      deriv_rk5_dev(D, c, q1, k5, nb, dt);
+      for (int m = 0; m < n4; ++m)
+        q1[m] = q0[m] + 1631.0 / 55296.0 * k1[m] + 175.0 / 512.0 * k2[m] + 575.0 / 13824.0 * k3[m] +
+                44275.0 / 110592.0 * k4[m] + 253.0 / 4096.0 * k5[m];
+      deriv_rk5_dev(D, c, q1, k6, nb, dt);
+      for (int m = 0; m < n4; ++m)
+        q1[m] = q0[m] + 37.0 / 378.0 * k1[m] + 250.0 / 621.0 * k3[m] + 125.0 / 594.0 * k4[m] + 512.0 / 1771.0 * k6[m];
+    }
+    for (int k = 1 - (s.time >= c.releasedate ? 1 : 0); k < nb; ++k) {
+      if (!D.forced_circular) {
+        s.x[k] = q1[k]; s.y[k] = q1[k + nb]; s.vx[k] = q1[k + 2 * nb]; s.vy[k] = q1[k + 3 * nb];
+      } else {
+        const double x = s.x[k], y = s.y[k], th = atan2(y, x), vx = s.vx[k], vy = s.vy[k];
+        const double r = sqrt(x * x + y * y);
+        const double om = (-y * vx + x * vy) / r / r, dth = om * dt;
+        s.x[k] = r * cos(th + dth); s.y[k] = r * sin(th + dth);
+        s.vx[k] = vx * cos(dth + th) - vy * sin(dth + th);
+        s.vy[k] = vx * sin(dth + th) + vy * cos(dth + th);
+      }
+    }
+    if (s.time < c.releasedate) {
+      const double x = s.x[0], y = s.y[0], r = sqrt(x * x + y * y), th = atan2(y, x);
+      const double rdot = (c.releaseradius - r) / (c.releasedate - s.time);
+      const double om = sqrt((1. + c.mass[0]) / r / r / r);
+      const double nrr = r + rdot * dt, den = r - nrr;
+      const double dth = (den != 0.0) ? 2. * dt * r * om / den * (sqrt(r / nrr) - 1.) : om * dt;
+      const double vx = rdot, vy = nrr * sqrt((1. + c.mass[0]) / nrr / nrr / nrr);
+      s.x[0] = nrr * cos(dth + th); s.y[0] = nrr * sin(dth + th);
+      s.vx[0] = vx * cos(dth + th) - vy * sin(dth + th);
+      s.vy[0] = vx * sin(dth + th) + vx * cos(dth + th);  // as Pframeforce.cu:154
+    }
+  }
+  // corotating frame (SourceEuler.cu:257-263)
+  s.domega = 0;
+  if (D.corotating && nb > 0) {
+    const double onew = get_psys_info_dev(D, c, s, 0) / dt;
+    s.domega = onew - s.omega_frame;
+    s.omega_frame = onew;
+  }
+  // RotatePsys (Psys.cu:234-251), by -OmegaFrame*dt
+  const double ang = s.omega_frame * dt;
+  const double sa = sin(ang), ca = cos(ang);
+  for (int k = 0; k < nb; ++k) {
+    double xt = s.x[k], yt = s.y[k];
+    s.x[k] = xt * ca + yt * sa; s.y[k] = -xt * sa + yt * ca;
+    xt = s.vx[k]; yt = s.vy[k];
+    s.vx[k] = xt * ca + yt * sa; s.vy[k] = -xt * sa + yt * ca;
+  }
+}
+
+// CorrectVthetaKernel (Kernels.cu:1352-1359)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_correct_vtheta(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  AT(D.vtheta, i, j) -= (real)s.domega * D.rmed[rb + i];
+}
+
+// ============================================================ viscosity nu
+// FViscosity (Viscosity.cu:19-37) per ring; with update_csmean the ring mean
+// of the sound speed is refreshed first (Make1Dprofile(1), Kernels.cu:740-756)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_visc_profile(Dev<real> D, int update_csmean, int alpha) {
+  FGX_RING_PROLOGUE
+  __shared__ double redd[FGX_TPB / 64];
+  if (update_csmean) {
+    double acc = 0;
+    if (D.adiabatic) {
+      for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)AT(D.cs, i, j);
+      acc = block_sum(acc, redd) / (double)ns;
+    } else acc = (double)D.cs_ring[rb + i];
+    if (threadIdx.x == 0) D.csmean[rb + i] = (real)acc;
+  }
+  if (threadIdx.x == 0) {
+    const double nu = fviscosity_dev(c, alpha, (double)D.rmed[rb + i], (double)D.csmean[rb + i], s.time - s.time0);
+    D.visc[rb + i] = (real)nu;
+  }
+}
+
+// ============================================================ sound speed
+// ComputeSoundSpeedKernel, adiabatic branch (Kernels.cu:190-191)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_sound_speed(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const real g = (real)c.adiabaticindex;
+  AT(D.cs, i, j) = sqrt(g * (g - (real)1) * AT(D.energy, i, j) / AT(D.dens, i, j));
+}
+
+// ================================================================ Substep1
+// Substep1KernelVrad + Substep1KernelVtheta (Kernels.cu:52-88), pressure
+// (ComputePressureFieldKernel, Kernels.cu:196-208) and potential evaluated in
+// place instead of stored
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const real dt = (real)s.dt;
+  const real om = (real)s.omega_frame;
+  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+  const real rmed = D.rmed[rb + i];
+  const real g1 = (real)c.adiabaticindex - (real)1;
+  auto press = [&](int ii, int jj) -> real {
+    if (D.adiabatic) return g1 * AT(D.energy, ii, jj);
+    const real cs = D.cs_ring[rb + ii];
+    return AT(D.dens, ii, jj) * cs * cs;
+  };
+  const real sig = AT(D.dens, i, j);
+  const real p = press(i, j);
+  const real phi = potential_at(D, c, s, rmed, j);
+  if (D.write_pressure) AT(D.pressure, i, j) = p;
+  // radial
+  if (i > 0) {
+    const real rm1 = D.rmed[rb + i - 1];
+    const real sigm = AT(D.dens, i - 1, j);
+    const real pm = press(i - 1, j);
+    const real phim = potential_at(D, c, s, rm1, j);
+    const real idr = D.invdiffrmed[rb + i];
+    const real gradp = (p - pm) * (real)2 / (sig + sigm) * idr;
+    const real gradphi = (phi - phim) * idr;
+    real vt2 = AT(D.vtheta, i, j) + AT(D.vtheta, i, jp) + AT(D.vtheta, i - 1, j) + AT(D.vtheta, i - 1, jp);
+    vt2 = vt2 / (real)4 + D.rinf[rb + i] * om;
+    vt2 = vt2 * vt2;
+    AT(D.vradint, i, j) = AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * D.invrinf[rb + i]);
+  } else {
+    AT(D.vradint, i, j) = 0;
+  }
+  // azimuthal
+  const real invdxt = (real)1 / D.dxtheta[rb + i];
+  const real sigl = AT(D.dens, i, jm);
+  const real pl = press(i, jm);
+  const real phil = potential_at(D, c, s, rmed, jm);
+  const real gradp = (p - pl) * (real)2 / (sig + sigl) * invdxt;
+  const real gradphi = (phi - phil) * invdxt;
+  real v = AT(D.vtheta, i, j) - dt * (gradp + gradphi);
+  v += dt * D.supp_torque[rb + i];
+  AT(D.vthetaint, i, j) = v;
+}
+
+// ============================================================ viscous terms
+// ViscousTermsKernelDRP + ViscousTermsKernelTAURP (Kernels.cu:843-880): the
+// strain rates stay in registers, only the three stresses (and the velocity
+// divergence when an energy equation needs it) are stored
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* __restrict__ vr,
+                                                      const real* __restrict__ vt, int write_divv) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+  const real invdphi = (real)((double)ns / (2.0 * kPI));
+  const real irm = D.invrmed[rb + i];
+  const real vr0 = AT(vr, i, j), vr1 = AT(vr, i + 1, j);
+  const real vt0 = AT(vt, i, j), vtp = AT(vt, i, jp);
+  const real drr = (vr1 - vr0) * D.invdiffrsup[rb + i];
+  const real dpp = (vtp - vt0) * invdphi * irm + (real)0.5 * (vr1 + vr0) * irm;
+  real divv = (vr1 * D.rsup[rb + i] - vr0 * D.rinf[rb + i]) * D.invdiffrsup[rb + i] * irm;
+  divv += (vtp - vt0) * invdphi * irm;
+  const real nu = D.visc[rb + i];
+  const real sig = AT(D.dens, i, j);
+  const real third = (real)(1.0 / 3.0);
+  AT(D.taurr, i, j) = (real)2 * sig * nu * (drr - third * divv);
+  AT(D.taupp, i, j) = (real)2 * sig * nu * (dpp - third * divv);
+  if (write_divv) AT(D.divv, i, j) = divv;
+  if (i > 0) {
+    const real drp = (real)0.5 * (D.rinf[rb + i] * (vt0 * irm - AT(vt, i - 1, j) * D.invrmed[rb + i - 1]) * D.invdiffrmed[rb + i] +
+                                  (vr0 - AT(vr, i, jm)) * invdphi * D.invrinf[rb + i]);
+    // corner density: column 0 of ring i as the reference indexes it
+    // (Kernels.cu:876), or column j-1 with the quirk fix
+    const real sc = D.fix_taurp ? AT(D.dens, i, jm) : AT(D.dens, i, 0);
+    AT(D.taurp, i, j) = (real)(2.0 * 0.25) * (sig + AT(D.dens, i - 1, j) + sc + AT(D.dens, i - 1, jm)) * nu * drp;
+  } else {
+    AT(D.taurp, i, j) = 0;
+  }
+}
+
+// UpdateVelocitiesKernel (Kernels.cu:142-160) + ApplySubKeplerianBoundary
+// (SideEuler.cu:250-272, Kernels.cu:1490-1503)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const real dt = (real)s.dt;
+  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+  const real invdphi = (real)((double)ns / (2.0 * kPI));
+  if (D.viscous && i > 0 && i < nr - 1) {
+    const real trp0 = AT(D.taurp, i, j), trp1 = AT(D.taurp, i + 1, j);
+    AT(D.vthetaint, i, j) +=
+        dt * D.invrmed[rb + i] *
+        ((D.rsup[rb + i] * trp1 - D.rinf[rb + i] * trp0) * D.invdiffrsup[rb + i] +
+         (AT(D.taupp, i, j) - AT(D.taupp, i, jm)) * invdphi + (real)0.5 * (trp0 + trp1)) /
+        ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i, jm)));
+  }
+  if (D.viscous && i > 0) {
+    AT(D.vradint, i, j) +=
+        dt * D.invrinf[rb + i] *
+        ((D.rmed[rb + i] * AT(D.taurr, i, j) - D.rmed[rb + i - 1] * AT(D.taurr, i - 1, j)) * D.invdiffrmed[rb + i] +
+         (AT(D.taurp, i, jp) - AT(D.taurp, i, j)) * invdphi - (real)0.5 * (AT(D.taupp, i, j) + AT(D.taupp, i - 1, j))) /
+        ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
+  }
+  if (!D.evanescent) {
+    if (i == 0) AT(D.vthetaint, i, j) = (real)(sqrt(s.vkep_in) - (double)D.rmed[rb] * s.omega_frame);
+    if (i == nr - 1) AT(D.vthetaint, i, j) = (real)(sqrt(s.vkep_out) - (double)D.rmed[rb + nr - 1] * s.omega_frame);
+  }
+}
+
+// ================================================================ Substep2
+// Substep2Kernel + Substep2Kernel2 (Kernels.cu:570-621): the artificial
+// pressures are recomputed where needed instead of stored; writes the new
+// velocities straight into the state (ActualiseGasVrad/Vtheta become no-ops)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_substep2(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const real dt = (real)s.dt;
+  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+  const real cv2 = (real)(kCVNR * kCVNR);
+  const real sig = AT(D.dens, i, j);
+  const real vri = AT(D.vradint, i, j), vrp = AT(D.vradint, i + 1, j);
+  real dv = vrp - vri;
+  const real qr = dv < 0 ? cv2 * sig * dv * dv : (real)0;
+  const real vti = AT(D.vthetaint, i, j), vtp = AT(D.vthetaint, i, jp), vtm = AT(D.vthetaint, i, jm);
+  dv = vtp - vti;
+  const real qt = dv < 0 ? cv2 * sig * dv * dv : (real)0;
+  const real sigl = AT(D.dens, i, jm);
+  dv = vti - vtm;
+  const real qtl = dv < 0 ? cv2 * sigl * dv * dv : (real)0;
+  const real invdxt = (real)1 / D.dxtheta[rb + i];
+  if (i > 0) {
+    const real sigm = AT(D.dens, i - 1, j);
+    dv = vri - AT(D.vradint, i - 1, j);
+    const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
+    AT(D.vrad, i, j) = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * D.invdiffrmed[rb + i];
+  } else {
+    AT(D.vrad, i, j) = 0;
+  }
+  AT(D.vtheta, i, j) = vti - dt * (real)2 / (sig + sigl) * (qt - qtl) * invdxt;
+  if (D.adiabatic)
+    AT(D.energyint, i, j) = AT(D.energy, i, j) - dt * qr * (vrp - vri) * D.invdiffrsup[rb + i] -
+                            dt * qt * (vtp - vti) * invdxt;
+}
+
+// ================================================================ Substep3
+// Substep3Kernel + Substep3Kernel2 (Kernels.cu:91-139)
+template <class real>
+__device__ __forceinline__ real qplus_at(const Dev<real>& D, const DiskConst& c, int rb, long long fb, int ns, int i, int j) {
+  const real nu = D.visc[rb + i];
+  if (nu == (real)0) return 0;
+  const real sig = AT(D.dens, i, j);
+  const real trr = AT(D.taurr, i, j), trp = AT(D.taurp, i, j), tpp = AT(D.taupp, i, j), dv = AT(D.divv, i, j);
+  real q = (real)0.5 / nu / sig * (trr * trr + trp * trp + tpp * tpp);
+  q += (real)(2.0 / 9.0) * nu * sig * dv * dv;
+  return q;
+}
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const real dt = (real)s.dt;
+  real qp;
+  if (i > 0) qp = qplus_at(D, c, rb, fb, ns, i, j);
+  else if (D.visc[rb + nr - 1] != (real)0) {
+    const real q1 = qplus_at(D, c, rb, fb, ns, 1, j), q2 = qplus_at(D, c, rb, fb, ns, 2, j);
+    qp = q1 * exp(log(q1 / q2) * log(D.rmed[rb] / D.rmed[rb + 1]) / log(D.rmed[rb + 1] / D.rmed[rb + 2]));
+  } else qp = 0;
+  AT(D.qplus, i, j) = qp;
+  const real g1 = (real)c.adiabaticindex - (real)1;
+  const real eint = AT(D.energyint, i, j), dvv = AT(D.divv, i, j);
+  real en;
+  if (!D.cooling) {
+    en = (dt * qp + eint) / ((real)1 + g1 * dt * dvv);
+  } else {
+    const real sig = AT(D.dens, i, j);
+    const real tc = D.coolingtime[rb + i], sm = D.sigmamed[rb + i];
+    const real num = D.energymed[rb + i] * dt * sig / sm + tc * eint + dt * tc * (qp - D.qplusmed[rb + i] * sig / sm);
+    const real den = dt + tc + g1 * dt * tc * dvv;
+    en = num / den;
+  }
+  AT(D.energy, i, j) = en;
+}
+
+// ============================================================== boundaries
+// OpenBoundaryKernel (Kernels.cu:624-637): one block per disk
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_open_boundary(Dev<real> D) {
+  const int d = blockIdx.x;
+  const DiskConst& c = D.dc[d];
+  if (!D.st[d].active) return;
+  const int ns = c.nsec, rb = c.ro;
+  const long long fb = c.fo;
+  const real sm0 = D.sigmamed[rb];
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    AT(D.dens, 0, j) = AT(D.dens, 1, j);
+    if (D.adiabatic) AT(D.energy, 0, j) = AT(D.energy, 1, j);
+    if (AT(D.vrad, 2, j) > 0 || AT(D.dens, 1, j) < sm0) AT(D.vrad, 1, j) = 0;
+    else AT(D.vrad, 1, j) = AT(D.vrad, 2, j);
+  }
+}
+
+// NonReflectingBoundary (SideEuler.cu:172-222) with ReduceCs, ReduceMean,
+// NonReflectingBoundaryKernel and MinusMeanKernel (Kernels.cu:640-723): one
+// block per disk
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
+  const int d = blockIdx.x;
+  const DiskConst& c = D.dc[d];
+  DiskState& s = D.st[d];
+  if (!s.active) return;
+  const int ns = c.nsec, nr = c.nrad, rb = c.ro;
+  const long long fb = c.fo;
+  __shared__ double redd[FGX_TPB / 64];
+  __shared__ int ia_sh[2];
+  auto csv = [&](int ii, int jj) -> real { return D.adiabatic ? AT(D.cs, ii, jj) : D.cs_ring[rb + ii]; };
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    a0 += (double)csv(0, j); a1 += (double)csv(1, j); a2 += (double)csv(nr - 1, j); a3 += (double)csv(nr - 2, j);
+  }
+  a0 = block_sum(a0, redd) / ns; a1 = block_sum(a1, redd) / ns;
+  a2 = block_sum(a2, redd) / ns; a3 = block_sum(a3, redd) / ns;
+  if (threadIdx.x == 0) {
+    double da = (pow((double)D.rinf[rb + 1], -1.5) - 1.0) / (.5 * (a0 + a1));
+    da *= ((double)D.rmed[rb + 1] - (double)D.rmed[rb]);
+    ia_sh[0] = (int)(da / 2.0 / kPI * (double)ns + .5);
+    double da2 = (pow((double)D.rinf[rb + nr - 2], -1.5) - 1.0) / (.5 * (a2 + a3));
+    da2 *= ((double)D.rmed[rb + nr - 1] - (double)D.rmed[rb + nr - 2]);
+    ia_sh[1] = (int)(da2 / 2.0 / kPI * (double)ns + .5);
+    s.nr_iangle = ia_sh[0]; s.nr_iangle2 = ia_sh[1];
+  }
+  __syncthreads();
+  const int ia = ia_sh[0], ia2 = ia_sh[1];
+  const real sm1 = D.sigmamed[rb + 1], sm2 = D.sigmamed[rb + nr - 2];
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    int jp = j + ia;
+    if (jp >= ns) jp -= ns;
+    if (jp < 0) jp += ns;
+    AT(D.dens, 0, jp) = AT(D.dens, 1, j);
+    if (D.adiabatic) AT(D.energy, 0, jp) = AT(D.energy, 1, j);
+    const real vm = -csv(1, j) * (AT(D.dens, 1, j) - sm1) / sm1;
+    AT(D.vrad, 1, j) = (real)2 * vm - AT(D.vrad, 2, j);
+    const int io = nr - 1;
+    int jo = j - ia2;
+    if (jo >= ns) jo -= ns;
+    if (jo < 0) jo += ns;
+    AT(D.dens, io, j) = AT(D.dens, io - 1, jo);
+    if (D.adiabatic) AT(D.energy, io, j) = AT(D.energy, io - 1, jo);
+    const real vm2 = csv(io, j) * (AT(D.dens, io - 1, j) - sm2) / sm2;
+    AT(D.vrad, io, j) = (real)2 * vm2 - AT(D.vrad, io - 1, j);
+  }
+  __syncthreads();
+  double m0 = 0, m1 = 0, e0 = 0, e1 = 0;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    m0 += (double)AT(D.dens, 0, j); m1 += (double)AT(D.dens, nr - 1, j);
+    if (D.adiabatic) { e0 += (double)AT(D.energy, 0, j); e1 += (double)AT(D.energy, nr - 1, j); }
+  }
+  const real md = (real)(block_sum(m0, redd) / ns), md2 = (real)(block_sum(m1, redd) / ns);
+  const real me = (real)(block_sum(e0, redd) / ns), me2 = (real)(block_sum(e1, redd) / ns);
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    AT(D.dens, 0, j) += D.sigmamed[rb] - md;
+    AT(D.dens, nr - 1, j) += D.sigmamed[rb + nr - 1] - md2;
+    if (D.adiabatic) {
+      AT(D.energy, 0, j) += D.energymed[rb] - me;
+      AT(D.energy, nr - 1, j) += D.energymed[rb + nr - 1] - me2;
+    }
+  }
+}
+
+// EvanescentBoundaryKernel (Kernels.cu:1076-1124)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const double r = D.rmed[rb + i];
+  if (!(r < s.ev_drmin || r > s.ev_drmax)) return;
+  const double dt = s.dt;
+  double lambda = 0, damping;
+  if (r < s.ev_drmin) {
+    damping = (r - s.ev_drmin) / ((double)D.rmed[rb] - s.ev_drmin);
+    lambda = damping * damping * 10.0 * dt / s.ev_tin;
+  }
+  if (r > s.ev_drmax) {
+    damping = (r - s.ev_drmax) / ((double)D.rmed[rb + nr - 1] - s.ev_drmax);
+    lambda = damping * damping * 10.0 * dt / s.ev_tout;
+  }
+  const double h = aspect_ratio_dev(c, r, s.time - s.time0);
+  double vt0 = sqrt(1.0 / r * (1.0 - (1.0 + c.sigmaslope - 2.0 * c.flaringindex) * h * h * pow(r, 2.0 * c.flaringindex)));
+  vt0 -= r * s.omega_frame;
+  const double vr0 = -3.0 * (double)D.visc[rb + i] / r * (-c.sigmaslope + .5);
+  const real l = (real)lambda, inv = (real)(1.0 / (1.0 + lambda));
+  AT(D.vrad, i, j) = (AT(D.vrad, i, j) + l * (real)vr0) * inv;
+  AT(D.vtheta, i, j) = (AT(D.vtheta, i, j) + l * (real)vt0) * inv;
+  AT(D.dens, i, j) = (AT(D.dens, i, j) + l * D.sigmamed[rb + i]) * inv;
+  if (D.adiabatic) AT(D.energy, i, j) = (AT(D.energy, i, j) + l * D.energymed[rb + i]) * inv;
+}
+
+// ========================================================= radial transport
+// LRMomentaKernel + StarRadKernel + StarRadKernel2 + DivisePolarGridKernel +
+// VanLeerRadialKernel (Kernels.cu:883-947, 1127-1154) for all quantities of
+// OneWindRad (TransportEuler.cu:54-73) in one sweep: each cell reads rings
+// i-2..i+2 and writes the radially advected density, the four momenta and
+// the energy
+template <class real>
+__device__ __forceinline__ real rad_slope(const real* w, int k, int base, int nr, const real* invdiffrmed) {
+  if (k <= 0 || k >= nr - 1) return 0;
+  const real dqm = (w[k - base] - w[k - 1 - base]) * invdiffrmed[k];
+  const real dqp = (w[k + 1 - base] - w[k - base]) * invdiffrmed[k + 1];
+  return (dqp * dqm > 0) ? (real)2 * dqp * dqm / (dqp + dqm) : (real)0;
+}
+// interface value at ring m (1 <= m <= nr-1) for velocity v
+template <class real>
+__device__ __forceinline__ real rad_star(const real* w, int m, int base, int nr, real v, real dt, const real* rmed,
+                                         const real* invdiffrmed) {
+  if (v > 0) return w[m - 1 - base] + (rmed[m] - rmed[m - 1] - v * dt) * (real)0.5 * rad_slope(w, m - 1, base, nr, invdiffrmed);
+  return w[m - base] - (rmed[m + 1] - rmed[m] + v * dt) * (real)0.5 * rad_slope(w, m, base, nr, invdiffrmed);
+}
+
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  const real dt = (real)s.dt;
+  const real om = (real)s.omega_frame;
+  const int jp = (j + 1 == ns) ? 0 : j + 1;
+  const int base = i - 2;
+  const real* rmed = D.rmed + rb;
+  const real* idr = D.invdiffrmed + rb;
+  // rings i-2..i+2 (clamped loads; clamped values are never used)
+  real sg[5], vrr[6], rmp[5], rmm[5], tmp[5], tmm[5], en[5];
+  for (int q = 0; q < 6; ++q) {
+    int k = base + q;
+    k = k < 0 ? 0 : (k > nr ? nr : k);
+    vrr[q] = AT(D.vrad, k, j);
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    int k = base + q;
+    k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
+    const real sgk = AT(D.dens, k, j);
+    sg[q] = sgk;
+    const real rk = rmed[k];
+    rmp[q] = sgk * vrr[q + 1 <= 5 ? q + 1 : 5];
+    rmm[q] = sgk * vrr[q];
+    tmp[q] = sgk * (AT(D.vtheta, k, jp) + rk * om) * rk;
+    tmm[q] = sgk * (AT(D.vtheta, k, j) + rk * om) * rk;
+    en[q] = D.adiabatic ? AT(D.energy, k, j) : (real)0;
+  }
+  // work arrays: quantity / (density + 1e-20)
+  real w_rmp[5], w_rmm[5], w_tmp[5], w_tmm[5], w_e[5], w_sg[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const real den = sg[q] + (real)1e-20;
+    w_rmp[q] = rmp[q] / den; w_rmm[q] = rmm[q] / den; w_tmp[q] = tmp[q] / den;
+    w_tmm[q] = tmm[q] / den; w_e[q] = en[q] / den; w_sg[q] = sg[q] / den;
+  }
+  const real v0 = vrr[2], v1 = vrr[3];   // vr(i), vr(i+1)
+  // density interface values (ComputeStarRad on Dens)
+  const real ds0 = (i == 0) ? (real)0 : rad_star(sg, i, base, nr, v0, dt, rmed, idr);
+  const real ds1 = (i + 1 == nr) ? (real)0 : rad_star(sg, i + 1, base, nr, v1, dt, rmed, idr);
+  const real dth = (real)(2.0 * kPI / (double)ns);
+  const real a0 = dt * dth * D.rinf[rb + i], a1 = dt * dth * D.rsup[rb + i];
+  const real isurf = D.invsurf[rb + i];
+  auto advect = [&](const real* w, real q0) -> real {
+    const real qs0 = (i == 0) ? (real)0 : rad_star(w, i, base, nr, v0, dt, rmed, idr);
+    const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w, i + 1, base, nr, v1, dt, rmed, idr);
+    real varq = a0 * qs0 * ds0 * v0;
+    varq -= a1 * qs1 * ds1 * v1;
+    return q0 + varq * isurf;
+  };
+  AT(D.t_rmp, i, j) = advect(w_rmp, rmp[2]);
+  AT(D.t_rmm, i, j) = advect(w_rmm, rmm[2]);
+  AT(D.t_tmp, i, j) = advect(w_tmp, tmp[2]);
+  AT(D.t_tmm, i, j) = advect(w_tmm, tmm[2]);
+  if (D.adiabatic) AT(D.t_e, i, j) = advect(w_e, en[2]);
+  {
+    const real qs0 = (i == 0) ? (real)0 : rad_star(w_sg, i, base, nr, v0, dt, rmed, idr);
+    const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w_sg, i + 1, base, nr, v1, dt, rmed, idr);
+    real varq = a0 * qs0 * ds0 * v0;
+    varq -= a1 * qs1 * ds1 * v1;
+    AT(D.t_dens, i, j) = sg[2] + varq * isurf;
+    if (i == 0 && D.open_inner) D.lost[c.so + j] = varq;
+  }
+}
+
+// ====================================================== azimuthal transport
+// OneWindTheta (TransportEuler.cu:171-188): ComputeAverageThetaVelocities,
+// ComputeResiduals, ComputeConstantResidual, both QuantitiesAdvection passes
+// (StarTheta, DivisePolarGrid, VanLeerTheta) and AdvectSHIFT, plus the
+// azimuthal half of ComputeVelocities (Kernels.cu:1177-1306).  One block per
+// ring; the ring lives in LDS through both passes and the integer shift is a
+// rotation of the LDS array on the way out.
+template <class real>
+__device__ __forceinline__ real th_slope(const real* w, int j, int ns, real invdxt) {
+  const int jm = (j == 0) ? ns - 1 : j - 1, jp = (j + 1 == ns) ? 0 : j + 1;
+  const real dqm = w[j] - w[jm], dqp = w[jp] - w[j];
+  return (dqp * dqm > 0) ? dqp * dqm / (dqp + dqm) * invdxt : (real)0;
+}
+template <class real>
+__device__ __forceinline__ real th_star(const real* w, int j, int ns, real v, real dt, real dxt, real invdxt) {
+  const real ksi = v * dt;
+  if (ksi > 0) {
+    const int jm = (j == 0) ? ns - 1 : j - 1;
+    return w[jm] + (dxt - ksi) * th_slope(w, jm, ns, invdxt);
+  }
+  return w[j] - (dxt + ksi) * th_slope(w, j, ns, invdxt);
+}
+
+// fargo integer shift with the reference's promotion order in FP32
+// (ComputeConstantResidualKernel, Kernels.cu:1216-1221)
+template <class real>
+__device__ __forceinline__ int fargo_shift_dev(real vmed, real invrmed, double dt, int ns, real& ntilde, real& nround);
+template <>
+__device__ __forceinline__ int fargo_shift_dev<float>(float vmed, float invrmed, double dt, int ns, float& ntilde,
+                                                      float& nround) {
+  const float prod = vmed * invrmed * (float)dt * (float)ns;
+  ntilde = (float)((double)prod / 2.0 / kPI);
+  nround = (float)floor((double)ntilde + 0.5);
+  return (int)(long)nround;
+}
+template <>
+__device__ __forceinline__ int fargo_shift_dev<double>(double vmed, double invrmed, double dt, int ns, double& ntilde,
+                                                       double& nround) {
+  ntilde = vmed * invrmed * dt * (double)ns / 2.0 / kPI;
+  nround = floor(ntilde + 0.5);
+  return (int)(long)nround;
+}
+
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D) {
+  extern __shared__ unsigned char fgx_smem[];
+  FGX_RING_PROLOGUE
+  if (i >= nr) return;
+  const int MS = D.max_nsec;
+  real* sD1 = reinterpret_cast<real*>(fgx_smem);  // density after the radial pass
+  real* sS1 = sD1 + MS;                            // DensStar, pass 1
+  real* sD2 = sD1 + 2 * MS;                        // density after pass 1
+  real* sS2 = sD1 + 3 * MS;                        // DensStar, pass 2
+  real* sV = sD1 + 4 * MS;                         // residual velocity
+  real* sQ = sD1 + 5 * MS;                         // quantity in flight
+  real* sA = sD1 + 6 * MS;                         // work = quantity / density
+  real* sH = sD1 + 7 * MS;                         // held ThetaMomP
+  __shared__ double redd[FGX_TPB / 64];
+  const long long row = fb + (long long)i * ns;
+  const double dtd = s.dt;
+  const real dt = (real)dtd;
+  // ring mean of vtheta (ComputeAverageThetaVelocitiesKernel), FP64 sum
+  double acc = 0;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)D.vtheta[row + j];
+  acc = block_sum(acc, redd);
+  const real vmed = (real)(acc / (double)ns);
+  const real rmed = D.rmed[rb + i], irm = D.invrmed[rb + i];
+  real ntilde, nround;
+  const int nsh = fargo_shift_dev<real>(vmed, irm, dtd, ns, ntilde, nround);
+  const real invdt = (real)1 / dt;
+  const real dpinvns = (real)(2.0 * kPI / (double)ns);
+  const real vconst = (ntilde - nround) * rmed * invdt * dpinvns;
+  const real v2 = D.fast_transport ? vconst : (real)0;
+  if (threadIdx.x == 0) { D.vmed[rb + i] = vmed; D.nshift[rb + i] = nsh; }
+  const real dxt = D.dxtheta[rb + i];
+  const real invdxt = (real)1 / dxt;
+  const real dxrad = (D.rsup[rb + i] - D.rinf[rb + i]) * dt;
+  const real invsurf = (real)1 / D.surf[rb + i];
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    sD1[j] = D.t_dens[row + j];
+    real v = D.vtheta[row + j] - vmed;
+    if (!D.fast_transport) v = vconst + v;
+    sV[j] = v;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sS1[j] = th_star(sD1, j, ns, sV[j], dt, dxt, invdxt);
+  // density, pass 1 -> sD2
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = sD1[j] / (sD1[j] + (real)1e-20);
+  __syncthreads();
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    const int jp = (j + 1 == ns) ? 0 : j + 1;
+    real varq = dxrad * th_star(sA, j, ns, sV[j], dt, dxt, invdxt) * sS1[j] * sV[j];
+    varq -= dxrad * th_star(sA, jp, ns, sV[jp], dt, dxt, invdxt) * sS1[jp] * sV[jp];
+    sD2[j] = sD1[j] + varq * invsurf;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sS2[j] = th_star(sD2, j, ns, v2, dt, dxt, invdxt);
+  __syncthreads();
+  // one pass of VanLeerTheta on sQ with divisor Dv, DensStar S, velocity v
+  auto pass = [&](const real* Dv, const real* S, const real* varr, real vu) {
+    for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = sQ[j] / (Dv[j] + (real)1e-20);
+    __syncthreads();
+    for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+      const int jp = (j + 1 == ns) ? 0 : j + 1;
+      const real vj = varr ? varr[j] : vu, vjp = varr ? varr[jp] : vu;
+      real varq = dxrad * th_star(sA, j, ns, vj, dt, dxt, invdxt) * S[j] * vj;
+      varq -= dxrad * th_star(sA, jp, ns, vjp, dt, dxt, invdxt) * S[jp] * vjp;
+      sQ[j] += varq * invsurf;
+    }
+    __syncthreads();
+  };
+  int nsm = nsh % ns;
+  if (nsm < 0) nsm += ns;
+  // quantity order: RadMomP, RadMomM, [energy], ThetaMomP (held in sH),
+  // ThetaMomM (left in sQ); every quantity sees the same divisors and
+  // interface densities, so the order is free here
+  const int order[5] = {0, 1, 4, 2, 3};
+  for (int o = 0; o < 5; ++o) {
+    const int q = order[o];
+    if (q == 4 && !D.adiabatic) continue;
+    const real* src = q == 0 ? D.t_rmp : q == 1 ? D.t_rmm : q == 2 ? D.t_tmp : q == 3 ? D.t_tmm : D.t_e;
+    for (int j = threadIdx.x; j < ns; j += FGX_TPB) sQ[j] = src[row + j];
+    __syncthreads();
+    pass(sD1, sS1, sV, (real)0);
+    pass(sD2, sS2, nullptr, v2);
+    if (q == 2) {
+      for (int j = threadIdx.x; j < ns; j += FGX_TPB) sH[j] = sQ[j];
+      __syncthreads();
+    } else if (q != 3) {
+      // AdvectSHIFTKernel (Kernels.cu:1276-1289): out[j] = in[j - Nshift]
+      real* dst = q == 0 ? D.t_rmp : q == 1 ? D.t_rmm : D.energy;
+      for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+        int js = j - nsm;
+        js += (js < 0) ? ns : 0;
+        dst[row + j] = sQ[js];
+      }
+      __syncthreads();
+    }
+  }
+  // density, pass 2 -> sD1 (the radial-pass density is no longer needed)
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = sD2[j] / (sD2[j] + (real)1e-20);
+  __syncthreads();
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    const int jp = (j + 1 == ns) ? 0 : j + 1;
+    real varq = dxrad * th_star(sA, j, ns, v2, dt, dxt, invdxt) * sS2[j] * v2;
+    varq -= dxrad * th_star(sA, jp, ns, v2, dt, dxt, invdxt) * sS2[jp] * v2;
+    sD1[j] = sD2[j] + varq * invsurf;
+  }
+  __syncthreads();
+  // shifted density and the azimuthal velocity of ComputeVelocitiesKernel
+  // (Kernels.cu:1303-1304), evaluated in the unshifted frame and rotated
+  const real om = (real)s.omega_frame;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    int js = j - nsm;
+    js += (js < 0) ? ns : 0;
+    const int jsm = (js == 0) ? ns - 1 : js - 1;
+    D.dens[row + j] = sD1[js];
+    D.vtheta[row + j] = (sH[jsm] + sQ[js]) / (sD1[js] + sD1[jsm] + (real)1e-15) / rmed - rmed * om;
+  }
+}
+
+// radial half of ComputeVelocitiesKernel (Kernels.cu:1299-1302)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_trans_vr(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  if (i == 0) { AT(D.vrad, i, j) = 0; return; }
+  AT(D.vrad, i, j) = (AT(D.t_rmp, i - 1, j) + AT(D.t_rmm, i, j)) / (AT(D.dens, i, j) + AT(D.dens, i - 1, j) + (real)1e-20);
+}
+
+// ============================================================ bookkeeping
+// start of an AlgoGas call (SourceEuler.cu:211-216)
+template <class real>
+__global__ void k_begin_dt(Dev<real> D) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= D.ndisk) return;
+  DiskState& s = D.st[d];
+  s.dtemp = 0;
+  s.sub_in_dt = 0;
+  s.active = 1;
+}
+// end of a substep (SourceEuler.cu:297-298) and the DT loop condition
+// (SourceEuler.cu:230); LostMass accumulation (TransportEuler.cu:71)
+template <class real>
+__global__ void __launch_bounds__(64) k_end_substep(Dev<real> D) {
+  const int d = blockIdx.x;
+  const DiskConst& c = D.dc[d];
+  DiskState& s = D.st[d];
+  if (!s.active) return;
+  double lost = 0;
+  if (D.open_inner) {
+    for (int j = threadIdx.x; j < c.nsec; j += 64) lost += (double)D.lost[c.so + j];
+    lost = wave_sum(lost);
+  }
+  if (threadIdx.x != 0) return;
+  s.lost_mass += lost;
+  s.time += s.dt;
+  s.substeps++;
+  s.sub_in_dt++;
+  s.rem_est = D.sloppy ? s.ncfl - s.sub_in_dt : s.ncfl - 1;
+  if (!(s.dtemp < 0.999999999 * c.DT)) {
+    s.active = 0;
+    s.dt_steps++;
+  }
+}
+// batch summary for the host loop: active disks and the largest CFL count
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
+  __shared__ int red[FGX_TPB / 64];
+  int a = 0, m = 0;
+  for (int d = threadIdx.x; d < D.ndisk; d += FGX_TPB) {
+    a += D.st[d].active;
+    const int rem = D.st[d].active ? D.st[d].rem_est : 0;
+    m = rem > m ? rem : m;
+  }
+  a = block_sum(a, red);
+  m = -block_min(-m, red);
+  if (threadIdx.x == 0) { D.summary[0] = a; D.summary[1] = m; }
+}
+
+// ============================================================== diagnostics
+// potential of the current substep into D.work (FillForcesArrays output)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_potential_field(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  AT(D.work, i, j) = potential_at(D, c, s, D.rmed[rb + i], j);
+}
+// ComputeTemperatureFieldKernel (Kernels.cu:211-223) into D.work
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_temperature(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  if (!D.adiabatic) AT(D.work, i, j) = AT(D.pressure, i, j) / AT(D.dens, i, j);
+  else AT(D.work, i, j) = (real)(c.adiabaticindex - 1.0) * AT(D.energy, i, j) / AT(D.dens, i, j);
+}
+// isothermal pressure at set-up (ComputePressureFieldKernel at InitEuler)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_pressure_init(Dev<real> D) {
+  FGX_CELL_PROLOGUE
+  if (i >= nr) return;
+  if (!D.adiabatic) {
+    const real cs = D.cs_ring[rb + i];
+    AT(D.pressure, i, j) = AT(D.dens, i, j) * cs * cs;
+  } else {
+    AT(D.pressure, i, j) = (real)(c.adiabaticindex - 1.0) * AT(D.energy, i, j);
+  }
+}
+// ComputeForce with all eleven Hill-cut variants (Force.cu:31-67,
+// Kernels.cu:804-839) for one point, per-ring FP64 partials into rpart[44]
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_force_full(Dev<real> D, int disk, double x, double y, double sm,
+                                                        double mass) {
+  const int i = blockIdx.x;
+  const DiskConst& c = D.dc[disk];
+  if (i >= c.nrad) return;
+  const int ns = c.nsec, rb = c.ro;
+  const long long fb = c.fo;
+  __shared__ double redd[FGX_TPB / 64];
+  const double a = sqrt(x * x + y * y);
+  const double rh = pow(mass / 3., 1. / 3.) * a + 1e-15;
+  const real rmed = D.rmed[rb + i], surf = D.surf[rb + i];
+  const bool inner = (double)rmed < (double)(float)a;
+  double f[22];
+  for (int k = 0; k < 22; ++k) f[k] = 0;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    const real cm = surf * AT(D.dens, i, j);
+    const real dx = rmed * D.cosj[c.so + j] - (real)x, dy = rmed * D.sinj[c.so + j] - (real)y;
+    const real d2 = dx * dx + dy * dy;
+    const real dist2 = d2 + (real)(sm * sm);
+    const real inv3 = (real)1 / dist2 / sqrt(dist2);
+    for (int k = 0; k < 11; ++k) {
+      real cut = 1;
+      if (k) {
+        const real rk = (real)(rh * (double)k / 10.0);
+        cut = (real)1 - exp(-d2 / (rk * rk));
+      }
+      f[k] += (double)(cm * dx * inv3 * cut);
+      f[11 + k] += (double)(cm * dy * inv3 * cut);
+    }
+  }
+  double* out = D.rpart + (size_t)i * 44;
+  for (int k = 0; k < 11; ++k) {
+    const double fx = block_sum(f[k], redd), fy = block_sum(f[11 + k], redd);
+    if (threadIdx.x == 0) {
+      out[k] = inner ? fx : 0; out[11 + k] = inner ? 0 : fx;
+      out[22 + k] = inner ? fy : 0; out[33 + k] = inner ? 0 : fy;
+    }
+  }
+}
+// CircumPlanetaryMassKernel (Kernels.cu:245-261), per-ring FP64 partials
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_mdcp(Dev<real> D, int disk, double xpl, double ypl, double rhill) {
+  const int i = blockIdx.x;
+  const DiskConst& c = D.dc[disk];
+  if (i >= c.nrad) return;
+  const int ns = c.nsec, rb = c.ro;
+  const long long fb = c.fo;
+  __shared__ double redd[FGX_TPB / 64];
+  const real rmed = D.rmed[rb + i], surf = D.surf[rb + i];
+  double acc = 0;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    const real dx = rmed * D.cosj[c.so + j] - (real)xpl, dy = rmed * D.sinj[c.so + j] - (real)ypl;
+    if ((double)sqrt(dx * dx + dy * dy) < rhill) acc += (double)(surf * AT(D.dens, i, j));
+  }
+  acc = block_sum(acc, redd);
+  if (threadIdx.x == 0) D.rpart[i] = acc;
+}
+
+}  // namespace fgx

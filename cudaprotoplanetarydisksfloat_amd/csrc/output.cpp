@@ -1,0 +1,242 @@
+// output.cpp -- the reference's output files and its executable's main loop.
+//
+// File names, layouts, number formats and cadence follow src/Output.cu,
+// src/Force.cu:76-138, src/SourceEuler.cu:159-171 and src/Main.cu:87-401.
+// Values the reference keeps in float globals are rounded to float before
+// printing with %.18g, so the text matches digit for digit in form.
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fgx_internal.h"
+
+using fgx::set_error;
+
+namespace {
+
+std::string outdir(const fgx_params& P) { return std::string(P.outputdir); }
+
+int write_raw(const std::string& dir, const std::string& name, const std::vector<float>& v) {
+  struct stat st;
+  if (!dir.empty() && stat(dir.c_str(), &st) == -1) mkdir(dir.c_str(), 0700);
+  FILE* f = std::fopen(name.c_str(), "wb");
+  if (!f) return set_error(FGX_EIO, "Unable to open '%s'", name.c_str());
+  std::fwrite(v.data(), sizeof(float), v.size(), f);
+  std::fclose(f);
+  return FGX_OK;
+}
+
+// WriteDiskPolar (Output.cu:355-386) and the FARGO gas<name><N>.dat form
+int write_field(fgx_disk* d, const fgx_params& P, int field, const char* name, const char* gasname, int index) {
+  const fgx::HostDisk* h = fgx::batch_host(d, 0);
+  std::vector<float> v((size_t)h->nrad * h->nsec);
+  int r = fgx_disk_read_field(d, field, FGX_F32, v.data(), v.size());
+  if (r) return r;
+  const std::string od = outdir(P);
+  if (P.ext_gas_names == 0 || P.ext_gas_names == 2) {
+    std::string dir = od + name;
+    r = write_raw(dir, dir + "/" + name + std::to_string(index) + ".dat", v);
+    if (r) return r;
+  }
+  if (P.ext_gas_names >= 1) {
+    r = write_raw("", od + gasname + std::to_string(index) + ".dat", v);
+    if (r) return r;
+  }
+  return FGX_OK;
+}
+
+FILE* open_append(const std::string& name) { return std::fopen(name.c_str(), "a"); }
+
+}  // namespace
+
+extern "C" {
+
+int fgx_disk_write_outputs(fgx_disk* d, int index) {
+  if (!d) return set_error(FGX_EPARAM, "null handle");
+  const fgx_params& P = fgx::batch_host(d, 0)->P;
+  int r = FGX_OK;
+  // SendOutput (Output.cu:406-421)
+  if (!P.is_disk) return FGX_OK;
+  if (P.write_density && (r = write_field(d, P, FGX_DENS, "dens", "gasdens", index))) return r;
+  if (P.write_velocity) {
+    if ((r = write_field(d, P, FGX_VRAD, "vrad", "gasvrad", index))) return r;
+    if ((r = write_field(d, P, FGX_VTHETA, "vtheta", "gasvtheta", index))) return r;
+  }
+  if (P.write_energy && P.adiabatic && (r = write_field(d, P, FGX_ENERGY, "energy", "gasenergy", index))) return r;
+  if (P.write_temperature && (r = write_field(d, P, FGX_TEMPERATURE, "temperature", "gasTemperature", index)))
+    return r;
+  if (P.write_divv && P.adiabatic && (r = write_field(d, P, FGX_DIVV, "divergence", "gasdivergence", index))) return r;
+  if (P.write_qplus && P.adiabatic && (r = write_field(d, P, FGX_QPLUS, "qplus", "gasqplus", index))) return r;
+  if (P.advect_label && (r = write_field(d, P, FGX_LABEL, "label", "gaslabel", index))) return r;
+  return FGX_OK;
+}
+
+int fgx_disk_write_logs(fgx_disk* d, int timestep) {
+  if (!d) return set_error(FGX_EPARAM, "null handle");
+  const fgx::HostDisk* h = fgx::batch_host(d, 0);
+  const fgx_params& P = h->P;
+  fgx_planets sys;
+  int r = fgx_disk_planets(d, &sys);
+  if (r) return r;
+  fgx_disk_info info;
+  if ((r = fgx_disk_info_get(d, &info))) return r;
+  const std::string od = outdir(P);
+  const float lost = (float)info.lost_mass, ptime = (float)info.physical_time, om = (float)info.omega_frame;
+  const float mdcp1 = (float)info.mdcp1, exces = (float)info.exces_mdcp;
+  // WriteBigPlanetSystemFile / WriteBigPlanetFile (Output.cu:321-353)
+  for (int k = 0; k < sys.nb; ++k) {
+    std::string name = od + "bigplanet" + std::to_string(k) + ".dat";
+    FILE* f = open_append(name);
+    if (!f) return set_error(FGX_EIO, "Can't write 'bigplanet.dat' file.");
+    std::fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", timestep,
+                 (double)(float)sys.x[k], (double)(float)sys.y[k], (double)(float)sys.vx[k], (double)(float)sys.vy[k],
+                 (double)(float)sys.mass[k], (double)lost, (double)ptime, (double)om, (double)mdcp1, (double)exces);
+    std::fclose(f);
+  }
+  // UpdateLog (Force.cu:76-138)
+  for (int k = 0; k < sys.nb; ++k) {
+    const double x = sys.x[k], y = sys.y[k], vx = sys.vx[k], vy = sys.vy[k], m = sys.mass[k];
+    const double rr = std::sqrt(x * x + y * y);
+    const double sm = P.roche_smoothing ? rr * std::pow(m / 3., 1. / 3.) * P.rochesmoothing
+                                        : P.thicknesssmoothing * fgx::aspect_ratio(P, rr, info.physical_time) *
+                                              std::pow(rr, 1.0 + P.flaringindex);
+    double gf[44];
+    if ((r = fgx_disk_force(d, x, y, sm, m, gf))) return r;
+    // the Force struct holds floats (include/Types.cuh:17-29)
+    const double fxi = (float)gf[0], fxei = (float)gf[10], fxo = (float)gf[11], fxeo = (float)gf[21];
+    const double fyi = (float)gf[22], fyei = (float)gf[32], fyo = (float)gf[33], fyeo = (float)gf[43];
+    std::string name = od + "tqwk" + std::to_string(k) + ".dat";
+    FILE* f = open_append(name);
+    if (!f) return set_error(FGX_EIO, "Can't open %s", name.c_str());
+    std::fprintf(f, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", timestep,
+                 x * fyi - y * fxi, x * fyo - y * fxo, x * fyei - y * fxei, x * fyeo - y * fxeo, vx * fxi + vy * fyi,
+                 vx * fxo + vy * fyo, vx * fxei + vy * fyei, vx * fxeo + vy * fyeo, (double)ptime);
+    std::fclose(f);
+    if (!P.self_gravity) {
+      for (int kk = 0; kk < 11; ++kk) {
+        std::string n2 = od + "tqwk" + std::to_string(k) + "_" + std::to_string(kk) + ".dat";
+        FILE* g = open_append(n2);
+        if (!g) return set_error(FGX_EIO, "Can't open %s", n2.c_str());
+        // GlobalForce holds the float reductions as doubles
+        const double a = (float)gf[kk], b = (float)gf[11 + kk], c = (float)gf[22 + kk], e = (float)gf[33 + kk];
+        std::fprintf(g, "%d\t%.18g\t%.18g\t%.18g\t%.18g\t%.18g\n", timestep, x * c - y * a, x * e - y * b,
+                     vx * a + vy * c, vx * b + vy * e, (double)ptime);
+        std::fclose(g);
+      }
+    }
+  }
+  return FGX_OK;
+}
+
+int fgx_disk_write_setup_files(fgx_disk* d) {
+  if (!d) return set_error(FGX_EPARAM, "null handle");
+  const fgx::HostDisk* h = fgx::batch_host(d, 0);
+  const fgx_params& P = h->P;
+  const std::string od = outdir(P);
+  // used_rad.dat (SourceEuler.cu:159-171): FP64 radii
+  {
+    std::string name = od + "used_rad.dat";
+    FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) return set_error(FGX_EIO, "Can't write %s.", name.c_str());
+    for (int i = 0; i <= h->nrad; ++i) std::fprintf(f, "%.18g\n", h->radii[i]);
+    std::fclose(f);
+  }
+  // WriteDim (Output.cu:388-404)
+  {
+    std::string name = od + "dims.dat";
+    FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) return set_error(FGX_EIO, "Unable to open %s.", name.c_str());
+    std::fprintf(f, "%d\t%d\t\t%d\t%d\t%f\t%d\t%d\t%d\n", 0, 0, 0, 0, (double)P.rmax, P.ntot / P.ninterm, P.nrad,
+                 P.nsec);
+    std::fclose(f);
+  }
+  // EmptyPlanetSystemFile (Output.cu:263-283)
+  for (int k = 0; k < h->sys.nb; ++k) {
+    std::string name = od + "planet" + std::to_string(k) + ".dat";
+    FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) return set_error(FGX_EIO, "Can't write %s file.", name.c_str());
+    std::fclose(f);
+  }
+  return FGX_OK;
+}
+
+static int usage(const char* exe) {
+  std::printf("Usage : %s [-abcdeimnptvz] [-(0-9)] [-s number] [-f scaling] parameters file\n", exe);
+  return 1;
+}
+
+// main() of the reference (Main.cu:87-401) over the C ABI.  FGX_PRECISION=64
+// in the environment selects the FP64 fields.
+int fgx_run_main(int argc, char** argv) {
+  if (argc == 1) return usage(argv[0]);
+  std::string parfile;
+  int disable = 0, sloppy = 0, cb = 0, store = 0, restart = 0, override_out = 0;
+  float scaling = 1.0f;
+  std::string newout;
+  for (int i = 1; i < argc; ++i) {
+    if (argv[i][0] == '-') {
+      if (std::strspn(argv[i], "-secndovtpfamzib0123456789") != std::strlen(argv[i])) return usage(argv[0]);
+      if (std::strchr(argv[i], 'n')) disable = 1;
+      if (std::strchr(argv[i], 'c')) sloppy = 1;
+      if (std::strchr(argv[i], 'b')) cb = 1;
+      if (std::strchr(argv[i], 'i')) store = 1;
+      if (std::strchr(argv[i], 's')) {
+        restart = 1;
+        if (++i >= argc) return usage(argv[0]);
+        if (std::atoi(argv[i]) < 0) return usage(argv[0]);
+      }
+      if (std::strchr(argv[i], 'o')) {
+        override_out = 1;
+        if (++i >= argc) return usage(argv[0]);
+        newout = argv[i];
+      } else if (std::strchr(argv[i], 'f')) {
+        if (++i >= argc) return usage(argv[0]);
+        scaling = (float)std::atof(argv[i]);
+        std::printf("Scaling factor = %g\n", scaling);
+        if (scaling <= 0) return usage(argv[0]);
+      }
+    } else parfile = argv[i];
+  }
+  if (store && !restart) {
+    std::printf("You cannot use tabulated surface density\nor surface internal energy in a non-restart run.\nAborted\n");
+    return 0;
+  }
+  if (parfile.empty()) return usage(argv[0]);
+  fgx_params P;
+  int r = fgx_params_load(parfile.c_str(), &P);
+  if (r) { std::fprintf(stderr, "%s\n", fgx_last_error()); return 1; }
+  if (override_out) std::snprintf(P.outputdir, sizeof P.outputdir, "%s", newout.c_str());
+  P.sloppy_cfl = sloppy; P.centrifugal_balance = cb; P.scaling_factor = scaling;
+  if (disable) return 0;
+  fgx_planets pl;
+  if ((r = fgx_planets_load(P.planetconfig, &P, &pl))) { std::fprintf(stderr, "%s\n", fgx_last_error()); return 1; }
+  std::printf("%d planet(s) found.\n", pl.nb);
+  const char* prec = std::getenv("FGX_PRECISION");
+  const int precision = (prec && std::strcmp(prec, "64") == 0) ? FGX_F64 : FGX_F32;
+  fgx_disk* d = nullptr;
+  if ((r = fgx_disk_create(&P, &pl, precision, &d))) { std::fprintf(stderr, "%s\n", fgx_last_error()); return 1; }
+  if ((r = fgx_disk_write_setup_files(d))) { std::fprintf(stderr, "%s\n", fgx_last_error()); fgx_disk_destroy(d); return 1; }
+  auto t0 = std::chrono::steady_clock::now();
+  int timestep = 0;
+  for (int i = 0; i <= P.ntot; ++i) {
+    if ((r = fgx_disk_write_logs(d, timestep))) break;
+    if (P.ninterm * (timestep = (i / P.ninterm)) == i) {
+      std::printf("\n*** OUTPUT %d ***\n", timestep);
+      if ((r = fgx_disk_write_outputs(d, timestep))) break;
+    }
+    if ((r = fgx_disk_step(d, 1))) break;
+  }
+  if (r) { std::fprintf(stderr, "%s\n", fgx_last_error()); fgx_disk_destroy(d); return 1; }
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::printf("%g\n", secs);
+  fgx_disk_destroy(d);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+/*
+ * fgx.h -- C ABI of the MI355X-native FARGO-style polar-grid disk engine.
+ *
+ * The hot path is the reference's per-DT hydro step AlgoGas
+ * (src/SourceEuler.cu:208-301) with everything it calls.  The reference
+ * exposes it as process-global state driven by main() (src/Main.cu:87-401);
+ * this header replaces that surface with handle-based entry points.  Each
+ * entry point below names the reference function(s) it stands in for.
+ *
+ * Conventions
+ *   - Every function returns FGX_OK (0) or a negative FGX_E* code; the text of
+ *     the last error on the calling thread is available from fgx_last_error().
+ *     Nothing here calls exit(): the reference's gpuErrchk/exit paths
+ *     (include/Kernels.cuh:5-13, src/Interpret.cu:173-197) become codes.
+ *   - Plain C types only.  Field buffers are host memory; the library copies.
+ *   - A field of a disk holds (nrad+1)*nsec values, ring-major (index i*nsec+j,
+ *     i = ring, j = sector); ring nrad is the reference's ghost ring
+ *     (src/Main.cu:199).
+ *   - A handle is not re-entrant; different handles may be driven from
+ *     different threads.
+ */
+#ifndef FGX_H
+#define FGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGX_VERSION 100
+
+/* ---- status codes ---------------------------------------------------- */
+#define FGX_OK          0
+#define FGX_EPARAM     -1   /* invalid parameter / parse error            */
+#define FGX_EDEVICE    -2   /* HIP runtime error                          */
+#define FGX_ENUMERIC   -3   /* non-finite or negative state detected       */
+#define FGX_EIO        -4   /* file could not be read or written          */
+#define FGX_ENOMEM     -5   /* allocation failed / over the memory budget */
+#define FGX_ERANGE     -6   /* index or size out of range                 */
+
+/* ---- arithmetic of the fields ------------------------------------------ */
+#define FGX_F32 0           /* FP32 fields (the reference's precision)     */
+#define FGX_F64 1           /* FP64 fields (verification mode)             */
+
+#define FGX_MAX_PLANETS 8
+
+/* ---- run parameters: the reference's .par keys and derived flags ---------
+ * Keys and defaults follow include/Param.cuh:9-61; derived flags follow
+ * src/Interpret.cu:126-202.  Floating keys are stored as float, as the
+ * reference stores them.  Fields after `ext_` are keys the reference does not
+ * know (it ignores unknown keys, src/Interpret.cu:35-95), so .par files that
+ * use them still run under the reference. */
+typedef struct fgx_params {
+  int   nrad, nsec, ninterm, ntot;
+  float dt, sigma0, masstaper, rmin, rmax, thicknesssmoothing, rochesmoothing,
+        aspectratio, viscosity, alphaviscosity, sigmaslope, releaseradius,
+        releasedate, omegaframe, imposeddiskdrift, flaringindex, eccentricity,
+        cavityradius, cavityratio, cavitywidth, transitionradius,
+        transitionratio, transitionwidth, lambdadoubling, adiabaticindex,
+        coolingtime0;
+  char  outputdir[512], planetconfig[512], innerboundary[64], labeladvection[64],
+        transport[64], radialspacing[64], disk[64], frame[64],
+        outersourcemass[64], writedensity[64], writevelocity[64],
+        writeenergy[64], writetemperature[64], writedivv[64], writeqplus[64],
+        indirectterm[64], excludehill[64], selfgravity[64], cicplanet[64],
+        forcedcircular[64], zmplus[64], adiabatic_s[64], cooling_s[64];
+  /* derived flags (YES=1 / NO=0), src/Interpret.cu:23-31,126-202 */
+  int   log_grid, fast_transport, open_inner, nonreflecting, evanescent,
+        is_disk, corotating, guiding_center, indirect_term, self_gravity,
+        sg_zero_mode, zmplus_f, adiabatic, cooling, exclude_hill, cic_planet,
+        forced_circular, viscosity_alpha, roche_smoothing, advect_label,
+        outer_source_mass, write_density, write_velocity, write_energy,
+        write_temperature, write_divv, write_qplus;
+  /* command-line switches of src/Main.cu:108-181 that affect the step */
+  int   sloppy_cfl;            /* -c */
+  int   centrifugal_balance;   /* -b */
+  float scaling_factor;        /* -f */
+  /* ext_: keys unknown to the reference ------------------------------- */
+  double ext_noise_amplitude;  /* NOISEAMPLITUDE: Sigma *= 1 + A(2u-1)       */
+  uint64_t ext_noise_seed;     /* NOISESEED (default 20261203)              */
+  int   ext_disk_id;           /* DISKID: added to the seed                 */
+  int   ext_substeps;          /* SUBSTEPS: >0 forces that many equal        */
+                               /* substeps per DT instead of ConditionCFL    */
+  int   ext_fix_taurp_corner;  /* QUIRKFIXTAURP: corner density uses j-1     */
+  int   ext_fix_pp_distance;   /* QUIRKFIXPPDIST: planet-planet |ri-rj|^3    */
+  int   ext_fix_sg_wrap;       /* QUIRKFIXSGWRAP: periodic j-1 in SG update  */
+  float ext_sg_softening;      /* SGSOFTENING: >0 overrides SG epsilon/H     */
+  int   ext_gas_names;         /* GASNAMES: 0 dens/ layout, 1 gasdens*.dat, 2 both */
+} fgx_params;
+
+/* ---- planetary system: the reference's PlanetarySystem (include/Types.cuh:2-15) */
+typedef struct fgx_planets {
+  int    nb;
+  double mass[FGX_MAX_PLANETS], x[FGX_MAX_PLANETS], y[FGX_MAX_PLANETS],
+         vx[FGX_MAX_PLANETS], vy[FGX_MAX_PLANETS], acc[FGX_MAX_PLANETS];
+  int    feel_disk[FGX_MAX_PLANETS], feel_others[FGX_MAX_PLANETS];
+  char   name[FGX_MAX_PLANETS][64];
+} fgx_planets;
+
+/* ---- fields and ring profiles readable from a disk ---------------------- */
+enum fgx_field {
+  FGX_DENS = 0, FGX_VRAD = 1, FGX_VTHETA = 2, FGX_ENERGY = 3,
+  FGX_TEMPERATURE = 4, FGX_SOUNDSPEED = 5, FGX_DIVV = 6, FGX_QPLUS = 7,
+  FGX_LABEL = 8, FGX_POTENTIAL = 9, FGX_NFIELDS = 10
+};
+enum fgx_ring {
+  FGX_R_RADII = 0, FGX_R_RMED = 1, FGX_R_SURF = 2, FGX_R_SIGMAMED = 3,
+  FGX_R_ENERGYMED = 4, FGX_R_VISCOSITY = 5, FGX_R_VMED = 6, FGX_R_NSHIFT = 7,
+  FGX_R_CS = 8, FGX_R_NEWDT = 9, FGX_NRINGS = 10
+};
+
+/* per-disk scalar state (host copy) */
+typedef struct fgx_disk_info {
+  int    nrad, nsec, precision;
+  double physical_time;        /* PhysicalTime (kept in FP64, see quirk Q14) */
+  double omega_frame;          /* OmegaFrame                                  */
+  double lost_mass;            /* LostMass (src/TransportEuler.cu:17,71)      */
+  double mdcp1, exces_mdcp;    /* src/SourceEuler.cu:294-295                  */
+  double last_dt;              /* dt of the last substep                      */
+  int    last_ncfl;            /* last gastimestepcfl                         */
+  long long substeps;          /* substeps completed since creation           */
+  long long dt_steps;          /* AlgoGas calls completed since creation      */
+  int    health;               /* 0 ok, else FGX_ENUMERIC flags               */
+} fgx_disk_info;
+
+typedef struct fgx_batch fgx_batch;
+typedef struct fgx_batch fgx_disk;   /* a disk is a batch of one */
+
+/* ---- library --------------------------------------------------------- */
+int         fgx_version(void);
+const char* fgx_last_error(void);
+/* sizeof(fgx_params), sizeof(fgx_planets): lets bindings check layouts */
+size_t      fgx_sizeof_params(void);
+size_t      fgx_sizeof_planets(void);
+int         fgx_device_count(int* n);
+
+/* ---- parameters (src/Interpret.cu, include/Param.cuh) ------------------- */
+/* defaults of include/Param.cuh:9-61 and src/Interpret.cu:23-31 */
+int fgx_params_default(fgx_params* p);
+/* ReadVariables (src/Interpret.cu:97-208) from a file / from text; the
+ * derived-flag pass and its validity checks are included */
+int fgx_params_load(const char* par_path, fgx_params* p);
+int fgx_params_parse(const char* text, fgx_params* p);
+/* SearchVariable (src/Interpret.cu:35-95) for one key followed by the
+ * derived-flag pass; key is case-insensitive, value is raw */
+int fgx_params_set(fgx_params* p, const char* key, const char* value);
+/* value of one key as text ("%.9g" for numbers) */
+int fgx_params_get(const fgx_params* p, const char* key, char* out, size_t n);
+
+/* ---- planets (src/Psys.cu:85-134) ----------------------------------------
+ * InitPlanetarySystem: CICPlanet placement needs the grid, so the params are
+ * passed; ECCENTRICITY comes from them as well. */
+int fgx_planets_load(const char* cfg_path, const fgx_params* p, fgx_planets* out);
+int fgx_planets_parse(const char* text, const fgx_params* p, fgx_planets* out);
+
+/* ---- one disk -----------------------------------------------------------
+ * fgx_disk_create = everything main() does before its loop
+ * (src/Main.cu:182-344): grid (FillPolar1DArrays, src/SourceEuler.cu:76-173),
+ * InitGasDensity/InitGasEnergy (src/Pframeforce.cu:173-192), self-gravity
+ * start-up (src/Main.cu:302-329), InitEuler (src/SourceEuler.cu:177-204).
+ * Optional seeded density noise (ext_noise_amplitude) is applied after
+ * InitGasDensity. precision is FGX_F32 or FGX_F64. */
+int fgx_disk_create(const fgx_params* p, const fgx_planets* pl, int precision,
+                    fgx_disk** out);
+/* n_dt calls of AlgoGas (src/SourceEuler.cu:208-301) */
+int fgx_disk_step(fgx_disk* d, int n_dt);
+int fgx_disk_destroy(fgx_disk* d);
+int fgx_disk_info_get(fgx_disk* d, fgx_disk_info* info);
+/* count >= nrad*nsec; with count >= (nrad+1)*nsec the ghost ring is included.
+ * dtype FGX_F32 / FGX_F64 is the host buffer's type. */
+int fgx_disk_read_field(fgx_disk* d, int field, int dtype, void* host, size_t count);
+int fgx_disk_write_field(fgx_disk* d, int field, int dtype, const void* host, size_t count);
+int fgx_disk_read_ring(fgx_disk* d, int ring, int dtype, void* host, size_t count);
+int fgx_disk_planets(fgx_disk* d, fgx_planets* out);
+int fgx_disk_set_planets(fgx_disk* d, const fgx_planets* pl);
+
+/* ---- output files (src/Output.cu, src/Force.cu:76-138) -------------------- */
+/* SendOutput (src/Output.cu:406-421) for output number `index` */
+int fgx_disk_write_outputs(fgx_disk* d, int index);
+/* WriteBigPlanetSystemFile (src/Output.cu:341-353) + UpdateLog
+ * (src/Force.cu:76-138) for time step `timestep` */
+int fgx_disk_write_logs(fgx_disk* d, int timestep);
+/* WriteDim (src/Output.cu:388-404), used_rad.dat (src/SourceEuler.cu:159-171),
+ * EmptyPlanetSystemFile (src/Output.cu:263-283) */
+int fgx_disk_write_setup_files(fgx_disk* d);
+/* disk->point force, all eleven Hill-cut variants (ComputeForce,
+ * src/Force.cu:31-67): out[44] laid out as the reference's GlobalForce
+ * (fxi[0..10], fxo[0..10], fyi[0..10], fyo[0..10]) */
+int fgx_disk_force(fgx_disk* d, double x, double y, double smoothing, double mass,
+                   double out[44]);
+
+/* ---- batches of many disks (no reference counterpart: the reference runs
+ * one disk per process, src/Main.cu:87-98).  All disks of one batch share
+ * the mode flags (adiabatic, boundary kind, transport, self-gravity) and may
+ * differ in grid size and every numeric parameter. */
+int fgx_batch_create(int n, const fgx_params* p, const fgx_planets* pl,
+                     int precision, fgx_batch** out);
+int fgx_batch_step(fgx_batch* b, int n_dt);
+int fgx_batch_size(fgx_batch* b, int* n);
+int fgx_batch_destroy(fgx_batch* b);
+int fgx_batch_info(fgx_batch* b, int disk, fgx_disk_info* info);
+int fgx_batch_read_field(fgx_batch* b, int disk, int field, int dtype, void* host,
+                         size_t count);
+int fgx_batch_write_field(fgx_batch* b, int disk, int field, int dtype,
+                          const void* host, size_t count);
+/* bytes of device memory a disk of these parameters needs */
+int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes);
+
+/* ---- profiling: average device time per kernel class ---------------------
+ * enable=1 brackets each launch with HIP events on the library's stream. */
+int fgx_profile_enable(fgx_batch* b, int enable);
+/* name of kernel class k (0..) or NULL past the end, and its totals */
+int fgx_profile_get(fgx_batch* b, int k, const char** name, double* total_ms,
+                    long long* launches, double* bytes_per_launch);
+int fgx_profile_reset(fgx_batch* b);
+
+/* ---- the reference executable: fargoGPU [flags] file.par (src/Main.cu:87-401) */
+int fgx_run_main(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FGX_H */
