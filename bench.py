@@ -1,0 +1,174 @@
+"""Benchmark: aggregate cell-updates/s of the 64-disk uneven batch.
+
+One step = one AlgoGas call (one DT, with per-disk CFL substeps) for every
+disk of the batch.  value = total cell-updates (cells x substeps, summed over
+disks) / wall time of the timed steps, inputs resident in HBM.  The same
+disks advanced one at a time through the single-disk path give `plain`.
+`roofline` is the dominant kernel's algorithmic bytes per launch over its
+average device time (HIP events on the library's stream, recorded during a
+second pass over the same steps); `cpu_baseline` times the FP64 OpenMP
+oracle on a bounded sample of the same disks.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s batched vs plain at 1e-5 rel L2; %HBM roofline; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def _cells(batch):
+    return [nr * ns for nr, ns in batch.shape]
+
+
+def _substeps(batch):
+    return [batch.info(d).substeps for d in range(batch.n)]
+
+
+def _updates(batch, before):
+    after = _substeps(batch)
+    return sum(c * (a - b) for c, a, b in zip(_cells(batch), after, before))
+
+
+def _workload(name):
+    from cudaprotoplanetarydisksfloat_amd import configs
+    if name == "batch":
+        disks = configs.batch_disks(n=64)
+        desc = {"workload": "64 isothermal disks, NRAD in {128,192,256} x NSEC in {384,512,768}, "
+                            "seeded q/h/alpha/slope, per-disk CFL, FARGO, closed boundaries",
+                "disks": 64, "seed": configs.BASE_SEED}
+    elif name == "large":
+        disks = [configs.large_disk()]
+        desc = {"workload": "512x1536 adiabatic disk, log grid, alpha 1e-3, evanescent boundaries, "
+                            "migrating planet (RK5)", "disks": 1}
+    else:
+        disks = [configs.plain_disk()]
+        desc = {"workload": "128x384 isothermal disk, one fixed-orbit planet, closed boundaries", "disks": 1}
+    loaded = [configs.load(p, c) for p, c in disks]
+    return loaded, desc
+
+
+def cpu_baseline(loaded, budget_s=20.0):
+    """FP64 OpenMP oracle on the first disks of the workload, one DT each,
+    until about budget_s of CPU work."""
+    import oracle
+    total, t0, ndisk = 0, time.perf_counter(), 0
+    for p, pl in loaded:
+        o = oracle.Oracle(p, pl, 1)
+        n = o.step(1)
+        total += n * o.nrad * o.nsec
+        ndisk += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": total / dt, "unit": "cell-updates/s", "cores": oracle.lib().orc_num_threads(),
+            "kind": "port", "sample": f"first {ndisk} disk(s) of the workload, 1 DT each, "
+                                      f"{total} cell-updates in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batch", choices=["batch", "large", "plain"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); disks shard across GPUs "
+              f"by disk, which this build does not launch", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from cudaprotoplanetarydisksfloat_amd import F32, F64, Batch
+
+    prec = F32 if args.precision == "f32" else F64
+    if args.copy_mib:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE
+        # counters calibrate the HBM-traffic counters of the workload kernels
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    loaded, desc = _workload(args.workload)
+    batch = Batch([p for p, _ in loaded], [pl for _, pl in loaded], prec)
+    batch.step(args.warmup)
+    torch.cuda.synchronize()
+    before = _substeps(batch)
+    t0 = time.perf_counter()
+    batch.step(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    updates = _updates(batch, before)
+    value = updates / wall
+
+    # second pass over as many steps with per-kernel HIP events
+    batch.profile(True)
+    before = _substeps(batch)
+    batch.step(args.steps)
+    torch.cuda.synchronize()
+    batch.profile(False)
+    prof_updates = _updates(batch, before)
+    stats = batch.profile_stats()
+    ncell = sum(_cells(batch))
+    kern = {}
+    for name, s in stats.items():
+        if s["launches"] == 0:
+            continue
+        bpc = s["bytes_per_launch"] / ncell  # algorithmic bytes per cell of one launch
+        kern[name] = {"ms": s["ms"], "launches": s["launches"], "avg_us": 1e3 * s["ms"] / s["launches"],
+                      "alg_bytes_per_launch": bpc * prof_updates / s["launches"]}
+    dom = max((k for k in kern if kern[k]["alg_bytes_per_launch"] > 0), key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    achieved = d["alg_bytes_per_launch"] / (d["avg_us"] * 1e-6) / 1e9
+    sub_ms = sum(v["ms"] for v in kern.values())
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_us": round(d["avg_us"], 2),
+                "kernels": {k: {"avg_us": round(v["avg_us"], 2), "share": round(v["ms"] / sub_ms, 3),
+                                "GBs": round(v["alg_bytes_per_launch"] / (v["avg_us"] * 1e-6) / 1e9, 1)}
+                            for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}}
+
+    plain = None
+    if not args.no_plain and len(loaded) > 1:
+        # the same disks one at a time through the single-disk path
+        solo = [Batch(p, pl, prec) for p, pl in loaded]
+        for b in solo:
+            b.step(1)
+        torch.cuda.synchronize()
+        befores = [_substeps(b) for b in solo]
+        t0 = time.perf_counter()
+        for b in solo:
+            b.step(args.steps)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t0
+        pup = sum(_updates(b, bf) for b, bf in zip(solo, befores))
+        plain = {"value": pup / pwall, "unit": "cell-updates/s", "wall_s": round(pwall, 3),
+                 "batched_over_plain": round(value / (pup / pwall), 3)}
+        for b in solo:
+            b.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(loaded)
+    line = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+            "config": dict(desc, cell_updates=updates, cells=ncell), "roofline": roofline,
+            "cpu_baseline": cpu, "plain": plain}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

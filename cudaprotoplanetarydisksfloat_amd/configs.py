@@ -1,0 +1,95 @@
+"""Workload definitions as reference-format .par / .cfg text.
+
+Each function returns (par_text, cfg_text) in the reference's parameter
+format (src/Interpret.cu:97-208, src/Psys.cu:85-134).  Keys the reference
+does not know (NOISEAMPLITUDE, NOISESEED, DISKID, SUBSTEPS) are ignored by
+it, so every file also runs under the reference.
+
+- plain_disk:    128 x 384 isothermal disk, arithmetic r in [0.4, 2.5],
+                 Sigma0 6.3662e-4 with slope 0.5, h = 0.05, nu = 1e-5, one
+                 q = 1e-3 planet on a fixed circular orbit at a = 1, closed
+                 boundaries, +-1e-4 density noise
+- large_disk:    512 x 1536 adiabatic disk (gamma 1.4), log r in [0.3, 3.0],
+                 alpha = 1e-3, flaring 0.25, Sigma ~ r^-1, migrating
+                 q = 3e-4 planet (RK5), evanescent boundaries
+- batch_disks:   64 isothermal disks with NRAD in {128,192,256}, NSEC in
+                 {384,512,768} and parameters drawn from a seed
+"""
+import numpy as np
+
+DT_ORBIT_20 = 0.314159265359   # DT of the reference's example runs (2pi/20)
+BASE_SEED = 20261203
+
+
+def _par(keys):
+    return "".join(f"{k}\t{v}\n" for k, v in keys.items())
+
+
+def plain_disk(nrad=128, nsec=384, noise=1e-4, disk_id=0, substeps=0):
+    keys = {
+        "AspectRatio": 0.05, "Sigma0": 6.3662e-4, "Viscosity": 1e-5, "SigmaSlope": 0.5,
+        "FlaringIndex": 0.0, "PlanetConfig": "in/Jup.cfg", "ThicknessSmoothing": 0.6,
+        "ForcedCircular": "YES", "Transport": "FARGO", "InnerBoundary": "RIGID", "Disk": "YES",
+        "OmegaFrame": 0.0, "Frame": "FIXED", "IndirectTerm": "YES", "Nrad": nrad, "Nsec": nsec,
+        "Rmin": 0.4, "Rmax": 2.5, "RadialSpacing": "ARITHMETIC", "Ntot": 100, "Ninterm": 20,
+        "DT": DT_ORBIT_20, "OutputDir": "out/", "Adiabatic": "NO",
+        "NoiseAmplitude": noise, "NoiseSeed": BASE_SEED, "DiskId": disk_id,
+    }
+    if substeps:
+        keys["Substeps"] = substeps
+    cfg = "Jupiter\t1.0\t0.001\t0.0\tNO\tNO\n"
+    return _par(keys), cfg
+
+
+def large_disk(nrad=512, nsec=1536, noise=1e-4, disk_id=0, substeps=0):
+    keys = {
+        "AspectRatio": 0.05, "Sigma0": 6.3662e-4, "AlphaViscosity": 1e-3, "SigmaSlope": 1.0,
+        "FlaringIndex": 0.25, "PlanetConfig": "in/planet.cfg", "ThicknessSmoothing": 0.6,
+        "ForcedCircular": "NO", "Transport": "FARGO", "InnerBoundary": "EVANESCENT",
+        "Disk": "YES", "OmegaFrame": 0.0, "Frame": "FIXED", "IndirectTerm": "YES",
+        "Nrad": nrad, "Nsec": nsec, "Rmin": 0.3, "Rmax": 3.0, "RadialSpacing": "LOGARITHM",
+        "Ntot": 2000, "Ninterm": 100, "DT": DT_ORBIT_20, "OutputDir": "out/",
+        "Adiabatic": "YES", "AdiabaticIndex": 1.4, "Cooling": "NO",
+        "NoiseAmplitude": noise, "NoiseSeed": BASE_SEED, "DiskId": disk_id,
+    }
+    if substeps:
+        keys["Substeps"] = substeps
+    cfg = "Planet\t1.0\t3e-4\t0.0\tYES\tYES\n"
+    return _par(keys), cfg
+
+
+def batch_disks(n=64, seed=BASE_SEED, noise=1e-4, substeps=0):
+    """Uneven isothermal batch: per-disk grid and physics drawn from a seed."""
+    rng = np.random.RandomState(seed % (2 ** 32))
+    out = []
+    for d in range(n):
+        nrad = int(rng.choice([128, 192, 256]))
+        nsec = int(rng.choice([384, 512, 768]))
+        q = float(10 ** rng.uniform(np.log10(1e-5), np.log10(3e-3)))
+        h = float(rng.uniform(0.03, 0.08))
+        alpha = float(10 ** rng.uniform(-4, -2))
+        slope = float(rng.uniform(-1.5, 0.0))
+        keys = {
+            "AspectRatio": f"{h:.6f}", "Sigma0": 6.3662e-4, "AlphaViscosity": f"{alpha:.6e}",
+            "SigmaSlope": f"{-slope:.6f}", "FlaringIndex": 0.0, "PlanetConfig": "in/planet.cfg",
+            "ThicknessSmoothing": 0.6, "ForcedCircular": "YES", "Transport": "FARGO",
+            "InnerBoundary": "RIGID", "Disk": "YES", "OmegaFrame": 0.0, "Frame": "FIXED",
+            "IndirectTerm": "YES", "Nrad": nrad, "Nsec": nsec, "Rmin": 0.4, "Rmax": 2.5,
+            "RadialSpacing": "ARITHMETIC", "Ntot": 500, "Ninterm": 100, "DT": DT_ORBIT_20,
+            "OutputDir": "out/", "Adiabatic": "NO", "NoiseAmplitude": noise,
+            "NoiseSeed": seed, "DiskId": d,
+        }
+        if substeps:
+            keys["Substeps"] = substeps
+        cfg = f"Planet\t1.0\t{q:.6e}\t0.0\tNO\tNO\n"
+        out.append((_par(keys), cfg))
+    return out
+
+
+def load(par_text, cfg_text, **overrides):
+    """Params and Planets of one workload (with optional key overrides)."""
+    from .fgx import Params, planets_from_text
+    p = Params(text=par_text)
+    for k, v in overrides.items():
+        p.set(k, v)
+    return p, planets_from_text(cfg_text, p)

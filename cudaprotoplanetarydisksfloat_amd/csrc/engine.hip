@@ -64,6 +64,7 @@ struct Batch : BatchBase {
   int* h_summary = nullptr;  // pinned
   int visc_dynamic = 0, alpha = 0, need_force = 1;
   size_t theta_shmem = 0;
+  std::vector<std::vector<double>> vref_h;  // reference vtheta profile per disk (nrad+1)
   // profiling
   struct Prof { std::string name; double ms = 0; long long n = 0; double bytes = 0; };
   bool prof_on = false;
@@ -225,6 +226,14 @@ struct Batch : BatchBase {
                       &D.qplusmed, &D.visc, &D.cs_ring, &D.csmean, &D.vmed, &D.newdt};
     for (real** r : rings) LCHK(dalloc(r, R));
     LCHK(dalloc(&D.nshift, R));
+    {
+      double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d;
+      LCHK(dalloc(&vref, R)); LCHK(dalloc(&gstar, R)); LCHK(dalloc(&rinf_d, R)); LCHK(dalloc(&invrinf_d, R));
+      LCHK(dalloc(&rmed_d, R)); LCHK(dalloc(&invdiffrmed_d, R)); LCHK(dalloc(&D.vmoy, R));
+      LCHK(dalloc(&cosj_d, S)); LCHK(dalloc(&sinj_d, S));
+      D.vref = vref; D.gstar = gstar; D.rinf_d = rinf_d; D.invrinf_d = invrinf_d; D.rmed_d = rmed_d;
+      D.invdiffrmed_d = invdiffrmed_d; D.cosj_d = cosj_d; D.sinj_d = sinj_d;
+    }
     LCHK(dalloc(&D.cosj, S)); LCHK(dalloc(&D.sinj, S)); LCHK(dalloc(&D.lost, S));
     DiskConst* dcd; DiskState* std_;
     LCHK(dalloc(&dcd, nd)); LCHK(dalloc(&std_, nd));
@@ -269,7 +278,10 @@ struct Batch : BatchBase {
     for (int d = 0; d < nd; ++d) {
       LCHK(upload_field(d, D.dens, hd[d].dens));
       LCHK(upload_field(d, D.vrad, hd[d].vrad));
-      LCHK(upload_field(d, D.vtheta, hd[d].vtheta));
+      std::vector<double> res(hd[d].vtheta);
+      for (int i = 0; i < hd[d].nrad; ++i)
+        for (int j = 0; j < hd[d].nsec; ++j) res[(size_t)i * hd[d].nsec + j] -= vref_h[d][i];
+      LCHK(upload_field(d, D.vtheta, res));
       if (ad) LCHK(upload_field(d, D.energy, hd[d].energy));
     }
     // initial pressure (InitEuler's ComputePressureField), used by the
@@ -305,11 +317,27 @@ struct Batch : BatchBase {
   int upload_field(int d, real* base, const std::vector<double>& v) {
     return upload_vec(base + dch[d].fo, v, v.size());
   }
+  int upload_d(const double* dst, const std::vector<double>& v) {
+    HIPCHK(hipMemcpy((void*)dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    return FGX_OK;
+  }
   int upload_rings() {
+    vref_h.assign(nd, {});
     for (int d = 0; d < nd; ++d) {
       const HostDisk& h = hd[d];
       const int o = dch[d].ro;
       const size_t n = (size_t)h.nrad + 1;
+      // FP64 tables: reference vtheta (the initial, ring-uniform profile),
+      // star gravity between ring centres, geometry
+      std::vector<double> vref(n, 0.0), gstar(n, 0.0), invrinf(n, 0.0);
+      for (int i = 0; i < h.nrad; ++i) vref[i] = h.vtheta[(size_t)i * h.nsec];
+      for (int i = 1; i < h.nrad; ++i) gstar[i] = (1.0 / h.rmed[i - 1] - 1.0 / h.rmed[i]) / (h.rmed[i] - h.rmed[i - 1]);
+      for (int i = 0; i < h.nrad; ++i) invrinf[i] = 1.0 / h.rinf[i];
+      vref_h[d] = vref;
+      LCHK(upload_d(D.vref + o, vref)); LCHK(upload_d(D.gstar + o, gstar));
+      LCHK(upload_d(D.rinf_d + o, h.rinf)); LCHK(upload_d(D.invrinf_d + o, invrinf));
+      LCHK(upload_d(D.rmed_d + o, h.rmed)); LCHK(upload_d(D.invdiffrmed_d + o, h.invdiffrmed));
+      LCHK(upload_d(D.cosj_d + dch[d].so, h.cosj)); LCHK(upload_d(D.sinj_d + dch[d].so, h.sinj));
       LCHK(upload_vec(D.rinf + o, h.rinf, n)); LCHK(upload_vec(D.rsup + o, h.rsup, n));
       LCHK(upload_vec(D.rmed + o, h.rmed, n)); LCHK(upload_vec(D.surf + o, h.surf, n));
       LCHK(upload_vec(D.invsurf + o, h.invsurf, n)); LCHK(upload_vec(D.invrmed + o, h.invrmed, n));
@@ -442,6 +470,14 @@ struct Batch : BatchBase {
       HIPCHK(hipMemcpyAsync(tmp.data(), src + dch[d].fo, nfull * sizeof(real), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
+    if (field == FGX_VTHETA) {
+      // vtheta is stored as a residual of the reference profile
+      for (size_t k = 0; k < n; ++k) {
+        const double v = (double)tmp[k] + vref_h[d][k / h.nsec];
+        if (dtype == FGX_F32) ((float*)host)[k] = (float)v; else ((double*)host)[k] = v;
+      }
+      return FGX_OK;
+    }
     if (dtype == FGX_F32) for (size_t k = 0; k < n; ++k) ((float*)host)[k] = (float)tmp[k];
     else for (size_t k = 0; k < n; ++k) ((double*)host)[k] = (double)tmp[k];
     return FGX_OK;
@@ -456,8 +492,11 @@ struct Batch : BatchBase {
       return set_error(FGX_ERANGE, "field %d is not writable", field);
     const size_t n = std::min(count, nfull);
     std::vector<real> tmp(n);
-    if (dtype == FGX_F32) for (size_t k = 0; k < n; ++k) tmp[k] = (real)((const float*)host)[k];
-    else for (size_t k = 0; k < n; ++k) tmp[k] = (real)((const double*)host)[k];
+    for (size_t k = 0; k < n; ++k) {
+      double v = dtype == FGX_F32 ? (double)((const float*)host)[k] : ((const double*)host)[k];
+      if (field == FGX_VTHETA) v -= vref_h[d][k / h.nsec];
+      tmp[k] = (real)v;
+    }
     HIPCHK(hipMemcpyAsync(dst + dch[d].fo, tmp.data(), n * sizeof(real), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
     if (field == FGX_DENS || field == FGX_ENERGY) {
@@ -486,6 +525,7 @@ struct Batch : BatchBase {
       case FGX_R_VMED: src = D.vmed; break;
       case FGX_R_CS: src = D.cs_ring; break;
       case FGX_R_NEWDT: src = D.newdt; break;
+      case FGX_R_INVRMED: src = D.invrmed; break;
       case FGX_R_NSHIFT: {
         std::vector<int> tmp(n);
         HIPCHK(hipMemcpyAsync(tmp.data(), D.nshift + dch[d].ro, n * sizeof(int), hipMemcpyDeviceToHost, stream));

@@ -33,6 +33,12 @@ struct Dev {
   int* nshift;
   // sector tables, nsec per disk
   real *cosj, *sinj, *lost;
+  // FP64 ring/sector tables for the terms whose FP32 rounding would dominate:
+  // the reference azimuthal velocity profile (vtheta and the angular momenta
+  // are stored as FP32 residuals from it), the star's radial gravity between
+  // ring centres, and the cell geometry of the potential
+  const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d;
+  double* vmoy;  // ring-mean vtheta of the CFL condition
   const DiskConst* dc;
   DiskState* st;
   double* fpart;       // per ring tile: 2*FGX_NFORCE_T force partial sums
@@ -143,21 +149,23 @@ __device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, 
 
 #define AT(a, ii, jj) a[fb + (long long)(ii) * ns + (jj)]
 
-// potential of star + planets at cell (i,j): FillForcesArraysKernel
-// (Kernels.cu:1322-1349) for all planets in one evaluation
+// potential of the planets and the indirect term at cell (i,j) in FP64:
+// FillForcesArraysKernel (Kernels.cu:1322-1349) for all planets in one
+// evaluation; the star's -1/r is handled per ring (its radial difference is
+// the FP64 table gstar, its azimuthal difference vanishes)
 template <class real>
-__device__ __forceinline__ real potential_at(const Dev<real>& D, const DiskConst& c, const DiskState& s,
-                                             real rmed, int j) {
-  const real x = rmed * D.cosj[c.so + j], y = rmed * D.sinj[c.so + j];
-  real pot = 0;
+__device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const DiskConst& c, const DiskState& s,
+                                                       double rmed, int j) {
+  const double x = rmed * D.cosj_d[c.so + j], y = rmed * D.sinj_d[c.so + j];
+  double pot = 0;
   for (int k = 0; k < c.nb; ++k) {
-    const real xp = (real)s.pot_x[k], yp = (real)s.pot_y[k], mp = (real)s.pot_m[k];
-    const real dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
-    real p = -mp / sqrt(dist + (real)s.pot_sm2[k]);
-    if (D.indirect) p += mp * (real)s.pot_inv3[k] * (x * xp + y * yp);
+    const double xp = s.pot_x[k], yp = s.pot_y[k], mp = s.pot_m[k];
+    const double dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
+    double p = -mp / sqrt(dist + s.pot_sm2[k]);
+    if (D.indirect) p += mp * s.pot_inv3[k] * (x * xp + y * yp);
     pot += p;
   }
-  pot += -(real)1 / rmed - ((real)s.ind_x * x + (real)s.ind_y * y);
+  pot -= s.ind_x * x + s.ind_y * y;
   return pot;
 }
 
@@ -171,28 +179,28 @@ __global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
   __shared__ double redd[FGX_TPB / 64];
   __shared__ real redr[FGX_TPB / 64];
   const long long row = fb + (long long)i * ns;
+  const double vr0 = D.vref[rb + i];
   double acc = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)D.vtheta[row + j];
   acc = block_sum(acc, redd);
-  const real vmoy = (real)(acc / (double)ns);
-  if (i == 0) {
-    if (threadIdx.x == 0) D.vmed[rb + i] = vmoy;
-    return;
-  }
+  const double mres = acc / (double)ns;   // ring mean of the residual
+  if (threadIdx.x == 0) D.vmoy[rb + i] = vr0 + mres;
+  if (i == 0) return;
   const real dxrad = D.rsup[rb + i] - D.rinf[rb + i];
   const real dxt = D.dxtheta[rb + i];
   const real mind = dxrad < dxt ? dxrad : dxt;
   real mn = (real)1e30;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    const real vt = D.vtheta[row + j];
+    const real rs = D.vtheta[row + j];
     const real vr = D.vrad[row + j];
     const real cs = D.adiabatic ? D.cs[row + j] : D.cs_ring[rb + i];
-    const real vres = D.fast_transport ? vt - vmoy : vt;
+    // Vresidual: vtheta - Vmoy (FARGO) or vtheta (standard)
+    const real vres = D.fast_transport ? (real)((double)rs - mres) : (real)(vr0 + (double)rs);
     const real i1 = cs / mind;
     const real i2 = fabs(vr) / dxrad;
     const real i3 = fabs(vres) / dxt;
     real dvr = D.vrad[row + ns + j] - vr;
-    real dvt = D.vtheta[row + (j + 1 == ns ? 0 : j + 1)] - vt;
+    real dvt = D.vtheta[row + (j + 1 == ns ? 0 : j + 1)] - rs;
     dvr = (dvr >= 0) ? (real)1e-10 : -dvr;
     dvt = (dvt >= 0) ? (real)1e-10 : -dvt;
     real i4 = dvr / dxrad;
@@ -203,7 +211,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
     mn = dt2 < mn ? dt2 : mn;
   }
   mn = block_min(mn, redr);
-  if (threadIdx.x == 0) { D.vmed[rb + i] = vmoy; D.newdt[rb + i] = mn; }
+  if (threadIdx.x == 0) D.newdt[rb + i] = mn;
 }
 
 // ============================================================ substep begin
@@ -229,8 +237,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
       mn = v < mn ? v : mn;
     }
     for (int i = threadIdx.x; i < nr - 1; i += FGX_TPB) {
-      const real a = D.vmed[rb + i] * D.invrmed[rb + i] - D.vmed[rb + i + 1] * D.invrmed[rb + i + 1];
-      const real v = (real)(2.0 * kPI * kCFLSECURITY / (double)c.nsec / fabs((double)a));
+      const double a = D.vmoy[rb + i] / D.rmed_d[rb + i] - D.vmoy[rb + i + 1] / D.rmed_d[rb + i + 1];
+      const real v = (real)(2.0 * kPI * kCFLSECURITY / (double)c.nsec / fabs(a));
       mn = v < mn ? v : mn;
     }
     newdt = block_min(mn, redr);
@@ -538,15 +546,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_sound_speed(Dev<real> D) {
 // ================================================================ Substep1
 // Substep1KernelVrad + Substep1KernelVtheta (Kernels.cu:52-88), pressure
 // (ComputePressureFieldKernel, Kernels.cu:196-208) and potential evaluated in
-// place instead of stored
+// place instead of stored.  The radial acceleration is a near-cancellation of
+// O(1) centrifugal and gravity terms, so it is formed in FP64 from the FP32
+// fields and FP64 ring tables; vthetaint is stored as a residual of vref.
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
-  const real dt = (real)s.dt;
-  const real om = (real)s.omega_frame;
+  const double dt = s.dt;
+  const double om = s.omega_frame;
   const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-  const real rmed = D.rmed[rb + i];
+  const double rmed = D.rmed_d[rb + i];
   const real g1 = (real)c.adiabaticindex - (real)1;
   auto press = [&](int ii, int jj) -> real {
     if (D.adiabatic) return g1 * AT(D.energy, ii, jj);
@@ -555,33 +565,35 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
   };
   const real sig = AT(D.dens, i, j);
   const real p = press(i, j);
-  const real phi = potential_at(D, c, s, rmed, j);
+  const double phi = potential_planets_at(D, c, s, rmed, j);
   if (D.write_pressure) AT(D.pressure, i, j) = p;
   // radial
   if (i > 0) {
-    const real rm1 = D.rmed[rb + i - 1];
+    const double rm1 = D.rmed_d[rb + i - 1];
     const real sigm = AT(D.dens, i - 1, j);
     const real pm = press(i - 1, j);
-    const real phim = potential_at(D, c, s, rm1, j);
-    const real idr = D.invdiffrmed[rb + i];
-    const real gradp = (p - pm) * (real)2 / (sig + sigm) * idr;
-    const real gradphi = (phi - phim) * idr;
-    real vt2 = AT(D.vtheta, i, j) + AT(D.vtheta, i, jp) + AT(D.vtheta, i - 1, j) + AT(D.vtheta, i - 1, jp);
-    vt2 = vt2 / (real)4 + D.rinf[rb + i] * om;
+    const double phim = potential_planets_at(D, c, s, rm1, j);
+    const double idr = D.invdiffrmed_d[rb + i];
+    const double gradp = (double)((p - pm) * (real)2 / (sig + sigm)) * idr;
+    const double gradphi = (phi - phim) * idr + D.gstar[rb + i];
+    double vt2 = 2.0 * D.vref[rb + i] + 2.0 * D.vref[rb + i - 1] +
+                 (double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) +
+                 (double)AT(D.vtheta, i - 1, j) + (double)AT(D.vtheta, i - 1, jp);
+    vt2 = vt2 / 4.0 + D.rinf_d[rb + i] * om;
     vt2 = vt2 * vt2;
-    AT(D.vradint, i, j) = AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * D.invrinf[rb + i]);
+    AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * D.invrinf_d[rb + i]));
   } else {
     AT(D.vradint, i, j) = 0;
   }
-  // azimuthal
+  // azimuthal (the star's potential is the same at j and j-1)
   const real invdxt = (real)1 / D.dxtheta[rb + i];
   const real sigl = AT(D.dens, i, jm);
   const real pl = press(i, jm);
-  const real phil = potential_at(D, c, s, rmed, jm);
+  const double phil = potential_planets_at(D, c, s, rmed, jm);
   const real gradp = (p - pl) * (real)2 / (sig + sigl) * invdxt;
-  const real gradphi = (phi - phil) * invdxt;
-  real v = AT(D.vtheta, i, j) - dt * (gradp + gradphi);
-  v += dt * D.supp_torque[rb + i];
+  const real gradphi = (real)(phi - phil) * invdxt;
+  real v = AT(D.vtheta, i, j) - (real)dt * (gradp + gradphi);
+  v += (real)dt * D.supp_torque[rb + i];
   AT(D.vthetaint, i, j) = v;
 }
 
@@ -610,7 +622,11 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
   AT(D.taupp, i, j) = (real)2 * sig * nu * (dpp - third * divv);
   if (write_divv) AT(D.divv, i, j) = divv;
   if (i > 0) {
-    const real drp = (real)0.5 * (D.rinf[rb + i] * (vt0 * irm - AT(vt, i - 1, j) * D.invrmed[rb + i - 1]) * D.invdiffrmed[rb + i] +
+    // the angular velocity difference between rings is formed in FP64 from
+    // the residuals and the reference profile
+    const double w0 = (D.vref[rb + i] + (double)vt0) / D.rmed_d[rb + i];
+    const double w1 = (D.vref[rb + i - 1] + (double)AT(vt, i - 1, j)) / D.rmed_d[rb + i - 1];
+    const real drp = (real)0.5 * ((real)(D.rinf_d[rb + i] * (w0 - w1) * D.invdiffrmed_d[rb + i]) +
                                   (vr0 - AT(vr, i, jm)) * invdphi * D.invrinf[rb + i]);
     // corner density: column 0 of ring i as the reference indexes it
     // (Kernels.cu:876), or column j-1 with the quirk fix
@@ -646,8 +662,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
         ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
   }
   if (!D.evanescent) {
-    if (i == 0) AT(D.vthetaint, i, j) = (real)(sqrt(s.vkep_in) - (double)D.rmed[rb] * s.omega_frame);
-    if (i == nr - 1) AT(D.vthetaint, i, j) = (real)(sqrt(s.vkep_out) - (double)D.rmed[rb + nr - 1] * s.omega_frame);
+    if (i == 0) AT(D.vthetaint, i, j) = (real)(sqrt(s.vkep_in) - D.rmed_d[rb] * s.omega_frame - D.vref[rb]);
+    if (i == nr - 1)
+      AT(D.vthetaint, i, j) =
+          (real)(sqrt(s.vkep_out) - D.rmed_d[rb + nr - 1] * s.omega_frame - D.vref[rb + nr - 1]);
   }
 }
 
@@ -834,7 +852,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
   const double vr0 = -3.0 * (double)D.visc[rb + i] / r * (-c.sigmaslope + .5);
   const real l = (real)lambda, inv = (real)(1.0 / (1.0 + lambda));
   AT(D.vrad, i, j) = (AT(D.vrad, i, j) + l * (real)vr0) * inv;
-  AT(D.vtheta, i, j) = (AT(D.vtheta, i, j) + l * (real)vt0) * inv;
+  AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - D.vref[rb + i])) / (1.0 + lambda));
   AT(D.dens, i, j) = (AT(D.dens, i, j) + l * D.sigmamed[rb + i]) * inv;
   if (D.adiabatic) AT(D.energy, i, j) = (AT(D.energy, i, j) + l * D.energymed[rb + i]) * inv;
 }
@@ -845,19 +863,19 @@ __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
 // OneWindRad (TransportEuler.cu:54-73) in one sweep: each cell reads rings
 // i-2..i+2 and writes the radially advected density, the four momenta and
 // the energy
-template <class real>
-__device__ __forceinline__ real rad_slope(const real* w, int k, int base, int nr, const real* invdiffrmed) {
+template <class T>
+__device__ __forceinline__ T rad_slope(const T* w, int k, int base, int nr, const T* invdiffrmed) {
   if (k <= 0 || k >= nr - 1) return 0;
-  const real dqm = (w[k - base] - w[k - 1 - base]) * invdiffrmed[k];
-  const real dqp = (w[k + 1 - base] - w[k - base]) * invdiffrmed[k + 1];
-  return (dqp * dqm > 0) ? (real)2 * dqp * dqm / (dqp + dqm) : (real)0;
+  const T dqm = (w[k - base] - w[k - 1 - base]) * invdiffrmed[k];
+  const T dqp = (w[k + 1 - base] - w[k - base]) * invdiffrmed[k + 1];
+  return (dqp * dqm > 0) ? (T)2 * dqp * dqm / (dqp + dqm) : (T)0;
 }
 // interface value at ring m (1 <= m <= nr-1) for velocity v
-template <class real>
-__device__ __forceinline__ real rad_star(const real* w, int m, int base, int nr, real v, real dt, const real* rmed,
-                                         const real* invdiffrmed) {
-  if (v > 0) return w[m - 1 - base] + (rmed[m] - rmed[m - 1] - v * dt) * (real)0.5 * rad_slope(w, m - 1, base, nr, invdiffrmed);
-  return w[m - base] - (rmed[m + 1] - rmed[m] + v * dt) * (real)0.5 * rad_slope(w, m, base, nr, invdiffrmed);
+template <class T>
+__device__ __forceinline__ T rad_star(const T* w, int m, int base, int nr, T v, T dt, const T* rmed,
+                                      const T* invdiffrmed) {
+  if (v > 0) return w[m - 1 - base] + (rmed[m] - rmed[m - 1] - v * dt) * (T)0.5 * rad_slope(w, m - 1, base, nr, invdiffrmed);
+  return w[m - base] - (rmed[m + 1] - rmed[m] + v * dt) * (T)0.5 * rad_slope(w, m, base, nr, invdiffrmed);
 }
 
 template <class real>
@@ -865,13 +883,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
   const real dt = (real)s.dt;
-  const real om = (real)s.omega_frame;
+  const double dtd = s.dt;
+  const double om = s.omega_frame;
   const int jp = (j + 1 == ns) ? 0 : j + 1;
   const int base = i - 2;
   const real* rmed = D.rmed + rb;
   const real* idr = D.invdiffrmed + rb;
+  const double* rmedd = D.rmed_d + rb;
+  const double* idrd = D.invdiffrmed_d + rb;
   // rings i-2..i+2 (clamped loads; clamped values are never used)
-  real sg[5], vrr[6], rmp[5], rmm[5], tmp[5], tmm[5], en[5];
+  real sg[5], vrr[6], rmp[5], rmm[5], en[5];
+  double tmp[5], tmm[5];
   for (int q = 0; q < 6; ++q) {
     int k = base + q;
     k = k < 0 ? 0 : (k > nr ? nr : k);
@@ -883,20 +905,24 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
     k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
     const real sgk = AT(D.dens, k, j);
     sg[q] = sgk;
-    const real rk = rmed[k];
     rmp[q] = sgk * vrr[q + 1 <= 5 ? q + 1 : 5];
     rmm[q] = sgk * vrr[q];
-    tmp[q] = sgk * (AT(D.vtheta, k, jp) + rk * om) * rk;
-    tmm[q] = sgk * (AT(D.vtheta, k, j) + rk * om) * rk;
+    // angular momenta Sigma*(vtheta + R*Omega)*R in FP64 (vtheta = vref + residual)
+    const double rk = rmedd[k];
+    const double l0 = (D.vref[rb + k] + rk * om) * rk;
+    tmp[q] = (double)sgk * (l0 + (double)AT(D.vtheta, k, jp) * rk);
+    tmm[q] = (double)sgk * (l0 + (double)AT(D.vtheta, k, j) * rk);
     en[q] = D.adiabatic ? AT(D.energy, k, j) : (real)0;
   }
   // work arrays: quantity / (density + 1e-20)
-  real w_rmp[5], w_rmm[5], w_tmp[5], w_tmm[5], w_e[5], w_sg[5];
+  real w_rmp[5], w_rmm[5], w_e[5], w_sg[5];
+  double w_tmp[5], w_tmm[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     const real den = sg[q] + (real)1e-20;
-    w_rmp[q] = rmp[q] / den; w_rmm[q] = rmm[q] / den; w_tmp[q] = tmp[q] / den;
-    w_tmm[q] = tmm[q] / den; w_e[q] = en[q] / den; w_sg[q] = sg[q] / den;
+    w_rmp[q] = rmp[q] / den; w_rmm[q] = rmm[q] / den; w_e[q] = en[q] / den; w_sg[q] = sg[q] / den;
+    const double dend = (double)sg[q] + 1e-20;
+    w_tmp[q] = tmp[q] / dend; w_tmm[q] = tmm[q] / dend;
   }
   const real v0 = vrr[2], v1 = vrr[3];   // vr(i), vr(i+1)
   // density interface values (ComputeStarRad on Dens)
@@ -912,19 +938,33 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
     varq -= a1 * qs1 * ds1 * v1;
     return q0 + varq * isurf;
   };
+  const double dthd = 2.0 * kPI / (double)ns;
+  const double a0d = dtd * dthd * D.rinf_d[rb + i], a1d = dtd * dthd * (double)D.rsup[rb + i];
+  const double isurfd = (double)D.invsurf[rb + i];
+  auto advect_d = [&](const double* w, double q0) -> double {
+    const double qs0 = (i == 0) ? 0.0 : rad_star(w, i, base, nr, (double)v0, dtd, rmedd, idrd);
+    const double qs1 = (i + 1 == nr) ? 0.0 : rad_star(w, i + 1, base, nr, (double)v1, dtd, rmedd, idrd);
+    double varq = a0d * qs0 * (double)ds0 * (double)v0;
+    varq -= a1d * qs1 * (double)ds1 * (double)v1;
+    return q0 + varq * isurfd;
+  };
   AT(D.t_rmp, i, j) = advect(w_rmp, rmp[2]);
   AT(D.t_rmm, i, j) = advect(w_rmm, rmm[2]);
-  AT(D.t_tmp, i, j) = advect(w_tmp, tmp[2]);
-  AT(D.t_tmm, i, j) = advect(w_tmm, tmm[2]);
   if (D.adiabatic) AT(D.t_e, i, j) = advect(w_e, en[2]);
+  real sgn;
   {
     const real qs0 = (i == 0) ? (real)0 : rad_star(w_sg, i, base, nr, v0, dt, rmed, idr);
     const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w_sg, i + 1, base, nr, v1, dt, rmed, idr);
     real varq = a0 * qs0 * ds0 * v0;
     varq -= a1 * qs1 * ds1 * v1;
-    AT(D.t_dens, i, j) = sg[2] + varq * isurf;
+    sgn = sg[2] + varq * isurf;
+    AT(D.t_dens, i, j) = sgn;
     if (i == 0 && D.open_inner) D.lost[c.so + j] = varq;
   }
+  // angular momenta leave as residuals of (stored density) * l0(ring)
+  const double l0i = (D.vref[rb + i] + rmedd[i] * om) * rmedd[i];
+  AT(D.t_tmp, i, j) = (real)(advect_d(w_tmp, tmp[2]) - (double)sgn * l0i);
+  AT(D.t_tmm, i, j) = (real)(advect_d(w_tmm, tmm[2]) - (double)sgn * l0i);
 }
 
 // ====================================================== azimuthal transport
@@ -992,7 +1032,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D) {
   double acc = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)D.vtheta[row + j];
   acc = block_sum(acc, redd);
-  const real vmed = (real)(acc / (double)ns);
+  const double mres = acc / (double)ns;             // ring mean of the residual
+  const real vmed = (real)(D.vref[rb + i] + mres);  // VMed
   const real rmed = D.rmed[rb + i], irm = D.invrmed[rb + i];
   real ntilde, nround;
   const int nsh = fargo_shift_dev<real>(vmed, irm, dtd, ns, ntilde, nround);
@@ -1007,7 +1048,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D) {
   const real invsurf = (real)1 / D.surf[rb + i];
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     sD1[j] = D.t_dens[row + j];
-    real v = D.vtheta[row + j] - vmed;
+    real v = (real)((double)D.vtheta[row + j] - mres);   // VthetaRes
     if (!D.fast_transport) v = vconst + v;
     sV[j] = v;
   }
@@ -1078,13 +1119,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D) {
   __syncthreads();
   // shifted density and the azimuthal velocity of ComputeVelocitiesKernel
   // (Kernels.cu:1303-1304), evaluated in the unshifted frame and rotated
-  const real om = (real)s.omega_frame;
+  // the angular momenta are residuals of density * l0 (k_trans_rad), so
+  // (TMP + TMM)/(S + 1e-15)/R - R*Omega = vref + residual below, exactly
+  const double rd = D.rmed_d[rb + i];
+  const double l0 = (D.vref[rb + i] + rd * s.omega_frame) * rd;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     int js = j - nsm;
     js += (js < 0) ? ns : 0;
     const int jsm = (js == 0) ? ns - 1 : js - 1;
     D.dens[row + j] = sD1[js];
-    D.vtheta[row + j] = (sH[jsm] + sQ[js]) / (sD1[js] + sD1[jsm] + (real)1e-15) / rmed - rmed * om;
+    const double S = (double)sD1[js] + (double)sD1[jsm];
+    D.vtheta[row + j] = (real)(((double)sH[jsm] + (double)sQ[js] - l0 * 1e-15) / (S + 1e-15) / rd);
   }
 }
 
@@ -1153,7 +1198,8 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_potential_field(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
-  AT(D.work, i, j) = potential_at(D, c, s, D.rmed[rb + i], j);
+  const double r = D.rmed_d[rb + i];
+  AT(D.work, i, j) = (real)(potential_planets_at(D, c, s, r, j) - 1.0 / r);
 }
 // ComputeTemperatureFieldKernel (Kernels.cu:211-223) into D.work
 template <class real>

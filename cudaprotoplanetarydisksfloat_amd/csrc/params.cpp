@@ -136,11 +136,23 @@ static void search_variable(fgx_params* p, const char* name, const char* sv) {
   if (std::strcmp(name, "QUIRKFIXSGWRAP") == 0) p->ext_fix_sg_wrap = ti;
   if (std::strcmp(name, "SGSOFTENING") == 0) p->ext_sg_softening = f;
   if (std::strcmp(name, "GASNAMES") == 0) p->ext_gas_names = ti;
+  // the executable's switches -c, -b, -f (Main.cu:122-170) as keys
+  if (std::strcmp(name, "SLOPPYCFL") == 0) p->sloppy_cfl = ti;
+  if (std::strcmp(name, "CENTRIFUGALBALANCE") == 0) p->centrifugal_balance = ti;
+  if (std::strcmp(name, "SCALINGFACTOR") == 0) p->scaling_factor = f;
 }
 
 // derived flags and checks (src/Interpret.cu:126-201)
 static int derive_flags(fgx_params* p) {
   auto eq = [](const char* a, const char* b) { return std::strcmp(a, b) == 0; };
+  // flags are derived once from the final key values, starting from their
+  // defaults (src/Interpret.cu:23-31), so a later key can undo an earlier one
+  p->log_grid = p->open_inner = p->nonreflecting = p->evanescent = p->corotating = 0;
+  p->guiding_center = p->self_gravity = p->sg_zero_mode = p->zmplus_f = p->adiabatic = 0;
+  p->cooling = p->exclude_hill = p->cic_planet = p->forced_circular = p->viscosity_alpha = 0;
+  p->roche_smoothing = p->advect_label = p->outer_source_mass = p->write_energy = 0;
+  p->write_temperature = p->write_divv = p->write_qplus = 0;
+  p->fast_transport = p->indirect_term = p->is_disk = p->write_density = p->write_velocity = 1;
   if (eq(p->labeladvection, "YES")) p->advect_label = 1;
   if (eq(p->outersourcemass, "YES")) p->outer_source_mass = 1;
   if (eq(p->transport, "STANDARD")) p->fast_transport = 0;
@@ -261,9 +273,12 @@ int params_get_key(const fgx_params* p, const char* key, char* out, size_t n) {
       {"ADVECTLABEL", (double)p->advect_label}, {"WRITE_DENSITY", (double)p->write_density},
       {"WRITE_VELOCITY", (double)p->write_velocity}, {"WRITE_ENERGY", (double)p->write_energy},
       {"WRITE_TEMPERATURE", (double)p->write_temperature}, {"WRITE_DIVV", (double)p->write_divv},
-      {"WRITE_QPLUS", (double)p->write_qplus}, {"SLOPPYCFL", (double)p->sloppy_cfl}};
+      {"WRITE_QPLUS", (double)p->write_qplus}, {"SLOPPYCFL", (double)p->sloppy_cfl},
+      {"CENTRIFUGALBALANCE", (double)p->centrifugal_balance}, {"SCALINGFACTOR", p->scaling_factor},
+      {"QUIRKFIXTAURP", (double)p->ext_fix_taurp_corner}, {"QUIRKFIXPPDIST", (double)p->ext_fix_pp_distance},
+      {"QUIRKFIXSGWRAP", (double)p->ext_fix_sg_wrap}};
   for (const auto& f : nums)
-    if (k == f.k) { std::snprintf(out, n, "%.9g", f.v); return FGX_OK; }
+    if (k == f.k) { std::snprintf(out, n, "%.17g", f.v); return FGX_OK; }
   struct S { const char* k; const char* v; };
   const S strs[] = {{"OUTPUTDIR", p->outputdir}, {"INNERBOUNDARY", p->innerboundary},
                     {"LABELADVECTION", p->labeladvection}, {"TRANSPORT", p->transport},

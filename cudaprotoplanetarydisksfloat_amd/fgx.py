@@ -1,0 +1,294 @@
+"""Python mirror of the fgx C ABI (include/fgx.h) over ctypes.
+
+The reference drives its hot path from C++ main() (src/Main.cu:87-401); this
+module exposes the same run surface to Python callers and tests: parameter
+files (.par / .cfg), one disk or a batch of disks stepped by AlgoGas calls,
+field and ring-profile read-back, planets, forces and per-kernel timing.
+Every call goes to the native library libfgx.so built next to this file; an
+import fails loudly when it is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libfgx.so")
+
+FGX_OK = 0
+F32, F64 = 0, 1
+MAX_PLANETS = 8
+
+FIELDS = {"dens": 0, "vrad": 1, "vtheta": 2, "energy": 3, "temperature": 4,
+          "soundspeed": 5, "divv": 6, "qplus": 7, "label": 8, "potential": 9}
+RINGS = {"radii": 0, "rmed": 1, "surf": 2, "sigmamed": 3, "energymed": 4,
+         "viscosity": 5, "vmed": 6, "nshift": 7, "cs": 8, "newdt": 9,
+         "invrmed": 10}
+
+
+class FgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fgx error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(_LIBPATH):
+        raise ImportError(
+            f"native library {_LIBPATH} is missing: build it with "
+            "`python -m cudaprotoplanetarydisksfloat_amd.build`")
+    return ctypes.CDLL(_LIBPATH)
+
+
+lib = _load()
+
+
+class Planets(ctypes.Structure):
+    """fgx_planets: the reference's PlanetarySystem (include/Types.cuh:2-15)."""
+    _fields_ = [("nb", ctypes.c_int),
+                ("mass", ctypes.c_double * MAX_PLANETS),
+                ("x", ctypes.c_double * MAX_PLANETS),
+                ("y", ctypes.c_double * MAX_PLANETS),
+                ("vx", ctypes.c_double * MAX_PLANETS),
+                ("vy", ctypes.c_double * MAX_PLANETS),
+                ("acc", ctypes.c_double * MAX_PLANETS),
+                ("feel_disk", ctypes.c_int * MAX_PLANETS),
+                ("feel_others", ctypes.c_int * MAX_PLANETS),
+                ("name", (ctypes.c_char * 64) * MAX_PLANETS)]
+
+    def as_dict(self):
+        n = self.nb
+        return {"nb": n, "mass": list(self.mass[:n]), "x": list(self.x[:n]),
+                "y": list(self.y[:n]), "vx": list(self.vx[:n]), "vy": list(self.vy[:n]),
+                "feel_disk": list(self.feel_disk[:n]), "feel_others": list(self.feel_others[:n])}
+
+
+class DiskInfo(ctypes.Structure):
+    _fields_ = [("nrad", ctypes.c_int), ("nsec", ctypes.c_int), ("precision", ctypes.c_int),
+                ("physical_time", ctypes.c_double), ("omega_frame", ctypes.c_double),
+                ("lost_mass", ctypes.c_double), ("mdcp1", ctypes.c_double),
+                ("exces_mdcp", ctypes.c_double), ("last_dt", ctypes.c_double),
+                ("last_ncfl", ctypes.c_int), ("substeps", ctypes.c_longlong),
+                ("dt_steps", ctypes.c_longlong), ("health", ctypes.c_int)]
+
+
+_vp, _cp, _sz, _i, _d = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+_sigs = {
+    "fgx_version": ([], _i), "fgx_last_error": ([], _cp),
+    "fgx_sizeof_params": ([], _sz), "fgx_sizeof_planets": ([], _sz),
+    "fgx_device_count": ([ctypes.POINTER(_i)], _i),
+    "fgx_params_default": ([_vp], _i), "fgx_params_load": ([_cp, _vp], _i),
+    "fgx_params_parse": ([_cp, _vp], _i), "fgx_params_set": ([_vp, _cp, _cp], _i),
+    "fgx_params_get": ([_vp, _cp, _cp, _sz], _i),
+    "fgx_planets_load": ([_cp, _vp, ctypes.POINTER(Planets)], _i),
+    "fgx_planets_parse": ([_cp, _vp, ctypes.POINTER(Planets)], _i),
+    "fgx_disk_create": ([_vp, ctypes.POINTER(Planets), _i, ctypes.POINTER(_vp)], _i),
+    "fgx_batch_create": ([_i, _vp, ctypes.POINTER(Planets), _i, ctypes.POINTER(_vp)], _i),
+    "fgx_disk_step": ([_vp, _i], _i), "fgx_batch_step": ([_vp, _i], _i),
+    "fgx_disk_destroy": ([_vp], _i), "fgx_batch_destroy": ([_vp], _i),
+    "fgx_batch_size": ([_vp, ctypes.POINTER(_i)], _i),
+    "fgx_batch_info": ([_vp, _i, ctypes.POINTER(DiskInfo)], _i),
+    "fgx_batch_read_field": ([_vp, _i, _i, _i, _vp, _sz], _i),
+    "fgx_batch_write_field": ([_vp, _i, _i, _i, _vp, _sz], _i),
+    "fgx_disk_read_ring": ([_vp, _i, _i, _vp, _sz], _i),
+    "fgx_disk_planets": ([_vp, ctypes.POINTER(Planets)], _i),
+    "fgx_disk_set_planets": ([_vp, ctypes.POINTER(Planets)], _i),
+    "fgx_disk_force": ([_vp, _d, _d, _d, _d, ctypes.POINTER(_d)], _i),
+    "fgx_disk_write_outputs": ([_vp, _i], _i), "fgx_disk_write_logs": ([_vp, _i], _i),
+    "fgx_disk_write_setup_files": ([_vp], _i),
+    "fgx_footprint_bytes": ([_vp, _i, ctypes.POINTER(_sz)], _i),
+    "fgx_profile_enable": ([_vp, _i], _i),
+    "fgx_profile_get": ([_vp, _i, ctypes.POINTER(_cp), ctypes.POINTER(_d),
+                         ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_d)], _i),
+    "fgx_profile_reset": ([_vp], _i),
+    "fgx_run_main": ([_i, ctypes.POINTER(_cp)], _i),
+}
+for _n, (_a, _r) in _sigs.items():
+    _f = getattr(lib, _n)
+    _f.argtypes, _f.restype = _a, _r
+
+
+def _check(code):
+    if code != FGX_OK:
+        raise FgxError(code, lib.fgx_last_error().decode(errors="replace"))
+
+
+def exported_symbols():
+    """Names the C ABI header declares (checked by the tests)."""
+    return list(_sigs)
+
+
+class Params:
+    """fgx_params: the reference's .par keys (include/Param.cuh) and flags."""
+
+    def __init__(self, text=None, path=None, **keys):
+        self.buf = ctypes.create_string_buffer(lib.fgx_sizeof_params())
+        if path is not None:
+            _check(lib.fgx_params_load(os.fsencode(path), self.buf))
+        elif text is not None:
+            _check(lib.fgx_params_parse(text.encode(), self.buf))
+        else:
+            _check(lib.fgx_params_default(self.buf))
+        for k, v in keys.items():
+            self.set(k, v)
+
+    def set(self, key, value):
+        if isinstance(value, bool):
+            value = int(value)
+        if isinstance(value, float):
+            value = repr(value)
+        _check(lib.fgx_params_set(self.buf, key.encode(), str(value).encode()))
+        return self
+
+    def get(self, key):
+        out = ctypes.create_string_buffer(600)
+        _check(lib.fgx_params_get(self.buf, key.encode(), out, 600))
+        s = out.value.decode()
+        try:
+            return float(s)
+        except ValueError:
+            return s
+
+    def __getitem__(self, key):
+        return self.get(key)
+
+    def copy(self):
+        p = Params.__new__(Params)
+        p.buf = ctypes.create_string_buffer(self.buf.raw, len(self.buf))
+        return p
+
+    @property
+    def ptr(self):
+        return ctypes.cast(self.buf, ctypes.c_void_p)
+
+    def footprint(self, precision=F32):
+        n = ctypes.c_size_t()
+        _check(lib.fgx_footprint_bytes(self.buf, precision, ctypes.byref(n)))
+        return n.value
+
+
+def planets_from_text(text, params):
+    pl = Planets()
+    _check(lib.fgx_planets_parse(text.encode(), params.buf, ctypes.byref(pl)))
+    return pl
+
+
+def planets_from_file(path, params):
+    pl = Planets()
+    _check(lib.fgx_planets_load(os.fsencode(path), params.buf, ctypes.byref(pl)))
+    return pl
+
+
+class Batch:
+    """N disks advanced together; a single disk is a batch of one."""
+
+    def __init__(self, params, planets, precision=F32):
+        if isinstance(params, Params):
+            params, planets = [params], [planets]
+        n = len(params)
+        sz = lib.fgx_sizeof_params()
+        arr = ctypes.create_string_buffer(sz * n)
+        for k, p in enumerate(params):
+            ctypes.memmove(ctypes.addressof(arr) + k * sz, p.buf, sz)
+        pls = (Planets * n)(*planets)
+        h = ctypes.c_void_p()
+        _check(lib.fgx_batch_create(n, arr, pls, precision, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+        self.precision = precision
+        self.params = [p.copy() for p in params]
+        self.shape = [(int(p.get("NRAD")), int(p.get("NSEC"))) for p in params]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.fgx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n_dt=1):
+        _check(lib.fgx_batch_step(self.h, n_dt))
+
+    def info(self, disk=0):
+        out = DiskInfo()
+        _check(lib.fgx_batch_info(self.h, disk, ctypes.byref(out)))
+        return out
+
+    def field(self, name, disk=0, ghost=False, dtype=np.float64):
+        nr, ns = self.shape[disk]
+        n = (nr + 1 if ghost else nr) * ns
+        out = np.empty(n, dtype=dtype)
+        _check(lib.fgx_batch_read_field(self.h, disk, FIELDS[name], F64 if dtype == np.float64 else F32,
+                                        out.ctypes.data, n))
+        return out.reshape(-1, ns)
+
+    def set_field(self, name, values, disk=0):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        _check(lib.fgx_batch_write_field(self.h, disk, FIELDS[name], F64, v.ctypes.data, v.size))
+
+    # single-disk conveniences (disk 0)
+    def ring(self, name):
+        nr, _ = self.shape[0]
+        out = np.empty(nr + 1, dtype=np.float64)
+        _check(lib.fgx_disk_read_ring(self.h, RINGS[name], F64, out.ctypes.data, nr + 1))
+        return out
+
+    def planets(self):
+        pl = Planets()
+        _check(lib.fgx_disk_planets(self.h, ctypes.byref(pl)))
+        return pl
+
+    def set_planets(self, pl):
+        _check(lib.fgx_disk_set_planets(self.h, ctypes.byref(pl)))
+
+    def force(self, x, y, smoothing, mass):
+        out = (ctypes.c_double * 44)()
+        _check(lib.fgx_disk_force(self.h, x, y, smoothing, mass, out))
+        return np.array(out[:])
+
+    def write_setup_files(self):
+        _check(lib.fgx_disk_write_setup_files(self.h))
+
+    def write_outputs(self, index):
+        _check(lib.fgx_disk_write_outputs(self.h, index))
+
+    def write_logs(self, timestep):
+        _check(lib.fgx_disk_write_logs(self.h, timestep))
+
+    # per-kernel device time (HIP events on the library's stream)
+    def profile(self, on=True):
+        _check(lib.fgx_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        _check(lib.fgx_profile_reset(self.h))
+
+    def profile_stats(self):
+        out = {}
+        k = 0
+        while True:
+            name, ms, n, b = ctypes.c_char_p(), ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+            if lib.fgx_profile_get(self.h, k, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(n),
+                                   ctypes.byref(b)) != 0:
+                break
+            out[name.value.decode()] = {"ms": ms.value, "launches": n.value, "bytes_per_launch": b.value}
+            k += 1
+        return out
+
+
+Disk = Batch
+
+
+def device_count():
+    n = ctypes.c_int()
+    _check(lib.fgx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def run_main(argv):
+    """The reference executable's main() (src/Main.cu:87-401) in-process."""
+    args = (ctypes.c_char_p * len(argv))(*[a.encode() for a in argv])
+    return lib.fgx_run_main(len(argv), args)

@@ -108,7 +108,7 @@ enum fgx_field {
 enum fgx_ring {
   FGX_R_RADII = 0, FGX_R_RMED = 1, FGX_R_SURF = 2, FGX_R_SIGMAMED = 3,
   FGX_R_ENERGYMED = 4, FGX_R_VISCOSITY = 5, FGX_R_VMED = 6, FGX_R_NSHIFT = 7,
-  FGX_R_CS = 8, FGX_R_NEWDT = 9, FGX_NRINGS = 10
+  FGX_R_CS = 8, FGX_R_NEWDT = 9, FGX_R_INVRMED = 10, FGX_NRINGS = 11
 };
 
 /* per-disk scalar state (host copy) */
@@ -146,7 +146,7 @@ int fgx_params_parse(const char* text, fgx_params* p);
 /* SearchVariable (src/Interpret.cu:35-95) for one key followed by the
  * derived-flag pass; key is case-insensitive, value is raw */
 int fgx_params_set(fgx_params* p, const char* key, const char* value);
-/* value of one key as text ("%.9g" for numbers) */
+/* value of one key as text ("%.17g" for numbers) */
 int fgx_params_get(const fgx_params* p, const char* key, char* out, size_t n);
 
 /* ---- planets (src/Psys.cu:85-134) ----------------------------------------

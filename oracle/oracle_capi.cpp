@@ -68,6 +68,7 @@ int get_ring(D& s, int r, double* out, size_t n) {
       case FGX_R_NSHIFT: v = s.Nshift[i]; break;
       case FGX_R_CS: v = (i < (size_t)s.NRAD) ? (double)s.SoundSpeed[i * s.NSEC] : 0.0; break;
       case FGX_R_NEWDT: v = s.newDT[i]; break;
+      case FGX_R_INVRMED: v = s.invRmed[i]; break;
       default: return FGX_ERANGE;
     }
     out[i] = v;

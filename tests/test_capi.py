@@ -1,0 +1,45 @@
+"""The C-ABI library loads and exports every entry point include/fgx.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "fgx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fgx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    lib = ctypes.CDLL(os.path.join(ROOT, "cudaprotoplanetarydisksfloat_amd", "libfgx.so"))
+    names = _declared()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_header():
+    from cudaprotoplanetarydisksfloat_amd import fgx
+    bound = set(fgx.exported_symbols())
+    assert set(_declared()) - bound <= {"fgx_disk_info_get", "fgx_disk_read_field", "fgx_disk_write_field"}
+
+
+def test_layout_sizes_agree():
+    from cudaprotoplanetarydisksfloat_amd import fgx
+    import oracle
+    assert fgx.lib.fgx_sizeof_params() == oracle.lib().orc_sizeof_params()
+    assert fgx.lib.fgx_sizeof_planets() == ctypes.sizeof(fgx.Planets) == oracle.lib().orc_sizeof_planets()
+
+
+def test_create_without_device_fails_cleanly():
+    from cudaprotoplanetarydisksfloat_amd import Batch, FgxError, configs, device_count
+    if device_count() > 0:
+        pytest.skip("a device is present")
+    p, pl = configs.load(*configs.plain_disk(nrad=16, nsec=32))
+    with pytest.raises(FgxError) as e:
+        Batch(p, pl)
+    assert e.value.code == -2
