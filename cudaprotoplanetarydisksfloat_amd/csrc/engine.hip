@@ -227,8 +227,12 @@ struct Batch : BatchBase {
     for (real** r : rings) LCHK(dalloc(r, R));
     LCHK(dalloc(&D.nshift, R));
     {
-      double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d;
+      double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d;
       LCHK(dalloc(&vref, R)); LCHK(dalloc(&gstar, R)); LCHK(dalloc(&rinf_d, R)); LCHK(dalloc(&invrinf_d, R));
+      LCHK(dalloc(&cs2_d, R)); D.cs2_d = cs2_d;
+      double *sigmamed_d, *energymed_d;
+      LCHK(dalloc(&sigmamed_d, R)); LCHK(dalloc(&energymed_d, R));
+      D.sigmamed_d = sigmamed_d; D.energymed_d = energymed_d;
       LCHK(dalloc(&rmed_d, R)); LCHK(dalloc(&invdiffrmed_d, R)); LCHK(dalloc(&D.vmoy, R));
       LCHK(dalloc(&cosj_d, S)); LCHK(dalloc(&sinj_d, S));
       D.vref = vref; D.gstar = gstar; D.rinf_d = rinf_d; D.invrinf_d = invrinf_d; D.rmed_d = rmed_d;
@@ -329,7 +333,10 @@ struct Batch : BatchBase {
       const size_t n = (size_t)h.nrad + 1;
       // FP64 tables: reference vtheta (the initial, ring-uniform profile),
       // star gravity between ring centres, geometry
-      std::vector<double> vref(n, 0.0), gstar(n, 0.0), invrinf(n, 0.0);
+      std::vector<double> vref(n, 0.0), gstar(n, 0.0), invrinf(n, 0.0), cs2(n, 0.0);
+      for (int i = 0; i < h.nrad; ++i) cs2[i] = h.cs_ring[i] * h.cs_ring[i];
+      LCHK(upload_d(D.cs2_d + o, cs2));
+      LCHK(upload_d(D.sigmamed_d + o, h.sigmamed)); LCHK(upload_d(D.energymed_d + o, h.energymed));
       for (int i = 0; i < h.nrad; ++i) vref[i] = h.vtheta[(size_t)i * h.nsec];
       for (int i = 1; i < h.nrad; ++i) gstar[i] = (1.0 / h.rmed[i - 1] - 1.0 / h.rmed[i]) / (h.rmed[i] - h.rmed[i - 1]);
       for (int i = 0; i < h.nrad; ++i) invrinf[i] = 1.0 / h.rinf[i];
@@ -748,7 +755,9 @@ int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes) {
   const size_t F = (size_t)(p->nrad + 1) * p->nsec, R = (size_t)p->nrad + 1;
   const size_t rs = precision == FGX_F64 ? 8 : 4;
   const int nfields = p->adiabatic ? 23 : 17;
-  *bytes = F * rs * nfields + R * rs * 21 + (size_t)p->nsec * rs * 3 + R * 8 * 2 * FGX_NFORCE_T + 4096;
+  // fields, ring tables (working precision and the FP64 set), sector tables
+  *bytes = F * rs * nfields + R * rs * 21 + R * 8 * 10 + (size_t)p->nsec * (rs * 3 + 8 * 2) +
+           R * 8 * 2 * FGX_NFORCE_T + 4096;
   return FGX_OK;
 }
 

@@ -37,7 +37,7 @@ struct Dev {
   // the reference azimuthal velocity profile (vtheta and the angular momenta
   // are stored as FP32 residuals from it), the star's radial gravity between
   // ring centres, and the cell geometry of the potential
-  const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d;
+  const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d, *energymed_d;
   double* vmoy;  // ring-mean vtheta of the CFL condition
   const DiskConst* dc;
   DiskState* st;
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
   }
   s.ntarget = nt;
   // ApplySubKeplerianBoundary values (SideEuler.cu:250-268), FP64
-  const double r0 = D.rmed[rb], rn = D.rmed[rb + nr - 1];
+  const double r0 = D.rmed_d[rb], rn = D.rmed_d[rb + nr - 1];
   const double fac = 1.0 + c.sigmaslope - 2.0 * c.flaringindex;
   const double h0 = aspect_ratio_dev(c, r0, tt), hn = aspect_ratio_dev(c, rn, tt);
   s.vkep_in = 1.0 / r0 * (1.0 - fac * h0 * h0 * pow(r0, 2.0 * c.flaringindex));
@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_profile(Dev<real> D, int updat
     if (threadIdx.x == 0) D.csmean[rb + i] = (real)acc;
   }
   if (threadIdx.x == 0) {
-    const double nu = fviscosity_dev(c, alpha, (double)D.rmed[rb + i], (double)D.csmean[rb + i], s.time - s.time0);
+    const double nu = fviscosity_dev(c, alpha, D.rmed_d[rb + i], (double)D.csmean[rb + i], s.time - s.time0);
     D.visc[rb + i] = (real)nu;
   }
 }
@@ -557,24 +557,25 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
   const double om = s.omega_frame;
   const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
   const double rmed = D.rmed_d[rb + i];
-  const real g1 = (real)c.adiabaticindex - (real)1;
-  auto press = [&](int ii, int jj) -> real {
-    if (D.adiabatic) return g1 * AT(D.energy, ii, jj);
-    const real cs = D.cs_ring[rb + ii];
-    return AT(D.dens, ii, jj) * cs * cs;
+  // pressure in FP64 from the FP32 state: Sigma*cs^2 with the exact
+  // isothermal cs^2 of the ring, or (gamma-1)*e
+  const double g1 = c.adiabaticindex - 1.0;
+  auto press = [&](int ii, int jj) -> double {
+    if (D.adiabatic) return g1 * (double)AT(D.energy, ii, jj);
+    return (double)AT(D.dens, ii, jj) * D.cs2_d[rb + ii];
   };
   const real sig = AT(D.dens, i, j);
-  const real p = press(i, j);
+  const double p = press(i, j);
   const double phi = potential_planets_at(D, c, s, rmed, j);
-  if (D.write_pressure) AT(D.pressure, i, j) = p;
+  if (D.write_pressure) AT(D.pressure, i, j) = (real)p;
   // radial
   if (i > 0) {
     const double rm1 = D.rmed_d[rb + i - 1];
     const real sigm = AT(D.dens, i - 1, j);
-    const real pm = press(i - 1, j);
+    const double pm = press(i - 1, j);
     const double phim = potential_planets_at(D, c, s, rm1, j);
     const double idr = D.invdiffrmed_d[rb + i];
-    const double gradp = (double)((p - pm) * (real)2 / (sig + sigm)) * idr;
+    const double gradp = (p - pm) * 2.0 / ((double)sig + (double)sigm) * idr;
     const double gradphi = (phi - phim) * idr + D.gstar[rb + i];
     double vt2 = 2.0 * D.vref[rb + i] + 2.0 * D.vref[rb + i - 1] +
                  (double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) +
@@ -588,9 +589,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
   // azimuthal (the star's potential is the same at j and j-1)
   const real invdxt = (real)1 / D.dxtheta[rb + i];
   const real sigl = AT(D.dens, i, jm);
-  const real pl = press(i, jm);
+  const double pl = press(i, jm);
   const double phil = potential_planets_at(D, c, s, rmed, jm);
-  const real gradp = (p - pl) * (real)2 / (sig + sigl) * invdxt;
+  const real gradp = (real)((p - pl) * 2.0 / ((double)sig + (double)sigl)) * invdxt;
   const real gradphi = (real)(phi - phil) * invdxt;
   real v = AT(D.vtheta, i, j) - (real)dt * (gradp + gradphi);
   v += (real)dt * D.supp_torque[rb + i];
@@ -726,7 +727,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
   if (i > 0) qp = qplus_at(D, c, rb, fb, ns, i, j);
   else if (D.visc[rb + nr - 1] != (real)0) {
     const real q1 = qplus_at(D, c, rb, fb, ns, 1, j), q2 = qplus_at(D, c, rb, fb, ns, 2, j);
-    qp = q1 * exp(log(q1 / q2) * log(D.rmed[rb] / D.rmed[rb + 1]) / log(D.rmed[rb + 1] / D.rmed[rb + 2]));
+    qp = q1 * (real)exp((double)log(q1 / q2) * log(D.rmed_d[rb] / D.rmed_d[rb + 1]) / log(D.rmed_d[rb + 1] / D.rmed_d[rb + 2]));
   } else qp = 0;
   AT(D.qplus, i, j) = qp;
   const real g1 = (real)c.adiabaticindex - (real)1;
@@ -783,11 +784,11 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
   a0 = block_sum(a0, redd) / ns; a1 = block_sum(a1, redd) / ns;
   a2 = block_sum(a2, redd) / ns; a3 = block_sum(a3, redd) / ns;
   if (threadIdx.x == 0) {
-    double da = (pow((double)D.rinf[rb + 1], -1.5) - 1.0) / (.5 * (a0 + a1));
-    da *= ((double)D.rmed[rb + 1] - (double)D.rmed[rb]);
+    double da = (pow(D.rinf_d[rb + 1], -1.5) - 1.0) / (.5 * (a0 + a1));
+    da *= (D.rmed_d[rb + 1] - D.rmed_d[rb]);
     ia_sh[0] = (int)(da / 2.0 / kPI * (double)ns + .5);
-    double da2 = (pow((double)D.rinf[rb + nr - 2], -1.5) - 1.0) / (.5 * (a2 + a3));
-    da2 *= ((double)D.rmed[rb + nr - 1] - (double)D.rmed[rb + nr - 2]);
+    double da2 = (pow(D.rinf_d[rb + nr - 2], -1.5) - 1.0) / (.5 * (a2 + a3));
+    da2 *= (D.rmed_d[rb + nr - 1] - D.rmed_d[rb + nr - 2]);
     ia_sh[1] = (int)(da2 / 2.0 / kPI * (double)ns + .5);
     s.nr_iangle = ia_sh[0]; s.nr_iangle2 = ia_sh[1];
   }
@@ -817,14 +818,18 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
     m0 += (double)AT(D.dens, 0, j); m1 += (double)AT(D.dens, nr - 1, j);
     if (D.adiabatic) { e0 += (double)AT(D.energy, 0, j); e1 += (double)AT(D.energy, nr - 1, j); }
   }
-  const real md = (real)(block_sum(m0, redd) / ns), md2 = (real)(block_sum(m1, redd) / ns);
-  const real me = (real)(block_sum(e0, redd) / ns), me2 = (real)(block_sum(e1, redd) / ns);
+  const double md = block_sum(m0, redd) / ns, md2 = block_sum(m1, redd) / ns;
+  const double me = block_sum(e0, redd) / ns, me2 = block_sum(e1, redd) / ns;
+  // ring means restored to the initial profile, in FP64
+  const double cd = D.sigmamed_d[rb] - md, cd2 = D.sigmamed_d[rb + nr - 1] - md2;
+  const double ce = D.adiabatic ? D.energymed_d[rb] - me : 0.0;
+  const double ce2 = D.adiabatic ? D.energymed_d[rb + nr - 1] - me2 : 0.0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    AT(D.dens, 0, j) += D.sigmamed[rb] - md;
-    AT(D.dens, nr - 1, j) += D.sigmamed[rb + nr - 1] - md2;
+    AT(D.dens, 0, j) = (real)((double)AT(D.dens, 0, j) + cd);
+    AT(D.dens, nr - 1, j) = (real)((double)AT(D.dens, nr - 1, j) + cd2);
     if (D.adiabatic) {
-      AT(D.energy, 0, j) += D.energymed[rb] - me;
-      AT(D.energy, nr - 1, j) += D.energymed[rb + nr - 1] - me2;
+      AT(D.energy, 0, j) = (real)((double)AT(D.energy, 0, j) + ce);
+      AT(D.energy, nr - 1, j) = (real)((double)AT(D.energy, nr - 1, j) + ce2);
     }
   }
 }
@@ -834,27 +839,28 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
-  const double r = D.rmed[rb + i];
+  const double r = D.rmed_d[rb + i];
   if (!(r < s.ev_drmin || r > s.ev_drmax)) return;
   const double dt = s.dt;
   double lambda = 0, damping;
   if (r < s.ev_drmin) {
-    damping = (r - s.ev_drmin) / ((double)D.rmed[rb] - s.ev_drmin);
+    damping = (r - s.ev_drmin) / (D.rmed_d[rb] - s.ev_drmin);
     lambda = damping * damping * 10.0 * dt / s.ev_tin;
   }
   if (r > s.ev_drmax) {
-    damping = (r - s.ev_drmax) / ((double)D.rmed[rb + nr - 1] - s.ev_drmax);
+    damping = (r - s.ev_drmax) / (D.rmed_d[rb + nr - 1] - s.ev_drmax);
     lambda = damping * damping * 10.0 * dt / s.ev_tout;
   }
   const double h = aspect_ratio_dev(c, r, s.time - s.time0);
   double vt0 = sqrt(1.0 / r * (1.0 - (1.0 + c.sigmaslope - 2.0 * c.flaringindex) * h * h * pow(r, 2.0 * c.flaringindex)));
   vt0 -= r * s.omega_frame;
   const double vr0 = -3.0 * (double)D.visc[rb + i] / r * (-c.sigmaslope + .5);
-  const real l = (real)lambda, inv = (real)(1.0 / (1.0 + lambda));
-  AT(D.vrad, i, j) = (AT(D.vrad, i, j) + l * (real)vr0) * inv;
-  AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - D.vref[rb + i])) / (1.0 + lambda));
-  AT(D.dens, i, j) = (AT(D.dens, i, j) + l * D.sigmamed[rb + i]) * inv;
-  if (D.adiabatic) AT(D.energy, i, j) = (AT(D.energy, i, j) + l * D.energymed[rb + i]) * inv;
+  // relaxation towards the FP64 initial profiles, evaluated in FP64
+  const double inv = 1.0 / (1.0 + lambda);
+  AT(D.vrad, i, j) = (real)(((double)AT(D.vrad, i, j) + lambda * vr0) * inv);
+  AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - D.vref[rb + i])) * inv);
+  AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * D.sigmamed_d[rb + i]) * inv);
+  if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * D.energymed_d[rb + i]) * inv);
 }
 
 // ========================================================= radial transport
@@ -883,17 +889,15 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
   const real dt = (real)s.dt;
-  const double dtd = s.dt;
   const double om = s.omega_frame;
   const int jp = (j + 1 == ns) ? 0 : j + 1;
   const int base = i - 2;
   const real* rmed = D.rmed + rb;
   const real* idr = D.invdiffrmed + rb;
   const double* rmedd = D.rmed_d + rb;
-  const double* idrd = D.invdiffrmed_d + rb;
   // rings i-2..i+2 (clamped loads; clamped values are never used)
-  real sg[5], vrr[6], rmp[5], rmm[5], en[5];
-  double tmp[5], tmm[5];
+  real sg[5], vrr[6], rmp[5], rmm[5], en[5], w_tmp[5], w_tmm[5];
+  double tmp_i = 0, tmm_i = 0;
   for (int q = 0; q < 6; ++q) {
     int k = base + q;
     k = k < 0 ? 0 : (k > nr ? nr : k);
@@ -907,22 +911,23 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
     sg[q] = sgk;
     rmp[q] = sgk * vrr[q + 1 <= 5 ? q + 1 : 5];
     rmm[q] = sgk * vrr[q];
-    // angular momenta Sigma*(vtheta + R*Omega)*R in FP64 (vtheta = vref + residual)
+    // specific angular momentum (vtheta + R*Omega)*R (vtheta = vref + residual);
+    // the momenta themselves are kept in FP64 only for ring i
     const double rk = rmedd[k];
     const double l0 = (D.vref[rb + k] + rk * om) * rk;
-    tmp[q] = (double)sgk * (l0 + (double)AT(D.vtheta, k, jp) * rk);
-    tmm[q] = (double)sgk * (l0 + (double)AT(D.vtheta, k, j) * rk);
+    const double lp = l0 + (double)AT(D.vtheta, k, jp) * rk, lm = l0 + (double)AT(D.vtheta, k, j) * rk;
+    const double dd = (double)sgk;
+    w_tmp[q] = (real)(dd * lp / (dd + 1e-20));
+    w_tmm[q] = (real)(dd * lm / (dd + 1e-20));
+    if (q == 2) { tmp_i = dd * lp; tmm_i = dd * lm; }
     en[q] = D.adiabatic ? AT(D.energy, k, j) : (real)0;
   }
   // work arrays: quantity / (density + 1e-20)
   real w_rmp[5], w_rmm[5], w_e[5], w_sg[5];
-  double w_tmp[5], w_tmm[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     const real den = sg[q] + (real)1e-20;
     w_rmp[q] = rmp[q] / den; w_rmm[q] = rmm[q] / den; w_e[q] = en[q] / den; w_sg[q] = sg[q] / den;
-    const double dend = (double)sg[q] + 1e-20;
-    w_tmp[q] = tmp[q] / dend; w_tmm[q] = tmm[q] / dend;
   }
   const real v0 = vrr[2], v1 = vrr[3];   // vr(i), vr(i+1)
   // density interface values (ComputeStarRad on Dens)
@@ -938,33 +943,36 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
     varq -= a1 * qs1 * ds1 * v1;
     return q0 + varq * isurf;
   };
-  const double dthd = 2.0 * kPI / (double)ns;
-  const double a0d = dtd * dthd * D.rinf_d[rb + i], a1d = dtd * dthd * (double)D.rsup[rb + i];
-  const double isurfd = (double)D.invsurf[rb + i];
-  auto advect_d = [&](const double* w, double q0) -> double {
-    const double qs0 = (i == 0) ? 0.0 : rad_star(w, i, base, nr, (double)v0, dtd, rmedd, idrd);
-    const double qs1 = (i + 1 == nr) ? 0.0 : rad_star(w, i + 1, base, nr, (double)v1, dtd, rmedd, idrd);
-    double varq = a0d * qs0 * (double)ds0 * (double)v0;
-    varq -= a1d * qs1 * (double)ds1 * (double)v1;
-    return q0 + varq * isurfd;
+  // flux divergence of a quantity (its FP32 rounding is far below the
+  // quantity itself); the quantity is added back in FP64 by the caller
+  auto fluxdiv = [&](const real* w) -> real {
+    const real qs0 = (i == 0) ? (real)0 : rad_star(w, i, base, nr, v0, dt, rmed, idr);
+    const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w, i + 1, base, nr, v1, dt, rmed, idr);
+    real varq = a0 * qs0 * ds0 * v0;
+    varq -= a1 * qs1 * ds1 * v1;
+    return varq * isurf;
   };
   AT(D.t_rmp, i, j) = advect(w_rmp, rmp[2]);
   AT(D.t_rmm, i, j) = advect(w_rmm, rmm[2]);
   if (D.adiabatic) AT(D.t_e, i, j) = advect(w_e, en[2]);
-  real sgn;
+  double sgx;   // advected density before its FP32 rounding
   {
     const real qs0 = (i == 0) ? (real)0 : rad_star(w_sg, i, base, nr, v0, dt, rmed, idr);
     const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w_sg, i + 1, base, nr, v1, dt, rmed, idr);
     real varq = a0 * qs0 * ds0 * v0;
     varq -= a1 * qs1 * ds1 * v1;
-    sgn = sg[2] + varq * isurf;
-    AT(D.t_dens, i, j) = sgn;
+    const real inc = varq * isurf;
+    sgx = (double)sg[2] + (double)inc;
+    AT(D.t_dens, i, j) = sg[2] + inc;
     if (i == 0 && D.open_inner) D.lost[c.so + j] = varq;
   }
-  // angular momenta leave as residuals of (stored density) * l0(ring)
+  // angular momenta leave as residuals of density * l0(ring), with the
+  // density taken before rounding: the azimuthal velocity rebuilt from them
+  // (vref + residual / (density * R)) then carries no trace of the density's
+  // storage rounding, as in an exact-arithmetic L / Sigma
   const double l0i = (D.vref[rb + i] + rmedd[i] * om) * rmedd[i];
-  AT(D.t_tmp, i, j) = (real)(advect_d(w_tmp, tmp[2]) - (double)sgn * l0i);
-  AT(D.t_tmm, i, j) = (real)(advect_d(w_tmm, tmm[2]) - (double)sgn * l0i);
+  AT(D.t_tmp, i, j) = (real)(tmp_i + (double)fluxdiv(w_tmp) - sgx * l0i);
+  AT(D.t_tmm, i, j) = (real)(tmm_i + (double)fluxdiv(w_tmm) - sgx * l0i);
 }
 
 // ====================================================== azimuthal transport
