@@ -22,6 +22,7 @@
 
 #include "fgx_internal.h"
 #include "kernels.hpp"
+#include "transport.hpp"
 
 namespace fgx {
 
@@ -60,6 +61,13 @@ struct Batch : BatchBase {
   Dev<real> D{};
   std::vector<void*> allocs;
   int n_ctiles = 0, n_rtiles = 0, max_nsec = 0, max_nrad = 0;
+  // radial strips (d, first ring, sector block) and the azimuthal ring lists:
+  // one per lane width K of the register-resident kernel, plus the rings
+  // whose sector count fits none (shared-memory kernel)
+  int n_stiles = 0, strip_rings = 16;
+  const CellTile* stiles = nullptr;
+  struct ThetaGroup { int k, w, n; const RingTile* tiles; };
+  std::vector<ThetaGroup> theta_groups;
   size_t F = 0, R = 0, S = 0;
   int* h_summary = nullptr;  // pinned
   int visc_dynamic = 0, alpha = 0, need_force = 1;
@@ -129,7 +137,8 @@ struct Batch : BatchBase {
   int launch(const char* name, double bytes_per_cell, K kern, int grid, int block, size_t shmem, A... args) {
     if (grid <= 0) return FGX_OK;
     Pending pd{};
-    if (prof_on) {
+    const bool timed = prof_on && !in_group;
+    if (timed) {
       pd.k = prof_index(name); pd.a = get_event(); pd.b = get_event();
       pd.bytes = bytes_per_cell * sizeof(real) * cells();
       hipEventRecord(pd.a, stream);
@@ -137,8 +146,22 @@ struct Batch : BatchBase {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, stream, args...);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(FGX_EDEVICE, "launch of %s failed: %s", name, hipGetErrorString(e));
-    if (prof_on) { hipEventRecord(pd.b, stream); pending.push_back(pd); }
+    if (timed) { hipEventRecord(pd.b, stream); pending.push_back(pd); }
     return FGX_OK;
+  }
+  // several launches timed as one kernel of the profile
+  bool in_group = false;
+  Pending grp{};
+  void group_begin(const char* name, double bytes_per_cell) {
+    if (prof_on) {
+      grp = Pending{prof_index(name), get_event(), get_event(), bytes_per_cell * sizeof(real) * cells()};
+      hipEventRecord(grp.a, stream);
+    }
+    in_group = true;
+  }
+  void group_end() {
+    if (prof_on) { hipEventRecord(grp.b, stream); pending.push_back(grp); }
+    in_group = false;
   }
 #define LCHK(x) do { int r_ = (x); if (r_) return r_; } while (0)
 
@@ -177,6 +200,7 @@ struct Batch : BatchBase {
       DiskConst& c = dch[d];
       std::memset(&c, 0, sizeof c);
       c.nrad = h.nrad; c.nsec = h.nsec;
+      F = (F + 63) & ~(size_t)63;   // every disk starts on a 256-byte boundary (vector loads)
       c.fo = (long long)F; c.ro = (int)R; c.so = (int)S;
       c.rt0 = (int)rt.size();
       F += (size_t)(h.nrad + 1) * h.nsec;
@@ -207,6 +231,27 @@ struct Batch : BatchBase {
       visc_dynamic |= c.visc_dynamic;
     }
     alpha = P0.viscosity_alpha;
+    // strip height: as tall as possible while the batch still yields about
+    // 2048 strip tiles (32 waves per CU), between 2 and 16 rings
+    {
+      double tiles16 = 0;
+      for (int d = 0; d < nd; ++d)
+        tiles16 += std::ceil(hd[d].nrad / 16.0) * std::ceil(hd[d].nsec / (double)FGX_TPB);
+      strip_rings = 16;
+      while (strip_rings > 2 && tiles16 * 16.0 / strip_rings < 2048) strip_rings /= 2;
+    }
+    std::vector<CellTile> st;
+    std::vector<std::vector<RingTile>> tg(17 * 17);   // index K*17 + W (K = 0: shared-memory kernel)
+    for (int d = 0; d < nd; ++d) {
+      const HostDisk& h = hd[d];
+      for (int i0 = 0; i0 < h.nrad; i0 += strip_rings)
+        for (int jb = 0; jb * FGX_TPB < h.nsec; ++jb) st.push_back(CellTile{d, i0, jb, 0});
+      int K = 0, W = 0;
+      theta_split<real>(h.nsec, K, W);
+      for (int i = 0; i < h.nrad; ++i) tg[K * 17 + W].push_back(RingTile{d, i});
+    }
+    while (st.size() % 8) st.push_back(CellTile{-1, 0, 0, 0});
+    n_stiles = (int)st.size();
     while (ct.size() % 8) ct.push_back(CellTile{-1, 0, 0, 0});
     while (rt.size() % 8) rt.push_back(RingTile{-1, 0});
     n_ctiles = (int)ct.size();
@@ -248,6 +293,22 @@ struct Batch : BatchBase {
     CellTile* ctd; RingTile* rtd;
     LCHK(dalloc(&ctd, ct.size())); LCHK(dalloc(&rtd, rt.size()));
     D.ctiles = ctd; D.rtiles = rtd;
+    {
+      CellTile* std_t;
+      LCHK(dalloc(&std_t, st.size()));
+      HIPCHK(hipMemcpyAsync((void*)std_t, st.data(), st.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
+      stiles = std_t;
+      for (int kw = 0; kw < 17 * 17; ++kw) {
+        std::vector<RingTile>& v = tg[kw];
+        if (v.empty()) continue;
+        while (v.size() % 8) v.push_back(RingTile{-1, 0});
+        RingTile* g;
+        LCHK(dalloc(&g, v.size()));
+        HIPCHK(hipMemcpyAsync((void*)g, v.data(), v.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
+        theta_groups.push_back(ThetaGroup{kw / 17, kw % 17, (int)v.size(), g});
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+    }
     D.max_nsec = max_nsec; D.ndisk = nd;
     D.adiabatic = ad; D.fast_transport = P0.fast_transport; D.open_inner = P0.open_inner;
     D.nonreflecting = P0.nonreflecting; D.evanescent = P0.evanescent; D.self_gravity = P0.self_gravity;
@@ -375,6 +436,38 @@ struct Batch : BatchBase {
   }
 
   // ------------------------------------------------------------------ step
+  // azimuthal transport: one launch per lane width K
+  template <int K>
+  int theta_k(const ThetaGroup& g) {
+    if constexpr (sizeof(real) == 8 && K > 8) {
+      return set_error(FGX_EDEVICE, "no FP64 azimuthal kernel for K=%d", g.k);
+    } else {
+      return launch("trans_theta", 0, k_trans_theta_arc<real, K>, g.n, 64 * g.w, 0, D, g.tiles, g.w);
+    }
+  }
+  int trans_theta(double bytes_per_cell) {
+    group_begin("trans_theta", bytes_per_cell);
+    int r = FGX_OK;
+    for (const ThetaGroup& g : theta_groups) {
+      switch (g.k) {
+        case 0: r = launch("trans_theta", 0, k_trans_theta<real>, g.n, FGX_TPB, theta_shmem, D, g.tiles); break;
+        case 1: r = theta_k<1>(g); break;
+        case 2: r = theta_k<2>(g); break;
+        case 3: r = theta_k<3>(g); break;
+        case 4: r = theta_k<4>(g); break;
+        case 5: r = theta_k<5>(g); break;
+        case 6: r = theta_k<6>(g); break;
+        case 8: r = theta_k<8>(g); break;
+        case 10: r = theta_k<10>(g); break;
+        case 12: r = theta_k<12>(g); break;
+        case 16: r = theta_k<16>(g); break;
+        default: r = set_error(FGX_EDEVICE, "no azimuthal kernel for K=%d", g.k);
+      }
+      if (r) break;
+    }
+    group_end();
+    return r;
+  }
   int boundary() {
     if (D.open_inner) LCHK(launch("open_boundary", 0, k_open_boundary<real>, nd, FGX_TPB, 0, D));
     if (D.nonreflecting) {
@@ -407,8 +500,8 @@ struct Batch : BatchBase {
       LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_ctiles, FGX_TPB, 0, D, (const real*)D.vrad, (const real*)D.vtheta, 1));
       LCHK(launch("substep3", 9, k_substep3<real>, n_ctiles, FGX_TPB, 0, D));
     }
-    LCHK(launch("trans_rad", ad ? 10 : 8, k_trans_rad<real>, n_ctiles, FGX_TPB, 0, D));
-    LCHK(launch("trans_theta", ad ? 11 : 9, k_trans_theta<real>, n_rtiles, FGX_TPB, theta_shmem, D));
+    LCHK(launch("trans_rad", ad ? 10 : 8, k_trans_rad_strip<real>, n_stiles, FGX_TPB, 0, D, stiles, strip_rings));
+    LCHK(trans_theta(ad ? 11 : 9));
     LCHK(launch("trans_vr", 4, k_trans_vr<real>, n_ctiles, FGX_TPB, 0, D));
     LCHK(boundary());
     LCHK(launch("end_substep", 0, k_end_substep<real>, nd, 64, 0, D));

@@ -863,118 +863,6 @@ __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
   if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * D.energymed_d[rb + i]) * inv);
 }
 
-// ========================================================= radial transport
-// LRMomentaKernel + StarRadKernel + StarRadKernel2 + DivisePolarGridKernel +
-// VanLeerRadialKernel (Kernels.cu:883-947, 1127-1154) for all quantities of
-// OneWindRad (TransportEuler.cu:54-73) in one sweep: each cell reads rings
-// i-2..i+2 and writes the radially advected density, the four momenta and
-// the energy
-template <class T>
-__device__ __forceinline__ T rad_slope(const T* w, int k, int base, int nr, const T* invdiffrmed) {
-  if (k <= 0 || k >= nr - 1) return 0;
-  const T dqm = (w[k - base] - w[k - 1 - base]) * invdiffrmed[k];
-  const T dqp = (w[k + 1 - base] - w[k - base]) * invdiffrmed[k + 1];
-  return (dqp * dqm > 0) ? (T)2 * dqp * dqm / (dqp + dqm) : (T)0;
-}
-// interface value at ring m (1 <= m <= nr-1) for velocity v
-template <class T>
-__device__ __forceinline__ T rad_star(const T* w, int m, int base, int nr, T v, T dt, const T* rmed,
-                                      const T* invdiffrmed) {
-  if (v > 0) return w[m - 1 - base] + (rmed[m] - rmed[m - 1] - v * dt) * (T)0.5 * rad_slope(w, m - 1, base, nr, invdiffrmed);
-  return w[m - base] - (rmed[m + 1] - rmed[m] + v * dt) * (T)0.5 * rad_slope(w, m, base, nr, invdiffrmed);
-}
-
-template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_trans_rad(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const real dt = (real)s.dt;
-  const double om = s.omega_frame;
-  const int jp = (j + 1 == ns) ? 0 : j + 1;
-  const int base = i - 2;
-  const real* rmed = D.rmed + rb;
-  const real* idr = D.invdiffrmed + rb;
-  const double* rmedd = D.rmed_d + rb;
-  // rings i-2..i+2 (clamped loads; clamped values are never used)
-  real sg[5], vrr[6], rmp[5], rmm[5], en[5], w_tmp[5], w_tmm[5];
-  double tmp_i = 0, tmm_i = 0;
-  for (int q = 0; q < 6; ++q) {
-    int k = base + q;
-    k = k < 0 ? 0 : (k > nr ? nr : k);
-    vrr[q] = AT(D.vrad, k, j);
-  }
-#pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    int k = base + q;
-    k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
-    const real sgk = AT(D.dens, k, j);
-    sg[q] = sgk;
-    rmp[q] = sgk * vrr[q + 1 <= 5 ? q + 1 : 5];
-    rmm[q] = sgk * vrr[q];
-    // specific angular momentum (vtheta + R*Omega)*R (vtheta = vref + residual);
-    // the momenta themselves are kept in FP64 only for ring i
-    const double rk = rmedd[k];
-    const double l0 = (D.vref[rb + k] + rk * om) * rk;
-    const double lp = l0 + (double)AT(D.vtheta, k, jp) * rk, lm = l0 + (double)AT(D.vtheta, k, j) * rk;
-    const double dd = (double)sgk;
-    w_tmp[q] = (real)(dd * lp / (dd + 1e-20));
-    w_tmm[q] = (real)(dd * lm / (dd + 1e-20));
-    if (q == 2) { tmp_i = dd * lp; tmm_i = dd * lm; }
-    en[q] = D.adiabatic ? AT(D.energy, k, j) : (real)0;
-  }
-  // work arrays: quantity / (density + 1e-20)
-  real w_rmp[5], w_rmm[5], w_e[5], w_sg[5];
-#pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    const real den = sg[q] + (real)1e-20;
-    w_rmp[q] = rmp[q] / den; w_rmm[q] = rmm[q] / den; w_e[q] = en[q] / den; w_sg[q] = sg[q] / den;
-  }
-  const real v0 = vrr[2], v1 = vrr[3];   // vr(i), vr(i+1)
-  // density interface values (ComputeStarRad on Dens)
-  const real ds0 = (i == 0) ? (real)0 : rad_star(sg, i, base, nr, v0, dt, rmed, idr);
-  const real ds1 = (i + 1 == nr) ? (real)0 : rad_star(sg, i + 1, base, nr, v1, dt, rmed, idr);
-  const real dth = (real)(2.0 * kPI / (double)ns);
-  const real a0 = dt * dth * D.rinf[rb + i], a1 = dt * dth * D.rsup[rb + i];
-  const real isurf = D.invsurf[rb + i];
-  auto advect = [&](const real* w, real q0) -> real {
-    const real qs0 = (i == 0) ? (real)0 : rad_star(w, i, base, nr, v0, dt, rmed, idr);
-    const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w, i + 1, base, nr, v1, dt, rmed, idr);
-    real varq = a0 * qs0 * ds0 * v0;
-    varq -= a1 * qs1 * ds1 * v1;
-    return q0 + varq * isurf;
-  };
-  // flux divergence of a quantity (its FP32 rounding is far below the
-  // quantity itself); the quantity is added back in FP64 by the caller
-  auto fluxdiv = [&](const real* w) -> real {
-    const real qs0 = (i == 0) ? (real)0 : rad_star(w, i, base, nr, v0, dt, rmed, idr);
-    const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w, i + 1, base, nr, v1, dt, rmed, idr);
-    real varq = a0 * qs0 * ds0 * v0;
-    varq -= a1 * qs1 * ds1 * v1;
-    return varq * isurf;
-  };
-  AT(D.t_rmp, i, j) = advect(w_rmp, rmp[2]);
-  AT(D.t_rmm, i, j) = advect(w_rmm, rmm[2]);
-  if (D.adiabatic) AT(D.t_e, i, j) = advect(w_e, en[2]);
-  double sgx;   // advected density before its FP32 rounding
-  {
-    const real qs0 = (i == 0) ? (real)0 : rad_star(w_sg, i, base, nr, v0, dt, rmed, idr);
-    const real qs1 = (i + 1 == nr) ? (real)0 : rad_star(w_sg, i + 1, base, nr, v1, dt, rmed, idr);
-    real varq = a0 * qs0 * ds0 * v0;
-    varq -= a1 * qs1 * ds1 * v1;
-    const real inc = varq * isurf;
-    sgx = (double)sg[2] + (double)inc;
-    AT(D.t_dens, i, j) = sg[2] + inc;
-    if (i == 0 && D.open_inner) D.lost[c.so + j] = varq;
-  }
-  // angular momenta leave as residuals of density * l0(ring), with the
-  // density taken before rounding: the azimuthal velocity rebuilt from them
-  // (vref + residual / (density * R)) then carries no trace of the density's
-  // storage rounding, as in an exact-arithmetic L / Sigma
-  const double l0i = (D.vref[rb + i] + rmedd[i] * om) * rmedd[i];
-  AT(D.t_tmp, i, j) = (real)(tmp_i + (double)fluxdiv(w_tmp) - sgx * l0i);
-  AT(D.t_tmm, i, j) = (real)(tmm_i + (double)fluxdiv(w_tmm) - sgx * l0i);
-}
-
 // ====================================================== azimuthal transport
 // OneWindTheta (TransportEuler.cu:171-188): ComputeAverageThetaVelocities,
 // ComputeResiduals, ComputeConstantResidual, both QuantitiesAdvection passes
@@ -1019,9 +907,17 @@ __device__ __forceinline__ int fargo_shift_dev<double>(double vmed, double invrm
 }
 
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D) {
+__global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const RingTile* __restrict__ tiles) {
   extern __shared__ unsigned char fgx_smem[];
-  FGX_RING_PROLOGUE
+  const RingTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  if (t.d < 0) return;
+  const DiskConst& c = D.dc[t.d];
+  DiskState& s = D.st[t.d];
+  if (!D.ignore_active && !s.active) return;
+  const int ns = c.nsec, nr = c.nrad;
+  const int i = t.i;
+  const long long fb = c.fo;
+  const int rb = c.ro;
   if (i >= nr) return;
   const int MS = D.max_nsec;
   real* sD1 = reinterpret_cast<real*>(fgx_smem);  // density after the radial pass

@@ -1,0 +1,559 @@
+// transport.hpp -- the two transport sweeps of the FARGO step, register
+// resident.
+//
+// Radial pass (OneWindRad, TransportEuler.cu:54-73 with LRMomentaKernel,
+// StarRadKernel/2, DivisePolarGridKernel and VanLeerRadialKernel,
+// Kernels.cu:883-947, 1127-1154): a thread owns one sector of a strip of
+// rings and walks outwards with a three-ring window, so every ring is
+// loaded once per strip (plus a four-ring halo), every van Leer slope and
+// every interface flux is formed once and shared by the two cells it
+// separates.
+//
+// Azimuthal pass (OneWindTheta, TransportEuler.cu:171-188, with the
+// ComputeAverageThetaVelocities, ComputeResiduals, ComputeConstantResidual,
+// StarTheta/2, VanLeerTheta, AdvectSHIFT and the azimuthal half of
+// ComputeVelocities kernels, Kernels.cu:1177-1306): one wave64 per ring; lane
+// l holds the K contiguous sectors [l*K, l*K+K) in registers through both
+// advection passes of all quantities, chunk-edge neighbours come from the
+// adjacent lanes by cross-lane reads, and global traffic goes through a
+// padded LDS transpose so that loads and stores stay coalesced.  The integer
+// shift is applied on the way out of LDS.
+#pragma once
+#include "kernels.hpp"
+
+
+namespace fgx {
+
+__device__ __forceinline__ float vl_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double vl_rcp(double x) { return 1.0 / x; }
+
+// whole-wave rotations by one lane (DPP wave_ror:1 / wave_rol:1): lane l
+// receives lane l-1 (lane 0 receives lane 63), resp. lane l+1 (lane 63
+// receives lane 0)
+__device__ __forceinline__ int dpp_ror1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x13C, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_rol1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ float lane_from_left(float x) { return __int_as_float(dpp_ror1(__float_as_int(x))); }
+__device__ __forceinline__ float lane_from_right(float x) { return __int_as_float(dpp_rol1(__float_as_int(x))); }
+__device__ __forceinline__ float lane_read(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+__device__ __forceinline__ double lane_from_left(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_ror1((int)(b & 0xffffffffLL)), hi = dpp_ror1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_from_right(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_rol1((int)(b & 0xffffffffLL)), hi = dpp_rol1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_read(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// ============================================================ radial sweep
+template <class real>
+struct RadRing {
+  real q[7];      // 0 density (for DensStar), 1..6 work = quantity / (density + 1e-20):
+                  // density, RadMomP, RadMomM, ThetaMomP, ThetaMomM, energy
+  real sg, vr0, vr1, e;
+  double tp, tm;  // angular momenta in FP64 (density * (vref + residual + R*Omega) * R)
+};
+
+// the disk's fields seen from its first element; offsets in 32 bits
+template <class real>
+struct RadFields {
+  const real *dens, *vrad, *vtheta, *energy;
+  const double *rmed_d, *vref;
+};
+
+template <class real>
+__device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, double om,
+                                         bool ad, RadRing<real>& r) {
+  k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);   // clamped rings only feed zero slopes / zero fluxes
+  const int o = k * ns;
+  const real sg = f.dens[o + j];
+  const real vr0 = f.vrad[o + j], vr1 = f.vrad[o + ns + j];
+  const double rk = f.rmed_d[k];
+  const double l0 = (f.vref[k] + rk * om) * rk;
+  const double lp = l0 + (double)f.vtheta[o + jp] * rk, lm = l0 + (double)f.vtheta[o + j] * rk;
+  // DivisePolarGrid: quantity / (density + 1e-20), as a product with one
+  // reciprocal per cell; the density's own work is exactly 1 wherever the
+  // 1e-20 is below its rounding (density >= 1e-12), as in the quotient
+  const real inv = vl_rcp(sg + (real)1e-20);
+  const real w1 = (sg >= (real)1e-12) ? (real)1 : sg / (sg + (real)1e-20);
+  r.sg = sg; r.vr0 = vr0; r.vr1 = vr1;
+  r.q[0] = sg;
+  r.q[1] = w1;
+  r.q[2] = (sg * vr1) * inv;
+  r.q[3] = (sg * vr0) * inv;
+  const double dd = (double)sg;
+  r.tp = dd * lp; r.tm = dd * lm;
+  if constexpr (sizeof(real) == 4) {
+    // (density * l) / (density + 1e-20): l times the density's work
+    r.q[4] = (real)lp * w1; r.q[5] = (real)lm * w1;
+  } else {
+    r.q[4] = (real)(r.tp / (dd + 1e-20)); r.q[5] = (real)(r.tm / (dd + 1e-20));
+  }
+  r.e = ad ? f.energy[o + j] : (real)0;
+  r.q[6] = r.e * inv;
+}
+
+// van Leer slope of ring k from rings k-1, k, k+1 (StarRadKernel); zero on
+// the first and last ring
+template <class real>
+__device__ __forceinline__ real rad_vl(real wm, real w0, real wp, real idm, real idp) {
+  const real dqm = (w0 - wm) * idm;
+  const real dqp = (wp - w0) * idp;
+  return (dqp * dqm > 0) ? (real)2 * dqp * dqm * vl_rcp(dqp + dqm) : (real)0;
+}
+
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
+  const CellTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  if (t.d < 0) return;
+  const DiskConst& c = D.dc[t.d];
+  const DiskState& s = D.st[t.d];
+  if (!D.ignore_active && !s.active) return;
+  const int ns = c.nsec, nr = c.nrad;
+  const int j = t.jb * FGX_TPB + (int)threadIdx.x;
+  if (j >= ns) return;
+  const long long fb = c.fo;
+  const int rb = c.ro;
+  const int i0 = t.i, i1 = (t.i + rs < nr) ? t.i + rs : nr;
+  const int jp = (j + 1 == ns) ? 0 : j + 1;
+  const bool ad = D.adiabatic;
+  const double om = s.omega_frame;
+  const real dt = (real)s.dt;
+  const real dth = (real)(2.0 * kPI / (double)ns);
+  const real* rmed = D.rmed + rb;
+  const real* idr = D.invdiffrmed + rb;
+  const int nq = ad ? 7 : 6;
+  const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0), D.rmed_d + rb,
+                          D.vref + rb};
+  real* const o_dens = D.t_dens + fb;
+  real* const o_rmp = D.t_rmp + fb;
+  real* const o_rmm = D.t_rmm + fb;
+  real* const o_tmp = D.t_tmp + fb;
+  real* const o_tmm = D.t_tmm + fb;
+  real* const o_e = D.t_e + (ad ? fb : 0);
+
+  auto slope_at = [&](int k, const RadRing<real>& m, const RadRing<real>& z, const RadRing<real>& p, int q) -> real {
+    if (k <= 0 || k >= nr - 1) return (real)0;
+    return rad_vl(m.q[q], z.q[q], p.q[q], idr[k], idr[k + 1]);
+  };
+  // prologue: rings i0-2 .. i0+1
+  RadRing<real> rm2, rm1, ra, rb1;
+  rad_load(F, ns, nr, i0 - 2, j, jp, om, ad, rm2);
+  rad_load(F, ns, nr, i0 - 1, j, jp, om, ad, rm1);
+  rad_load(F, ns, nr, i0, j, jp, om, ad, ra);
+  rad_load(F, ns, nr, i0 + 1, j, jp, om, ad, rb1);
+  real slA[7], slM[7], flux[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    if (q >= nq) { slA[q] = slM[q] = (real)0; continue; }
+    slM[q] = slope_at(i0 - 1, rm2, rm1, ra, q);
+    slA[q] = slope_at(i0, rm1, ra, rb1, q);
+  }
+  // flux through interface i0 (zero at the inner edge)
+  {
+    const real v = ra.vr0;
+    real st[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      if (q >= nq) { st[q] = 0; continue; }
+      st[q] = (v > 0) ? rm1.q[q] + (rmed[i0] - rmed[i0 - 1 < 0 ? 0 : i0 - 1] - v * dt) * (real)0.5 * slM[q]
+                      : ra.q[q] - (rmed[i0 + 1] - rmed[i0] + v * dt) * (real)0.5 * slA[q];
+    }
+    const real a = dt * dth * D.rinf[rb + i0];
+    flux[0] = 0;
+#pragma unroll
+    for (int q = 1; q < 7; ++q) flux[q] = (i0 == 0 || q >= nq) ? (real)0 : a * st[q] * st[0] * v;
+  }
+  const real* isurf = D.invsurf + rb;
+  for (int i = i0; i < i1; ++i) {
+    RadRing<real> rc;
+    rad_load(F, ns, nr, i + 2, j, jp, om, ad, rc);
+    // interface i+1 (zero flux at the outer edge)
+    const real v = rb1.vr0;
+    const bool edge = (i + 1 == nr);
+    const real a = edge ? (real)0 : dt * dth * D.rinf[rb + i + 1];
+    real fnew[7], slB[7];
+    real st0 = 0;
+    fnew[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      if (q >= nq) { slB[q] = 0; fnew[q] = 0; continue; }
+      slB[q] = slope_at(i + 1, ra, rb1, rc, q);
+      const real st = (v > 0) ? ra.q[q] + (rmed[i + 1] - rmed[i] - v * dt) * (real)0.5 * slA[q]
+                              : rb1.q[q] - (rmed[i + 2 <= nr ? i + 2 : nr] - rmed[i + 1] + v * dt) * (real)0.5 * slB[q];
+      if (q == 0) st0 = st;
+      else fnew[q] = edge ? (real)0 : a * st * st0 * v;
+    }
+    // cell i: quantity + (flux in - flux out) / Surf
+    const real is = isurf[i];
+    const real inc = (flux[1] - fnew[1]) * is;
+    const real sg = ra.sg;
+    const int oc = i * ns + j;
+    o_dens[oc] = sg + inc;
+    if (i == 0 && D.open_inner) D.lost[c.so + j] = flux[1] - fnew[1];
+    o_rmp[oc] = sg * ra.vr1 + (flux[2] - fnew[2]) * is;
+    o_rmm[oc] = sg * ra.vr0 + (flux[3] - fnew[3]) * is;
+    // angular momenta leave as residuals of density * l0(ring), with the
+    // density taken before its FP32 rounding, so the azimuthal velocity
+    // rebuilt from them carries no trace of that rounding
+    const double sgx = (double)sg + (double)inc;
+    const double rd = F.rmed_d[i];
+    const double l0i = (F.vref[i] + rd * om) * rd;
+    o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - sgx * l0i);
+    o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - sgx * l0i);
+    if (ad) o_e[oc] = ra.e + (flux[6] - fnew[6]) * is;
+    // slide the window
+#pragma unroll
+    for (int q = 0; q < 7; ++q) { flux[q] = fnew[q]; slA[q] = slB[q]; }
+    ra = rb1;
+    rb1 = rc;
+  }
+}
+
+// ========================================================= azimuthal sweep
+// K contiguous values at p (aligned to the vector width) into registers
+template <class real, int K>
+__device__ __forceinline__ void ld_chunk(const real* __restrict__ p, real (&v)[K]) {
+  if constexpr (sizeof(real) == 4 && K % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < K / 4; ++q) {
+      const float4 x = reinterpret_cast<const float4*>(p)[q];
+      v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+    }
+  } else if constexpr (sizeof(real) == 4 && K % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+      const float2 x = reinterpret_cast<const float2*>(p)[q];
+      v[2 * q] = x.x; v[2 * q + 1] = x.y;
+    }
+  } else if constexpr (sizeof(real) == 8 && K % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+      const double2 x = reinterpret_cast<const double2*>(p)[q];
+      v[2 * q] = x.x; v[2 * q + 1] = x.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = p[k];
+  }
+}
+
+// A ring is split into W arcs of T*K sectors, one wave64 per arc (T = 64 when
+// W > 1, T <= 64 when W == 1); lane l of wave w holds the K contiguous sectors
+// starting at (w*64 + l)*K.  Neighbours inside a wave come from DPP lane
+// rotations, across arcs through two LDS words per wave edge (double
+// buffered, one barrier per exchange).  Every disk's field offset is a
+// multiple of 64 elements, so each lane's chunk is aligned for vector loads.
+template <class real, int K>
+__global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real> D, const RingTile* __restrict__ tiles,
+                                                                       int W) {
+  __shared__ real ex[2][4][16];
+  __shared__ double redw[16];
+  const RingTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  if (t.d < 0) return;
+  const DiskConst& c = D.dc[t.d];
+  const DiskState& s = D.st[t.d];
+  if (!D.ignore_active && !s.active) return;
+  const int ns = c.nsec, i = t.i, rb = c.ro;
+  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int T = (W > 1) ? 64 : ns / K;
+  const bool on = lane < T;
+  const int wl = (w == 0) ? W - 1 : w - 1, wr = (w + 1 == W) ? 0 : w + 1;
+  // lanes past the ring read its first chunk (never stored)
+  const long long row = c.fo + (long long)i * ns;
+  const int j0 = on ? (w * 64 + lane) * K : 0;   // first sector of this lane
+  const long long my = row + j0;
+  const bool ad = D.adiabatic;
+  int xb = 0;
+
+  // neighbour lanes along the ring; in(): values of this lane's left and
+  // right neighbours for up to three exchanged quantities at once
+  auto fix_single = [&](real& l1, real& l2, real& r1, real x1, real x2, real xr) {
+    // W == 1 and T < 64: the wrap comes from lane T-1, resp. lane 0
+    if (T < 64) {
+      const real e1 = lane_read(x1, T - 1), e2 = lane_read(x2, T - 1), er = lane_read(xr, 0);
+      if (lane == 0) { l1 = e1; l2 = e2; }
+      if (lane == T - 1) r1 = er;
+    }
+  };
+  // left neighbours of x1 (this lane's X[K-1]) and x2 (X[K-2]), right
+  // neighbour of xr (X[0])
+  auto halo3 = [&](real x1, real x2, real xr, real& l1, real& l2, real& r1) {
+    l1 = lane_from_left(x1);
+    l2 = lane_from_left(x2);
+    r1 = lane_from_right(xr);
+    if (W > 1) {
+      if (lane == 63) { ex[xb][0][w] = x1; ex[xb][1][w] = x2; }
+      if (lane == 0) ex[xb][2][w] = xr;
+      __syncthreads();
+      if (lane == 0) { l1 = ex[xb][0][wl]; l2 = ex[xb][1][wl]; }
+      if (lane == 63) r1 = ex[xb][2][wr];
+      xb ^= 1;
+    } else {
+      fix_single(l1, l2, r1, x1, x2, xr);
+    }
+  };
+  // left neighbour of a value that is already a left-neighbour value (K == 1,
+  // W == 1)
+  auto left_of = [&](real x) -> real {
+    real r = lane_from_left(x);
+    if (T < 64) {
+      const real e = lane_read(x, T - 1);
+      if (lane == 0) r = e;
+    }
+    return r;
+  };
+  auto right1 = [&](real xr) -> real {
+    real r = lane_from_right(xr);
+    if (W > 1) {
+      if (lane == 0) ex[xb][3][w] = xr;
+      __syncthreads();
+      if (lane == 63) r = ex[xb][3][wr];
+      xb ^= 1;
+    } else if (T < 64) {
+      const real e = lane_read(xr, 0);
+      if (lane == T - 1) r = e;
+    }
+    return r;
+  };
+  auto left2 = [&](real x1, real x2, real& l1, real& l2) {
+    l1 = lane_from_left(x1);
+    l2 = lane_from_left(x2);
+    if (W > 1) {
+      if (lane == 63) { ex[xb][0][w] = x1; ex[xb][1][w] = x2; }
+      __syncthreads();
+      if (lane == 0) { l1 = ex[xb][0][wl]; l2 = ex[xb][1][wl]; }
+      xb ^= 1;
+    } else if (T < 64) {
+      const real e1 = lane_read(x1, T - 1), e2 = lane_read(x2, T - 1);
+      if (lane == 0) { l1 = e1; l2 = e2; }
+    }
+  };
+
+  // everything this ring needs first is requested at once
+  real V[K], Dn[K], Qn[K];
+  ld_chunk<real, K>(D.vtheta + my, V);
+  ld_chunk<real, K>(D.t_dens + my, Dn);
+  ld_chunk<real, K>(D.t_rmp + my, Qn);
+
+  // ring mean of vtheta (FP64 sum of the residuals)
+  double acc = 0;
+  if (on) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc += (double)V[k];
+  }
+  acc = wave_sum(acc);
+  if (W > 1) {
+    if (lane == 0) redw[w] = acc;
+    __syncthreads();
+    acc = 0;
+    for (int q = 0; q < W; ++q) acc += redw[q];
+  }
+  const double mres = acc / (double)ns;
+  const real vmed = (real)(D.vref[rb + i] + mres);
+  const real rmed = D.rmed[rb + i], irm = D.invrmed[rb + i];
+  real ntilde, nround;
+  const int nsh = fargo_shift_dev<real>(vmed, irm, s.dt, ns, ntilde, nround);
+  const real dt = (real)s.dt;
+  const real invdt = (real)1 / dt;
+  const real dpinvns = (real)(2.0 * kPI / (double)ns);
+  const real vconst = (ntilde - nround) * rmed * invdt * dpinvns;
+  const real v2 = D.fast_transport ? vconst : (real)0;
+  if (threadIdx.x == 0) { D.vmed[rb + i] = vmed; D.nshift[rb + i] = nsh; }
+  int nsm = nsh % ns;
+  if (nsm < 0) nsm += ns;
+  const real dxt = D.dxtheta[rb + i];
+  const real invdxt = (real)1 / dxt;
+  const real dxrad = (D.rsup[rb + i] - D.rinf[rb + i]) * dt;
+  const real invsurf = (real)1 / D.surf[rb + i];
+  // AdvectSHIFTKernel: out[j] = in[j - Nshift], i.e. sector j lands at
+  // j + Nshift
+  const int o0 = [&] { int o = j0 + nsm; return o >= ns ? o - ns : o; }();
+  auto store_shift = [&](real* dst, const real(&v)[K]) {
+    if (!on) return;
+    int o = o0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      dst[row + o] = v[k];
+      o = (o + 1 == ns) ? 0 : o + 1;
+    }
+  };
+
+  // van Leer slope without the 1/dxt factor (folded into the coefficients)
+  auto slope = [&](real xm, real x0, real xp) -> real {
+    const real dqm = x0 - xm, dqp = xp - x0;
+    return (dqp * dqm > 0) ? dqp * dqm * vl_rcp(dqp + dqm) : (real)0;
+  };
+  // interface values (StarThetaKernel2) of X; per-sector upwind bits and
+  // coefficients ((dxt - ksi) if ksi > 0, else (dxt + ksi)) / dxt
+  auto star = [&](const real(&X)[K], unsigned pos, const real(&cf)[K], real(&out)[K]) {
+    real xl1, xl2, xr1;
+    halo3(X[K - 1], (K >= 2) ? X[K - 2] : X[0], X[0], xl1, xl2, xr1);
+    if constexpr (K == 1) xl2 = left_of(xl1);   // W == 1 only
+    real slp = slope(xl2, xl1, X[0]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const real xm = (k == 0) ? xl1 : X[k - 1];
+      const real xp = (k == K - 1) ? xr1 : X[k + 1];
+      const real sl = slope(xm, X[k], xp);
+      out[k] = ((pos >> k) & 1u) ? xm + cf[k] * slp : X[k] - cf[k] * sl;
+      slp = sl;
+    }
+  };
+  // the same with one velocity for the whole ring
+  auto star_u = [&](const real(&X)[K], bool pos, real cf, real(&out)[K]) {
+    real xl1, xl2, xr1;
+    halo3(X[K - 1], (K >= 2) ? X[K - 2] : X[0], X[0], xl1, xl2, xr1);
+    if constexpr (K == 1) xl2 = left_of(xl1);
+    real slp = slope(xl2, xl1, X[0]);
+    if (pos) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const real xm = (k == 0) ? xl1 : X[k - 1];
+        const real xp = (k == K - 1) ? xr1 : X[k + 1];
+        const real sl = slope(xm, X[k], xp);
+        out[k] = xm + cf * slp;
+        slp = sl;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const real xm = (k == 0) ? xl1 : X[k - 1];
+        const real xp = (k == K - 1) ? xr1 : X[k + 1];
+        out[k] = X[k] - cf * slope(xm, X[k], xp);
+      }
+    }
+  };
+  // flux divergence of one VanLeerTheta pass: F = star(work) * SV with SV =
+  // dxrad * DensStar * v / Surf
+  auto apply = [&](real(&X)[K], real(&F)[K], const real(&SV)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) F[k] *= SV[k];
+    const real fr = right1(F[0]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) X[k] += F[k] - ((k == K - 1) ? fr : F[k + 1]);
+  };
+  // pass 1 (residual velocity), pass 2 (uniform fractional velocity) of one
+  // quantity; work = X / (DensInt + 1e-20)
+  unsigned pos1 = 0;
+  real cf1[K], SV1[K], invD1[K], SV2[K], invD2[K];
+  const real ksi2 = v2 * dt;
+  const bool pos2 = ksi2 > 0;
+  const real cf2 = (pos2 ? dxt - ksi2 : dxt + ksi2) * invdxt;
+  auto both = [&](real(&X)[K]) {
+    real A[K], F[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) A[k] = X[k] * invD1[k];
+    star(A, pos1, cf1, F);
+    apply(X, F, SV1);
+#pragma unroll
+    for (int k = 0; k < K; ++k) A[k] = X[k] * invD2[k];
+    star_u(A, pos2, cf2, F);
+    apply(X, F, SV2);
+  };
+
+  // pass-1 velocity (ComputeResiduals); STANDARD transport folds the
+  // constant part in and the second pass moves nothing
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    real v = (real)((double)V[k] - mres);
+    if (!D.fast_transport) v = vconst + v;
+    V[k] = v;
+    const real ksi = v * dt;
+    pos1 |= (ksi > 0 ? 1u : 0u) << k;
+    cf1[k] = ((ksi > 0) ? dxt - ksi : dxt + ksi) * invdxt;
+  }
+  // density: DensStar and pass 1
+  star(Dn, pos1, cf1, SV1);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    SV1[k] = dxrad * SV1[k] * V[k] * invsurf;
+    invD1[k] = (real)1 / (Dn[k] + (real)1e-20);
+  }
+  {
+    real A[K], F[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) A[k] = Dn[k] * invD1[k];
+    star(A, pos1, cf1, F);
+    apply(Dn, F, SV1);
+  }
+  // pass-2 coefficients from the density after pass 1
+  star_u(Dn, pos2, cf2, SV2);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    SV2[k] = dxrad * SV2[k] * v2 * invsurf;
+    invD2[k] = (real)1 / (Dn[k] + (real)1e-20);
+  }
+  // quantities in the order RadMomP, RadMomM, [energy], ThetaMomP (held),
+  // ThetaMomM (left in Qn), each loaded while the previous one is advected
+  const real* srcs[5] = {D.t_rmp, D.t_rmm, ad ? D.t_e : D.t_tmp, ad ? D.t_tmp : D.t_tmm, D.t_tmm};
+  real* dsts[3] = {D.t_rmp, D.t_rmm, D.energy};
+  const int nq = ad ? 5 : 4, oh = nq - 2;   // oh: ThetaMomP
+  real Q[K], H[K];
+#pragma nounroll
+  for (int o = 0; o < nq; ++o) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) Q[k] = Qn[k];
+    if (o + 1 < nq) ld_chunk<real, K>(srcs[o + 1] + my, Qn);
+    both(Q);
+    if (o < oh) {
+      store_shift(dsts[o], Q);
+    } else if (o == oh) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) H[k] = Q[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) Qn[k] = Q[k];
+    }
+  }
+  // density, pass 2
+  {
+    real A[K], F[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) A[k] = Dn[k] * invD2[k];
+    star_u(A, pos2, cf2, F);
+    apply(Dn, F, SV2);
+  }
+  store_shift(D.dens, Dn);
+  // azimuthal velocity of ComputeVelocitiesKernel at the face j-1/2:
+  // (TMP[j-1] + TMM[j]) / (Sigma[j] + Sigma[j-1] + 1e-15) / R - R*Omega; with
+  // the momenta stored as residuals of density * l0 this is vref + the
+  // residual below, exactly
+  const double rd = D.rmed_d[rb + i];
+  const double l0 = (D.vref[rb + i] + rd * s.omega_frame) * rd;
+  real hl, dl;
+  left2(H[K - 1], Dn[K - 1], hl, dl);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double hm = (double)((k == 0) ? hl : H[k - 1]);
+    const double dm = (double)((k == 0) ? dl : Dn[k - 1]);
+    V[k] = (real)((hm + (double)Qn[k] - l0 * 1e-15) / ((double)Dn[k] + dm + 1e-15) / rd);
+  }
+  store_shift(D.vtheta, V);
+}
+
+// split of a ring of nsec sectors: W arcs of 64*K sectors (K in {3, 2, 4},
+// 2 <= W <= 16), else one wave of T <= 64 lanes with nsec = T*K; K = 0 when
+// nothing fits (shared-memory kernel)
+template <class real>
+inline void theta_split(int nsec, int& K, int& W) {
+  static const int ka[] = {3, 2, 4};
+  for (int k : ka)
+    if (nsec % (64 * k) == 0 && nsec / (64 * k) >= 2 && nsec / (64 * k) <= 16) { K = k; W = nsec / (64 * k); return; }
+  static const int kf[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16};
+  static const int kd[] = {1, 2, 3, 4, 5, 6, 8};
+  const int* ks = sizeof(real) == 4 ? kf : kd;
+  const int n = sizeof(real) == 4 ? 10 : 7;
+  W = 1;
+  for (int a = 0; a < n; ++a)
+    if (nsec % ks[a] == 0 && nsec / ks[a] <= 64) { K = ks[a]; return; }
+  K = 0;
+}
+
+}  // namespace fgx

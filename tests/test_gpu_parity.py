@@ -77,6 +77,24 @@ def test_ten_fixed_steps(gpu, case, precision):
     assert abs(disk.info().physical_time - orc.info()["time"]) < 1e-12
 
 
+@pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("nsec", [384, 512, 768, 1536, 1030])
+def test_wide_rings(gpu, nsec, precision):
+    """Rings split over several waves (384: 2 arcs, 512 and 768: 4, 1536: 8)
+    and a sector count no split fits (1030, shared-memory kernel); CFL-driven
+    substeps, which must agree in number."""
+    par, cfg = configs.plain_disk(nrad=16, nsec=nsec)
+    disk, orc, names = pair(par, cfg, precision, DT=DT_SMALL)
+    for _ in range(4):
+        disk.step(1)
+        orc.step(1)
+        assert disk.info().substeps == orc.info()["substeps"]
+    err = compare(disk, orc, names)
+    print(nsec, precision, err)
+    for n, e in err.items():
+        assert e < TOL[precision], (nsec, n, e)
+
+
 def test_cfl_driven_substep_counts(gpu):
     """Without a forced substep count, both sides must pick the same CFL
     counts (integer) and stay within tolerance."""
