@@ -24,7 +24,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 
 HIP_SOURCES = ["engine.hip"]
-CXX_SOURCES = ["params.cpp", "setup.cpp", "capi.cpp", "output.cpp"]
+CXX_SOURCES = ["params.cpp", "setup.cpp", "capi.cpp", "output.cpp", "sg.cpp"]
 
 
 def _run(cmd):

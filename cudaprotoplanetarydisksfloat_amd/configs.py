@@ -12,6 +12,8 @@ it, so every file also runs under the reference.
 - large_disk:    512 x 1536 adiabatic disk (gamma 1.4), log r in [0.3, 3.0],
                  alpha = 1e-3, flaring 0.25, Sigma ~ r^-1, migrating
                  q = 3e-4 planet (RK5), evanescent boundaries
+- sg_disk:       256 x 768 self-gravitating disk, log r in [0.5, 4.0], Sigma ~ r^-1.5,
+                 alpha = 1e-4, q = 1e-3 planet, +-1e-3 noise
 - batch_disks:   64 isothermal disks with NRAD in {128,192,256}, NSEC in
                  {384,512,768} and parameters drawn from a seed
 """
@@ -55,6 +57,25 @@ def large_disk(nrad=512, nsec=1536, noise=1e-4, disk_id=0, substeps=0):
     if substeps:
         keys["Substeps"] = substeps
     cfg = "Planet\t1.0\t3e-4\t0.0\tYES\tYES\n"
+    return _par(keys), cfg
+
+
+def sg_disk(nrad=256, nsec=768, noise=1e-3, disk_id=0, substeps=0, sigma0=6.155e-4):
+    """Self-gravitating disk: log r in [0.5, 4.0], Sigma ~ r^-1.5, h = 0.05,
+    alpha = 1e-4, one q = 1e-3 planet at a = 1.  The default Sigma0 gives a
+    disk mass of 0.01 inside [0.5, 4] (Toomre Q ~ 26 at r = 1)."""
+    keys = {
+        "AspectRatio": 0.05, "Sigma0": sigma0, "AlphaViscosity": 1e-4, "SigmaSlope": 1.5,
+        "FlaringIndex": 0.0, "PlanetConfig": "in/Jup.cfg", "ThicknessSmoothing": 0.6,
+        "ForcedCircular": "YES", "Transport": "FARGO", "InnerBoundary": "RIGID", "Disk": "YES",
+        "OmegaFrame": 0.0, "Frame": "FIXED", "IndirectTerm": "YES", "Nrad": nrad, "Nsec": nsec,
+        "Rmin": 0.5, "Rmax": 4.0, "RadialSpacing": "LOGARITHM", "Ntot": 1000, "Ninterm": 100,
+        "DT": DT_ORBIT_20, "OutputDir": "out/", "Adiabatic": "NO", "SelfGravity": "YES",
+        "NoiseAmplitude": noise, "NoiseSeed": BASE_SEED, "DiskId": disk_id,
+    }
+    if substeps:
+        keys["Substeps"] = substeps
+    cfg = "Jupiter\t1.0\t0.001\t0.0\tNO\tNO\n"
     return _par(keys), cfg
 
 

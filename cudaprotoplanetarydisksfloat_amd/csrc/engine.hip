@@ -23,6 +23,7 @@
 #include "fgx_internal.h"
 #include "kernels.hpp"
 #include "transport.hpp"
+#include "sg.hpp"
 
 namespace fgx {
 
@@ -64,7 +65,8 @@ struct Batch : BatchBase {
   // radial strips (d, first ring, sector block) and the azimuthal ring lists:
   // one per lane width K of the register-resident kernel, plus the rings
   // whose sector count fits none (shared-memory kernel)
-  int n_stiles = 0, strip_rings = 16;
+  int n_stiles = 0, strip_rings = 16, n_sgctiles = 0;
+  size_t sg_shm_row = 0, sg_shm_col = 0;
   const CellTile* stiles = nullptr;
   struct ThetaGroup { int k, w, n; const RingTile* tiles; };
   std::vector<ThetaGroup> theta_groups;
@@ -182,7 +184,6 @@ struct Batch : BatchBase {
           p.ext_fix_taurp_corner != P0.ext_fix_taurp_corner || p.ext_fix_pp_distance != P0.ext_fix_pp_distance ||
           p.is_disk != P0.is_disk)
         return set_error(FGX_EPARAM, "disk %d of the batch differs from disk 0 in a mode flag", d);
-      if (p.self_gravity) return set_error(FGX_EPARAM, "self-gravity is not available in this build");
       if (p.corotating && pls[d].nb == 0) return set_error(FGX_EPARAM, "a corotating frame needs a planet");
       if (!p.is_disk) return set_error(FGX_EPARAM, "DISK NO (planets only) is not supported");
     }
@@ -190,9 +191,18 @@ struct Batch : BatchBase {
     HIPCHK(hipHostMalloc((void**)&h_summary, 4 * sizeof(int)));
     hd.resize(nd);
     for (int d = 0; d < nd; ++d) LCHK(setup_grid_and_density(ps[d], pls[d], hd[d]));
-    for (int d = 0; d < nd; ++d) setup_velocities(hd[d], {});
+    // self-gravity set-up (FP64, host): kernel transforms, the initial radial
+    // SG profile for the planets' and the gas's circular velocities
+    std::vector<std::vector<double>> khr(nd), kht(nd);
+    for (int d = 0; d < nd; ++d) {
+      std::vector<double> axisg;
+      if (hd[d].P.self_gravity) LCHK(sg_host_setup(hd[d], khr[d], kht[d], axisg));
+      setup_velocities(hd[d], axisg);
+    }
     // layout
     dch.resize(nd);
+    size_t SGW = 0, SGK = 0, STW = 0;
+    std::vector<RingTile> sgct;
     std::vector<CellTile> ct;
     std::vector<RingTile> rt;
     for (int d = 0; d < nd; ++d) {
@@ -229,6 +239,17 @@ struct Batch : BatchBase {
       c.substeps_forced = p.ext_substeps;
       c.visc_dynamic = (p.viscosity_alpha || p.lambdadoubling != 0.0f) ? 1 : 0;
       visc_dynamic |= c.visc_dynamic;
+      if (p.self_gravity) {
+        c.sg_eps = h.sg_eps; c.sg_rstep = h.sg_rstep; c.sg_tstep = h.sg_tstep;
+        c.sgw_o = (long long)SGW; c.sgk_o = (long long)SGK;
+        c.tw_ns = (int)STW; c.tw_nr = (int)(STW + h.nsec);
+        sg_factor(h.nsec, c.fac_ns, &c.nf_ns);
+        sg_factor(2 * h.nrad, c.fac_nr, &c.nf_nr);
+        SGW += (size_t)h.nrad * h.nsec;
+        SGK += (size_t)2 * h.nrad * h.nsec;
+        STW += (size_t)h.nsec + 2 * h.nrad;
+        for (int k = 0; 2 * k <= h.nsec; ++k) sgct.push_back(RingTile{d, k});   // column pairs (k, nsec-k)
+      }
     }
     alpha = P0.viscosity_alpha;
     // strip height: as tall as possible while the batch still yields about
@@ -265,7 +286,31 @@ struct Batch : BatchBase {
     LCHK(f(&D.divv, ad)); LCHK(f(&D.qplus, ad)); LCHK(f(&D.energyint, ad)); LCHK(f(&D.pressure, true));
     LCHK(f(&D.t_dens, true)); LCHK(f(&D.t_rmp, true)); LCHK(f(&D.t_rmm, true));
     LCHK(f(&D.t_tmp, true)); LCHK(f(&D.t_tmm, true)); LCHK(f(&D.t_e, ad)); LCHK(f(&D.work, true));
-    LCHK(f(&D.sg_accr, false)); LCHK(f(&D.sg_acct, false));
+    const int sg = P0.self_gravity;
+    LCHK(f(&D.sg_accr, sg)); LCHK(f(&D.sg_acct, sg));
+    if (sg) {
+      while (sgct.size() % 8) sgct.push_back(RingTile{-1, 0});
+      n_sgctiles = (int)sgct.size();
+      cxd *w, *kr, *kt, *tw;
+      RingTile* sgt;
+      LCHK(dalloc(&w, SGW)); LCHK(dalloc(&kr, SGK)); LCHK(dalloc(&kt, SGK)); LCHK(dalloc(&tw, STW));
+      LCHK(dalloc(&D.axisg, R)); LCHK(dalloc(&sgt, sgct.size()));
+      D.sgw = w; D.sgkr = kr; D.sgkt = kt; D.sgtw = tw; D.sg_ctiles = sgt;
+      HIPCHK(hipMemcpyAsync((void*)sgt, sgct.data(), sgct.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
+      std::vector<cxd> twh(STW);
+      for (int d = 0; d < nd; ++d) {
+        const DiskConst& c = dch[d];
+        const int lens[2] = {c.nsec, 2 * c.nrad}, offs[2] = {c.tw_ns, c.tw_nr};
+        for (int q = 0; q < 2; ++q)
+          for (int m = 0; m < lens[q]; ++m) {
+            const double a = 2.0 * kPI * (double)m / (double)lens[q];
+            twh[offs[q] + m] = cxd{std::cos(a), -std::sin(a)};   // exp(-2 pi i m / n)
+          }
+        HIPCHK(hipMemcpy((void*)(kr + c.sgk_o), khr[d].data(), khr[d].size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)(kt + c.sgk_o), kht[d].data(), kht[d].size() * sizeof(double), hipMemcpyHostToDevice));
+      }
+      HIPCHK(hipMemcpy((void*)tw, twh.data(), twh.size() * sizeof(cxd), hipMemcpyHostToDevice));
+    }
     real** rings[] = {&D.rinf, &D.rsup, &D.rmed, &D.surf, &D.invsurf, &D.invrmed, &D.invrinf, &D.invdiffrmed,
                       &D.invdiffrsup, &D.dxtheta, &D.supp_torque, &D.sigmamed, &D.energymed, &D.coolingtime,
                       &D.qplusmed, &D.visc, &D.cs_ring, &D.csmean, &D.vmed, &D.newdt};
@@ -356,18 +401,41 @@ struct Batch : BatchBase {
     if (ad) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D));
     D.ignore_active = 0;
     // LDS of the azimuthal transport kernel
-    theta_shmem = (size_t)FGX_THETA_ARRAYS * max_nsec * sizeof(real);
+    int fb_nsec = 0;   // widest ring of the shared-memory azimuthal kernel
+    for (int d = 0; d < nd; ++d) {
+      int K = 0, W = 0;
+      theta_split<real>(hd[d].nsec, K, W);
+      if (K == 0) fb_nsec = std::max(fb_nsec, hd[d].nsec);
+    }
+    theta_shmem = (size_t)FGX_THETA_ARRAYS * fb_nsec * sizeof(real);
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     size_t lds_max = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : prop.sharedMemPerBlock;
     if (theta_shmem + 1024 > lds_max)
-      return set_error(FGX_EPARAM, "NSEC=%d needs %zu B of LDS per ring in this precision; the device has %zu", max_nsec,
+      return set_error(FGX_EPARAM, "NSEC=%d needs %zu B of LDS per ring in this precision; the device has %zu", fb_nsec,
                        theta_shmem, lds_max);
     if (theta_shmem > 48 * 1024)
       HIPCHK(hipFuncSetAttribute((const void*)k_trans_theta<real>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)theta_shmem));
+    if (sg) {
+      // self-gravity transforms: two rows of nsec, four columns of 2*nrad (FP64 complex)
+      sg_shm_row = 2 * (size_t)max_nsec * sizeof(cxd);
+      sg_shm_col = 8 * (size_t)max_nrad * sizeof(cxd);
+      if (sg_shm_row + 1024 > lds_max || sg_shm_col + 1024 > lds_max)
+        return set_error(FGX_EPARAM, "self-gravity transforms of NSEC=%d, 2*NRAD=%d exceed the %zu B of LDS", max_nsec,
+                         2 * max_nrad, lds_max);
+      const void* kf[3] = {(const void*)k_sg_rows_fwd<real>, (const void*)k_sg_rows_inv<real>,
+                           (const void*)k_sg_cols<real>};
+      const size_t sz[3] = {sg_shm_row, sg_shm_row, sg_shm_col};
+      for (int q = 0; q < 3; ++q)
+        if (sz[q] > 48 * 1024) HIPCHK(hipFuncSetAttribute(kf[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)sz[q]));
+      // the SG field of the initial density (read_field before the first step)
+      D.ignore_active = 1;
+      LCHK(self_gravity());
+      D.ignore_active = 0;
+    }
     HIPCHK(hipStreamSynchronize(stream));
     return FGX_OK;
   }
@@ -468,6 +536,15 @@ struct Batch : BatchBase {
     group_end();
     return r;
   }
+  // compute_selfgravity (SgMain.cu:22-47) on the current density: the three
+  // transform passes, leaving SG_Accr, SG_Acct and their ring means
+  int self_gravity() {
+    const double r = 1.0 / sizeof(real);
+    LCHK(launch("sg_rows", 20 * r, k_sg_rows_fwd<real>, n_rtiles, FGX_TPB, sg_shm_row, D));
+    LCHK(launch("sg_cols", 96 * r, k_sg_cols<real>, n_sgctiles, FGX_TPB, sg_shm_col, D));
+    LCHK(launch("sg_rows", 20 * r + 2, k_sg_rows_inv<real>, n_rtiles, FGX_TPB, sg_shm_row, D));
+    return FGX_OK;
+  }
   int boundary() {
     if (D.open_inner) LCHK(launch("open_boundary", 0, k_open_boundary<real>, nd, FGX_TPB, 0, D));
     if (D.nonreflecting) {
@@ -487,6 +564,10 @@ struct Batch : BatchBase {
     if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
     LCHK(boundary());
     LCHK(launch("substep1", ad ? 6 : 5, k_substep1<real>, n_ctiles, FGX_TPB, 0, D));
+    if (D.self_gravity) {
+      LCHK(self_gravity());
+      LCHK(launch("sg_update", 6, k_sg_update<real>, n_ctiles, FGX_TPB, 0, D));
+    }
     if (D.viscous) {
       if (visc_dynamic) LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
       LCHK(launch("visc_tau", 6, k_visc_tau<real>, n_ctiles, FGX_TPB, 0, D, (const real*)D.vradint,
@@ -539,6 +620,8 @@ struct Batch : BatchBase {
       case FGX_SOUNDSPEED: return D.adiabatic ? D.cs : nullptr;
       case FGX_DIVV: return D.adiabatic ? D.divv : nullptr;
       case FGX_QPLUS: return D.adiabatic ? D.qplus : nullptr;
+      case FGX_SG_ACCR: return D.self_gravity ? D.sg_accr : nullptr;
+      case FGX_SG_ACCT: return D.self_gravity ? D.sg_acct : nullptr;
       default: return nullptr;
     }
   }

@@ -42,6 +42,7 @@ struct HostDisk {
   // initial fields, (nrad+1)*nsec
   std::vector<double> dens, vrad, vtheta, energy, label;
   double omega_frame = 0, hill_radius = 0;
+  double sg_eps = 0, sg_rstep = 0, sg_tstep = 0;   // SGP_eps, SGP_rstep, SGP_tstep
 };
 
 // grid, profiles, density (with noise) and energy; velocities are filled by
@@ -55,6 +56,14 @@ double fviscosity(const fgx_params& P, double r, double csmean, double t);
 double aspect_ratio(const fgx_params& P, double r, double t);
 // GetPsysInfo(FREQUENCY) (Psys.cu:162-232)
 double psys_frequency(const fgx_params& P, const fgx_planets& s);
+// self-gravity (sg.cpp): radix factorisation of a transform length (4, 2, 3,
+// 5), and the per-disk set-up: FP64 kernel transforms (interleaved re/im,
+// (2*nrad) x nsec), the ring mean of the initial radial SG acceleration, and
+// the planets' SG-corrected circular velocities (Init_planetarysys_withSG)
+#define FGX_SG_MAXFAC 24
+int sg_factor(int n, int* fac, int* nf);
+int sg_host_setup(HostDisk& h, std::vector<double>& khat_r, std::vector<double>& khat_t,
+                  std::vector<double>& axisg);
 
 // engine accessors used by the output writers (engine.hip)
 const HostDisk* batch_host(fgx_batch* b, int d);
@@ -83,7 +92,16 @@ struct DiskConst {
   int substeps_forced;     // SUBSTEPS key, 0 = CFL
   int visc_dynamic;        // recompute the viscosity profile each substep
   double sg_eps, sg_rstep, sg_tstep;
+  // self-gravity transforms: offsets of this disk's work rows (nrad*nsec
+  // complex), kernel transforms ((2*nrad)*nsec complex) and twiddle tables
+  // (nsec and 2*nrad entries), radix factorisations of nsec and 2*nrad
+  long long sgw_o, sgk_o;
+  int tw_ns, tw_nr, nf_ns, nf_nr;
+  int fac_ns[24], fac_nr[24];
 };
+
+// a complex FP64 value (self-gravity transforms)
+struct cxd { double x, y; };
 
 // mutable FP64 state of one disk (planets, time, frame, per-substep scalars)
 struct DiskState {

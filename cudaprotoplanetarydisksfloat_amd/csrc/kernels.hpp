@@ -39,6 +39,12 @@ struct Dev {
   // ring centres, and the cell geometry of the potential
   const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d, *energymed_d;
   double* vmoy;  // ring-mean vtheta of the CFL condition
+  // self-gravity: transform work rows, kernel transforms, twiddles, ring
+  // mean of the radial SG acceleration, column-pair tiles (d, k)
+  cxd* sgw;
+  const cxd *sgkr, *sgkt, *sgtw;
+  double* axisg;
+  const RingTile* sg_ctiles;
   const DiskConst* dc;
   DiskState* st;
   double* fpart;       // per ring tile: 2*FGX_NFORCE_T force partial sums
@@ -663,10 +669,16 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
         ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
   }
   if (!D.evanescent) {
-    if (i == 0) AT(D.vthetaint, i, j) = (real)(sqrt(s.vkep_in) - D.rmed_d[rb] * s.omega_frame - D.vref[rb]);
-    if (i == nr - 1)
-      AT(D.vthetaint, i, j) =
-          (real)(sqrt(s.vkep_out) - D.rmed_d[rb + nr - 1] * s.omega_frame - D.vref[rb + nr - 1]);
+    // with self-gravity the Keplerian values include -r * <SG_Accr>_ring
+    // (ApplySubKeplerianBoundary, SideEuler.cu:159-166)
+    if (i == 0) {
+      const double v = s.vkep_in - (D.self_gravity ? D.rmed_d[rb] * D.axisg[rb] : 0.0);
+      AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb] * s.omega_frame - D.vref[rb]);
+    }
+    if (i == nr - 1) {
+      const double v = s.vkep_out - (D.self_gravity ? D.rmed_d[rb + nr - 1] * D.axisg[rb + nr - 1] : 0.0);
+      AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb + nr - 1] * s.omega_frame - D.vref[rb + nr - 1]);
+    }
   }
 }
 

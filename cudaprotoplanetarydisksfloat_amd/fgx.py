@@ -20,7 +20,8 @@ F32, F64 = 0, 1
 MAX_PLANETS = 8
 
 FIELDS = {"dens": 0, "vrad": 1, "vtheta": 2, "energy": 3, "temperature": 4,
-          "soundspeed": 5, "divv": 6, "qplus": 7, "label": 8, "potential": 9}
+          "soundspeed": 5, "divv": 6, "qplus": 7, "label": 8, "potential": 9,
+          "sg_accr": 10, "sg_acct": 11}
 RINGS = {"radii": 0, "rmed": 1, "surf": 2, "sigmamed": 3, "energymed": 4,
          "viscosity": 5, "vmed": 6, "nshift": 7, "cs": 8, "newdt": 9,
          "invrmed": 10}

@@ -103,7 +103,9 @@ typedef struct fgx_planets {
 enum fgx_field {
   FGX_DENS = 0, FGX_VRAD = 1, FGX_VTHETA = 2, FGX_ENERGY = 3,
   FGX_TEMPERATURE = 4, FGX_SOUNDSPEED = 5, FGX_DIVV = 6, FGX_QPLUS = 7,
-  FGX_LABEL = 8, FGX_POTENTIAL = 9, FGX_NFIELDS = 10
+  FGX_LABEL = 8, FGX_POTENTIAL = 9,
+  FGX_SG_ACCR = 10, FGX_SG_ACCT = 11,   /* self-gravity accelerations SG_Accr, SG_Acct (SgMain.cu) */
+  FGX_NFIELDS = 12
 };
 enum fgx_ring {
   FGX_R_RADII = 0, FGX_R_RMED = 1, FGX_R_SURF = 2, FGX_R_SIGMAMED = 3,

@@ -33,6 +33,8 @@ std::vector<typename std::remove_reference<decltype(std::declval<D>().Dens)>::ty
     case FGX_QPLUS: return &s.Qplus;
     case FGX_LABEL: return &s.Label;
     case FGX_POTENTIAL: return &s.Potential;
+    case FGX_SG_ACCR: return &s.SG_Accr;
+    case FGX_SG_ACCT: return &s.SG_Acct;
     default: return nullptr;
   }
 }

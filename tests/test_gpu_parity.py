@@ -52,11 +52,18 @@ def _case(name):
         par, cfg = configs.plain_disk(nrad=48, nsec=144)
         cfg = "P1\t1.0\t1e-3\t0.0\tYES\tYES\nP2\t1.6\t3e-4\t0.0\tYES\tYES\n"
         return par, cfg, {"FORCEDCIRCULAR": "NO", "EXCLUDEHILL": "YES"}
+    if name == "self_gravity":
+        par, cfg = configs.sg_disk(nrad=24, nsec=96)
+        return par, cfg, {}
+    if name == "self_gravity_radix5":
+        par, cfg = configs.sg_disk(nrad=20, nsec=80, sigma0=4e-3)
+        return par, cfg, {"ADIABATIC": "YES", "ALPHAVISCOSITY": 0, "VISCOSITY": 1e-5}
     raise KeyError(name)
 
 
 CASES = ["plain", "plain_log_alpha", "adiabatic_evanescent", "adiabatic_cooling_open", "nonreflecting",
-         "nonreflecting_adiabatic", "open", "standard_transport", "guiding_center", "two_planets"]
+         "nonreflecting_adiabatic", "open", "standard_transport", "guiding_center", "two_planets",
+         "self_gravity", "self_gravity_radix5"]
 
 
 @pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
@@ -93,6 +100,25 @@ def test_wide_rings(gpu, nsec, precision):
     print(nsec, precision, err)
     for n, e in err.items():
         assert e < TOL[precision], (nsec, n, e)
+
+
+@pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["self_gravity", "self_gravity_radix5"])
+def test_initial_self_gravity_field(gpu, case, precision):
+    """SG accelerations of the initial density (the three transform passes)
+    against the oracle's FFT solution, itself pinned to a direct sum."""
+    par, cfg, keys = _case(case)
+    disk, orc, names = pair(par, cfg, precision, DT=DT_SMALL, SUBSTEPS=1, **keys)
+    nr, ns = disk.shape[0]
+    want = {"sg_accr": np.zeros((nr, ns)), "sg_acct": np.zeros((nr, ns))}
+    # the oracle's field of the density it was just given (the GPU's)
+    assert oracle.lib().orc_sg_accel(orc.h, want["sg_accr"].ctypes.data, want["sg_acct"].ctypes.data) == 0
+    for name in ("sg_accr", "sg_acct"):
+        e = rel_l2(disk.field(name), want[name])
+        assert e < (1e-7 if precision == F32 else 1e-12), (name, e)
+    # planets start on SG-corrected circular orbits
+    gp, op = disk.planets(), orc.planets(type(disk.planets()))
+    assert abs(gp.vy[0] - op.vy[0]) < 1e-14
 
 
 def test_cfl_driven_substep_counts(gpu):

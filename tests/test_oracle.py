@@ -178,3 +178,43 @@ def test_float_instance_fargo_shift_rounding():
         ntilde = f(np.float64(prod) / 2.0 / math.pi)
         want = int(math.floor(np.float64(ntilde) + 0.5))
         assert L.orc_fargo_shift_f32(float(f(vmed)), float(f(invr)), dt, nsec) == want
+
+
+def test_self_gravity_matches_direct_sum():
+    """The FFT convolution of SgMain.cu equals the direct sum of the softened
+    gravity of every other cell.  A mass m' at (r', t') seen at (r, t) has the
+    potential -G m' / D, D^2 = r^2 + r'^2 - 2 r r' cos(t - t') + eps^2 r^2, so
+    a_r = -G m' (r (1 + eps^2) - r' cos) / D^3 and a_t = -G m' r' sin / D^3,
+    with m' = Sigma' r'^2 du dt on the log grid (du = ln(R_out/R_in)/NRAD)."""
+    nr, ns = 6, 12
+    p, _, o = make(nrad=nr, nsec=ns, RADIALSPACING="LOGARITHM", SELFGRAVITY="YES", NOISEAMPLITUDE=0)
+    rng = np.random.RandomState(5)
+    full = np.zeros((nr + 1, ns))
+    full[:nr] = 1e-3 * (0.5 + rng.rand(nr, ns))
+    o.set_field("dens", full)
+    accr = np.zeros((nr, ns))
+    acct = np.zeros((nr, ns))
+    assert oracle.lib().orc_sg_accel(o.h, accr.ctypes.data, acct.ctypes.data) == 0
+    radii = o.ring("radii")
+    r = o.ring("rmed")[:nr]
+    du = math.log(radii[nr] / radii[0]) / nr
+    dt = 2 * math.pi / ns
+    eps = float(np.float32(p["THICKNESSSMOOTHING"])) * float(np.float32(p["ASPECTRATIO"]))
+    t = 2 * math.pi * np.arange(ns) / ns
+    m = full[:nr] * (r ** 2)[:, None] * du * dt
+    want_r = np.zeros((nr, ns))
+    want_t = np.zeros((nr, ns))
+    for i in range(nr):
+        for j in range(ns):
+            c = np.cos(t[j] - t)[None, :]
+            s = np.sin(t[j] - t)[None, :]
+            rp = r[:, None]
+            d3 = (r[i] ** 2 + rp ** 2 - 2 * r[i] * rp * c + eps ** 2 * r[i] ** 2) ** 1.5
+            fr = -m * (r[i] * (1 + eps ** 2) - rp * c) / d3
+            ft = -m * rp * s / d3
+            fr[i, j] = 0.0
+            ft[i, j] = 0.0
+            want_r[i, j] = fr.sum()
+            want_t[i, j] = ft.sum()
+    assert np.allclose(accr, want_r, rtol=1e-9, atol=1e-12 * np.abs(want_r).max())
+    assert np.allclose(acct, want_t, rtol=1e-9, atol=1e-12 * np.abs(want_t).max())
