@@ -9,7 +9,11 @@ average device time (HIP events on the library's stream, recorded during a
 second pass over the same steps); `cpu_baseline` times the FP64 OpenMP
 oracle on a bounded sample of the same disks.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|plain|sg]
+                    [--serve-requests N]
+
+A serving phase follows (fgx_server_*): Poisson-arriving disk runs of mixed
+sizes served concurrently; its p50/p99 completion times go into "serving".
 """
 import argparse
 import json
@@ -48,6 +52,10 @@ def _workload(name):
         disks = [configs.large_disk()]
         desc = {"workload": "512x1536 adiabatic disk, log grid, alpha 1e-3, evanescent boundaries, "
                             "migrating planet (RK5)", "disks": 1}
+    elif name == "sg":
+        disks = [configs.sg_disk()]
+        desc = {"workload": "256x768 self-gravitating disk, log r in [0.5, 4], Sigma ~ r^-1.5, Toomre Q ~ 26, "
+                            "one planet", "disks": 1}
     else:
         disks = [configs.plain_disk()]
         desc = {"workload": "128x384 isothermal disk, one fixed-orbit planet, closed boundaries", "disks": 1}
@@ -55,21 +63,59 @@ def _workload(name):
     return loaded, desc
 
 
+def serving(n_req, precision, dt_range=(4, 24), snapshot_every=4):
+    """The serving workload (Poisson arrivals at 4/s, grids 128x384..1024x3072,
+    1-3 planets, 20% self-gravity) with request lengths scaled down from
+    U[500, 5000] DTs to U[dt_range] so the run takes seconds; completion time =
+    finish - arrival of each request."""
+    import numpy as np
+    from cudaprotoplanetarydisksfloat_amd import Server, configs
+    import torch
+    reqs = configs.serve_requests(n=n_req, rate=4.0, dt_range=dt_range)
+    budget = int(torch.cuda.get_device_properties(0).total_memory * 0.9)
+    srv = Server(precision=precision, hbm_budget=budget, max_batch_disks=64, snapshot_every=snapshot_every,
+                 batch_window=0.0)
+    for par, cfg, n_dt, t in reqs:
+        p, pl = configs.load(par, cfg)
+        srv.submit(p, pl, n_dt, t)
+    t0 = time.perf_counter()
+    srv.run()
+    wall = time.perf_counter() - t0
+    infos = [srv.request(k) for k in range(len(reqs))]
+    srv.close()
+    lat = np.array([i.finish - i.arrival for i in infos])
+    cu = sum(i.cell_updates for i in infos)
+    return {"requests": len(reqs), "p50_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_s": round(float(np.percentile(lat, 99)), 4), "max_s": round(float(lat.max()), 4),
+            "cell_updates_per_s": cu / wall, "makespan_s": round(wall, 3),
+            "last_arrival_s": round(reqs[-1][3], 3), "batches": len({i.batch for i in infos}),
+            "snapshots": int(sum(i.snapshots for i in infos)), "self_gravity_requests":
+                sum(1 for par, _, _, _ in reqs if "SelfGravity\tYES" in par),
+            "workload": "Poisson 4/s, grids 128x384..1024x3072, 1-3 planets, 20% SG, "
+                        f"U[{dt_range[0]},{dt_range[1]}] DTs per request (scaled from U[500,5000]), "
+                        f"snapshot every {snapshot_every} DTs"}
+
+
 def cpu_baseline(loaded, budget_s=20.0):
     """FP64 OpenMP oracle on the first disks of the workload, one DT each,
     until about budget_s of CPU work."""
     import oracle
-    total, t0, ndisk = 0, time.perf_counter(), 0
-    for p, pl in loaded:
-        o = oracle.Oracle(p, pl, 1)
-        n = o.step(1)
-        total += n * o.nrad * o.nsec
-        ndisk += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
+    total, t0, ndisk, ndt = 0, time.perf_counter(), 0, 0
+    orcs = []
+    while time.perf_counter() - t0 < budget_s:
+        # one more DT of every disk, disks created on first use
+        for k, (p, pl) in enumerate(loaded):
+            if k == len(orcs):
+                orcs.append(oracle.Oracle(p, pl, 1))
+                ndisk += 1
+            o = orcs[k]
+            total += o.step(1) * o.nrad * o.nsec
+            ndt += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
     dt = time.perf_counter() - t0
     return {"value": total / dt, "unit": "cell-updates/s", "cores": oracle.lib().orc_num_threads(),
-            "kind": "port", "sample": f"first {ndisk} disk(s) of the workload, 1 DT each, "
+            "kind": "port", "sample": f"{ndisk} disk(s) of the workload, {ndt} DTs in all, "
                                       f"{total} cell-updates in {dt:.1f} s"}
 
 
@@ -78,10 +124,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "large", "plain"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "large", "plain", "sg"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--serve-requests", type=int, default=40,
+                    help="requests of the serving phase (p50/p99 completion time); 0 skips it")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -161,12 +209,13 @@ def main():
         for b in solo:
             b.close()
 
+    serve = serving(args.serve_requests, prec) if args.serve_requests > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(loaded)
     line = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
             "config": dict(desc, cell_updates=updates, cells=ncell), "roofline": roofline,
-            "cpu_baseline": cpu, "plain": plain}
+            "cpu_baseline": cpu, "plain": plain, "serving": serve}
     print(json.dumps(line))
 
 

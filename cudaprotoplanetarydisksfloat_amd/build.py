@@ -21,10 +21,10 @@ ARCH = os.environ.get("FGX_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-command-line-argument"]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function"]
 
 HIP_SOURCES = ["engine.hip"]
-CXX_SOURCES = ["params.cpp", "setup.cpp", "capi.cpp", "output.cpp", "sg.cpp"]
+CXX_SOURCES = ["params.cpp", "setup.cpp", "capi.cpp", "output.cpp", "sg.cpp", "server.cpp"]
 
 
 def _run(cmd):
@@ -51,7 +51,7 @@ def build(verbose=False):
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     run_obj = _compile(os.path.join(CSRC, "fgx_run.cpp"))
     _run(["g++", "-o", EXE, run_obj, "-L" + HERE, "-lfgx", "-Wl,-rpath,$ORIGIN"])

@@ -14,6 +14,8 @@ it, so every file also runs under the reference.
                  q = 3e-4 planet (RK5), evanescent boundaries
 - sg_disk:       256 x 768 self-gravitating disk, log r in [0.5, 4.0], Sigma ~ r^-1.5,
                  alpha = 1e-4, q = 1e-3 planet, +-1e-3 noise
+- serve_requests: Poisson-arriving disk runs of the serving workload: grids
+                 128x384 .. 1024x3072, 1-3 Hill-stable planets, 20% self-gravity
 - batch_disks:   64 isothermal disks with NRAD in {128,192,256}, NSEC in
                  {384,512,768} and parameters drawn from a seed
 """
@@ -114,3 +116,58 @@ def load(par_text, cfg_text, **overrides):
     for k, v in overrides.items():
         p.set(k, v)
     return p, planets_from_text(cfg_text, p)
+
+
+SERVE_GRIDS = ((128, 384), (256, 768), (512, 1536), (1024, 3072))
+SERVE_GRID_WEIGHTS = (0.4, 0.3, 0.2, 0.1)
+
+
+def _hill_stable_orbits(rng, nplanets, lo=0.7, hi=1.6):
+    """Semi-major axes in [lo, hi] and masses log-U[1e-5, 3e-3] with every
+    neighbouring pair at least 2*sqrt(3) mutual Hill radii apart."""
+    for _ in range(1000):
+        q = 10 ** rng.uniform(np.log10(1e-5), np.log10(3e-3), nplanets)
+        a = np.sort(rng.uniform(lo, hi, nplanets))
+        ok = True
+        for k in range(nplanets - 1):
+            rh = ((q[k] + q[k + 1]) / 3.0) ** (1.0 / 3.0) * 0.5 * (a[k] + a[k + 1])
+            ok &= (a[k + 1] - a[k]) >= 2 * np.sqrt(3.0) * rh
+        if ok:
+            return a, q
+    raise RuntimeError("no Hill-stable configuration found")
+
+
+def serve_requests(n=40, rate=4.0, seed=BASE_SEED, dt_range=(500, 5000), sg_frac=0.2, noise=1e-4):
+    """Serving workload: n requests with Poisson arrivals at `rate` per second
+    (seeded), each a disk run of U[dt_range] DTs on one of SERVE_GRIDS, with
+    the batch workload's physics draws, 1-3 Hill-stable planets at
+    a in U[0.7, 1.6] and self-gravity (log grid, r in [0.5, 4]) for a
+    fraction sg_frac.  Returns [(par_text, cfg_text, n_dt, arrival_s)]."""
+    rng = np.random.RandomState((seed + 5) % (2 ** 32))
+    t = 0.0
+    out = []
+    for d in range(n):
+        t += rng.exponential(1.0 / rate)
+        nrad, nsec = SERVE_GRIDS[rng.choice(len(SERVE_GRIDS), p=SERVE_GRID_WEIGHTS)]
+        sg = rng.uniform() < sg_frac
+        h = float(rng.uniform(0.03, 0.08))
+        alpha = float(10 ** rng.uniform(-4, -2))
+        slope = float(rng.uniform(-1.5, 0.0))
+        npl = int(rng.randint(1, 4))
+        a, q = _hill_stable_orbits(rng, npl)
+        n_dt = int(rng.randint(dt_range[0], dt_range[1] + 1))
+        keys = {
+            "AspectRatio": f"{h:.6f}", "Sigma0": 6.3662e-4, "AlphaViscosity": f"{alpha:.6e}",
+            "SigmaSlope": f"{-slope:.6f}", "FlaringIndex": 0.0, "PlanetConfig": "in/planets.cfg",
+            "ThicknessSmoothing": 0.6, "ForcedCircular": "YES", "Transport": "FARGO",
+            "InnerBoundary": "RIGID", "Disk": "YES", "OmegaFrame": 0.0, "Frame": "FIXED",
+            "IndirectTerm": "YES", "Nrad": nrad, "Nsec": nsec,
+            "Rmin": 0.5 if sg else 0.4, "Rmax": 4.0 if sg else 2.5,
+            "RadialSpacing": "LOGARITHM" if sg else "ARITHMETIC",
+            "Ntot": n_dt, "Ninterm": 100, "DT": DT_ORBIT_20, "OutputDir": "out/", "Adiabatic": "NO",
+            "SelfGravity": "YES" if sg else "NO",
+            "NoiseAmplitude": noise, "NoiseSeed": seed, "DiskId": 1000 + d,
+        }
+        cfg = "".join(f"P{k}\t{a[k]:.6f}\t{q[k]:.6e}\t0.0\tNO\tNO\n" for k in range(npl))
+        out.append((_par(keys), cfg, n_dt, t))
+    return out

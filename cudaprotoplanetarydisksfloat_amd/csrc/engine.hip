@@ -51,6 +51,7 @@ struct BatchBase {
   virtual int profile_reset() = 0;
   virtual const HostDisk& host(int d) const = 0;
   virtual int size() const = 0;
+  virtual int freeze(int disk, int on) = 0;
 };
 
 template <class real>
@@ -808,6 +809,14 @@ struct Batch : BatchBase {
     return FGX_OK;
   }
   int profile_reset() override { prof.clear(); return FGX_OK; }
+  int freeze(int d, int on) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    const int v = on ? 1 : 0;
+    HIPCHK(hipMemcpyAsync((char*)D.st + (size_t)d * sizeof(DiskState) + offsetof(DiskState, frozen), &v, sizeof(int),
+                          hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return FGX_OK;
+  }
 };
 
 }  // namespace fgx
@@ -862,6 +871,10 @@ int fgx_disk_step(fgx_disk* d, int n) {
 int fgx_batch_step(fgx_batch* b, int n) { return fgx_disk_step(b, n); }
 int fgx_disk_destroy(fgx_disk* d) { delete d; return FGX_OK; }
 int fgx_batch_destroy(fgx_batch* b) { delete b; return FGX_OK; }
+int fgx_batch_freeze(fgx_batch* b, int disk, int frozen) {
+  if (!b) return set_error(FGX_EPARAM, "null handle");
+  return b->impl->freeze(disk, frozen);
+}
 int fgx_batch_size(fgx_batch* b, int* n) {
   if (!b || !n) return set_error(FGX_EPARAM, "null argument");
   *n = b->impl->size();

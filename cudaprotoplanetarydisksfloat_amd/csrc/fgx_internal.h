@@ -123,6 +123,7 @@ struct DiskState {
   long long substeps, dt_steps;
   int ncfl, active, sub_in_dt, health, rem_est;
   int nr_iangle, nr_iangle2;            // non-reflecting shifts of this call
+  int frozen;                           // skipped by every later DT (served request done)
 };
 
 // a field tile: ring i of disk d, sectors [jb*256, jb*256+256)

@@ -1065,6 +1065,7 @@ __global__ void k_begin_dt(Dev<real> D) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= D.ndisk) return;
   DiskState& s = D.st[d];
+  if (s.frozen) { s.active = 0; return; }
   s.dtemp = 0;
   s.sub_in_dt = 0;
   s.active = 1;

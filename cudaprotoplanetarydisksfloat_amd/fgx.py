@@ -73,6 +73,21 @@ class DiskInfo(ctypes.Structure):
                 ("dt_steps", ctypes.c_longlong), ("health", ctypes.c_int)]
 
 
+class ServerConfig(ctypes.Structure):
+    """fgx_server_config (include/fgx.h)."""
+    _fields_ = [("precision", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("max_batch_disks", ctypes.c_int), ("snapshot_every", ctypes.c_int),
+                ("batch_window", ctypes.c_double), ("keep_final", ctypes.c_int)]
+
+
+class RequestInfo(ctypes.Structure):
+    """fgx_request_info: timestamps (s from the start of the run) and work."""
+    _fields_ = [("arrival", ctypes.c_double), ("admit", ctypes.c_double), ("finish", ctypes.c_double),
+                ("substeps", ctypes.c_longlong), ("cell_updates", ctypes.c_longlong),
+                ("n_dt", ctypes.c_int), ("snapshots", ctypes.c_int), ("batch", ctypes.c_int),
+                ("batch_size", ctypes.c_int), ("status", ctypes.c_int)]
+
+
 _vp, _cp, _sz, _i, _d = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
 _sigs = {
     "fgx_version": ([], _i), "fgx_last_error": ([], _cp),
@@ -103,6 +118,13 @@ _sigs = {
                          ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_d)], _i),
     "fgx_profile_reset": ([_vp], _i),
     "fgx_run_main": ([_i, ctypes.POINTER(_cp)], _i),
+    "fgx_batch_freeze": ([_vp, _i, _i], _i),
+    "fgx_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)], _i),
+    "fgx_server_submit": ([_vp, _vp, ctypes.POINTER(Planets), _i, _d, ctypes.POINTER(_i)], _i),
+    "fgx_server_run": ([_vp], _i),
+    "fgx_server_request": ([_vp, _i, ctypes.POINTER(RequestInfo)], _i),
+    "fgx_server_read_field": ([_vp, _i, _i, _i, _vp, _sz], _i),
+    "fgx_server_destroy": ([_vp], _i),
 }
 for _n, (_a, _r) in _sigs.items():
     _f = getattr(lib, _n)
@@ -293,3 +315,48 @@ def run_main(argv):
     """The reference executable's main() (src/Main.cu:87-401) in-process."""
     args = (ctypes.c_char_p * len(argv))(*[a.encode() for a in argv])
     return lib.fgx_run_main(len(argv), args)
+
+
+class Server:
+    """Serving loop (fgx_server_*): requests arrive at given times (seconds
+    from the start of run()), are admitted under an HBM budget and served in
+    concurrent batches; per-request admit/finish times come back."""
+
+    def __init__(self, precision=F32, hbm_budget=256 << 30, max_batch_disks=64, snapshot_every=100,
+                 batch_window=0.0, keep_final=False):
+        cfg = ServerConfig(precision, hbm_budget, max_batch_disks, snapshot_every, batch_window, int(keep_final))
+        self.h = ctypes.c_void_p()
+        _check(lib.fgx_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+        self.shapes = []
+
+    def submit(self, params, planets, n_dt, arrival):
+        rid = ctypes.c_int()
+        _check(lib.fgx_server_submit(self.h, ctypes.cast(params.buf, ctypes.c_void_p), ctypes.byref(planets),
+                                     int(n_dt), float(arrival), ctypes.byref(rid)))
+        self.shapes.append((int(params["NRAD"]), int(params["NSEC"])))
+        return rid.value
+
+    def run(self):
+        _check(lib.fgx_server_run(self.h))
+
+    def request(self, rid):
+        info = RequestInfo()
+        _check(lib.fgx_server_request(self.h, rid, ctypes.byref(info)))
+        return info
+
+    def field(self, rid, name):
+        nr, ns = self.shapes[rid]
+        out = np.empty((nr + 1) * ns, dtype=np.float64)
+        _check(lib.fgx_server_read_field(self.h, rid, FIELDS[name], F64, out.ctypes.data, out.size))
+        return out.reshape(nr + 1, ns)[:nr]
+
+    def close(self):
+        if self.h:
+            lib.fgx_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

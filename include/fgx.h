@@ -207,8 +207,45 @@ int fgx_batch_read_field(fgx_batch* b, int disk, int field, int dtype, void* hos
                          size_t count);
 int fgx_batch_write_field(fgx_batch* b, int disk, int field, int dtype,
                           const void* host, size_t count);
+/* freeze (1) or thaw (0) disk `disk` of a batch: later fgx_batch_step calls
+ * leave a frozen disk untouched (its tiles are skipped).  No counterpart in
+ * the reference; the serving loop retires finished requests with it. */
+int fgx_batch_freeze(fgx_batch* b, int disk, int frozen);
 /* bytes of device memory a disk of these parameters needs */
 int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes);
+
+/* ---- serving loop (north-star C5; no counterpart in the reference) ------
+ * Requests (one disk each: parameters, planets, number of DTs, arrival time
+ * in seconds from the start of fgx_server_run) are admitted in arrival order
+ * while the summed device footprint stays within hbm_budget; requests that
+ * arrive together and share the batch mode flags form one batch.  Each live
+ * batch advances on its own stream and host thread, one DT at a time; a
+ * request that has done its DTs is frozen and its finish time recorded;
+ * every snapshot_every DTs the fields of each running request are copied to
+ * host memory (the reference writes its outputs every NINTERM steps,
+ * src/Main.cu:348-371).  A request's fields evolve exactly as they would
+ * alone. */
+typedef struct fgx_server fgx_server;
+typedef struct fgx_server_config {
+  int    precision;        /* FGX_F32 / FGX_F64                                */
+  size_t hbm_budget;       /* bytes of device memory the live batches may use  */
+  int    max_batch_disks;  /* most requests in one batch                       */
+  int    snapshot_every;   /* DTs between snapshots (0: none)                  */
+  double batch_window;     /* seconds to wait for more arrivals to batch with  */
+  int    keep_final;       /* keep each request's final fields for read-back   */
+} fgx_server_config;
+typedef struct fgx_request_info {
+  double arrival, admit, finish;      /* seconds from the start of the run   */
+  long long substeps, cell_updates;   /* CFL substeps done, cells x substeps */
+  int    n_dt, snapshots, batch, batch_size, status;   /* status: FGX_OK or error */
+} fgx_request_info;
+int fgx_server_create(const fgx_server_config* cfg, fgx_server** out);
+int fgx_server_submit(fgx_server* s, const fgx_params* p, const fgx_planets* pl, int n_dt,
+                      double arrival, int* id);
+int fgx_server_run(fgx_server* s);      /* serves every submitted request    */
+int fgx_server_request(fgx_server* s, int id, fgx_request_info* out);
+int fgx_server_read_field(fgx_server* s, int id, int field, int dtype, void* host, size_t count);
+int fgx_server_destroy(fgx_server* s);
 
 /* ---- profiling: average device time per kernel class ---------------------
  * enable=1 brackets each launch with HIP events on the library's stream. */

@@ -1,0 +1,71 @@
+"""Serving loop (fgx_server_*): admission, concurrent batches, retirement,
+snapshots, and results identical to the same disks run alone."""
+import numpy as np
+import pytest
+
+from cudaprotoplanetarydisksfloat_amd import F32, Batch, Server, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    out = []
+    for k in range(5):
+        par, cfg = configs.plain_disk(nrad=32 + 8 * k, nsec=96, disk_id=k)
+        out.append((par, cfg, 3 + k, 0.05 * k, {"VISCOSITY": 0, "ALPHAVISCOSITY": 1e-3 * (k + 1)}))
+    par, cfg = configs.sg_disk(nrad=24, nsec=96, disk_id=9)
+    out.append((par, cfg, 4, 0.12, {}))
+    return out
+
+
+def test_serving_timestamps_and_alone_equality(gpu):
+    reqs = _requests()
+    srv = Server(precision=F32, hbm_budget=1 << 30, max_batch_disks=3, snapshot_every=2,
+                 batch_window=0.02, keep_final=True)
+    loaded = []
+    for par, cfg, n_dt, t, keys in reqs:
+        p, pl = configs.load(par, cfg, **keys)
+        loaded.append((p, pl, n_dt))
+        srv.submit(p, pl, n_dt, t)
+    srv.run()
+    batches = set()
+    for rid, (p, pl, n_dt) in enumerate(loaded):
+        info = srv.request(rid)
+        print(rid, info.arrival, info.admit, info.finish, info.batch, info.batch_size, info.substeps, info.snapshots)
+        assert info.status == 0
+        assert info.admit >= info.arrival - 1e-6 and info.finish > info.admit
+        assert info.n_dt == n_dt and info.substeps >= n_dt
+        assert info.cell_updates == info.substeps * int(p["NRAD"]) * int(p["NSEC"])
+        assert info.snapshots == (n_dt - 1) // 2
+        batches.add(info.batch)
+        # the same disk alone gives bit-identical fields
+        alone = Batch(p, pl, F32)
+        alone.step(n_dt)
+        for name in ("dens", "vrad", "vtheta"):
+            assert np.array_equal(srv.field(rid, name), alone.field(name)), (rid, name)
+        assert alone.info().substeps == info.substeps
+        alone.close()
+    assert len(batches) >= 2   # the self-gravity request cannot share a batch with the others
+    srv.close()
+
+
+def test_serving_budget_serialises(gpu):
+    """With a budget for one request at a time every request still completes,
+    each admitted only after the previous one finished."""
+    reqs = _requests()[:3]
+    loaded = [configs.load(par, cfg, **keys) for par, cfg, _, _, keys in reqs]
+    from cudaprotoplanetarydisksfloat_amd.fgx import lib
+    import ctypes
+    sizes = []
+    for p, _ in loaded:
+        b = ctypes.c_size_t()
+        assert lib.fgx_footprint_bytes(ctypes.cast(p.buf, ctypes.c_void_p), F32, ctypes.byref(b)) == 0
+        sizes.append(b.value)
+    srv = Server(precision=F32, hbm_budget=max(sizes) + 1, max_batch_disks=8, snapshot_every=0)
+    for (p, pl), (_, _, n_dt, t, _) in zip(loaded, reqs):
+        srv.submit(p, pl, n_dt, 0.0)
+    srv.run()
+    infos = [srv.request(k) for k in range(3)]
+    for a, b in zip(infos, infos[1:]):
+        assert b.admit >= a.finish - 1e-6
+    srv.close()
