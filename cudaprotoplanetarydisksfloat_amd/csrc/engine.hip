@@ -238,7 +238,9 @@ struct Batch : BatchBase {
       c.viscosity = p.viscosity; c.alphaviscosity = p.alphaviscosity;
       c.hill_radius = h.hill_radius; c.sigma0 = p.sigma0; c.eccentricity = p.eccentricity;
       c.substeps_forced = p.ext_substeps;
-      c.visc_dynamic = (p.viscosity_alpha || p.lambdadoubling != 0.0f) ? 1 : 0;
+      // the viscosity profile changes only through the ring-mean sound speed
+      // (adiabatic alpha viscosity) or the time-dependent transition scale
+      c.visc_dynamic = ((p.viscosity_alpha && p.adiabatic) || p.lambdadoubling != 0.0f) ? 1 : 0;
       visc_dynamic |= c.visc_dynamic;
       if (p.self_gravity) {
         c.sg_eps = h.sg_eps; c.sg_rstep = h.sg_rstep; c.sg_tstep = h.sg_tstep;
@@ -289,6 +291,7 @@ struct Batch : BatchBase {
     LCHK(f(&D.t_tmp, true)); LCHK(f(&D.t_tmm, true)); LCHK(f(&D.t_e, ad)); LCHK(f(&D.work, true));
     const int sg = P0.self_gravity;
     LCHK(f(&D.sg_accr, sg)); LCHK(f(&D.sg_acct, sg));
+    LCHK(dalloc(&D.phi, F));
     if (sg) {
       while (sgct.size() % 8) sgct.push_back(RingTile{-1, 0});
       n_sgctiles = (int)sgct.size();
@@ -344,6 +347,8 @@ struct Batch : BatchBase {
       LCHK(dalloc(&std_t, st.size()));
       HIPCHK(hipMemcpyAsync((void*)std_t, st.data(), st.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
       stiles = std_t;
+      D.stiles = std_t;
+      D.strip_rings = strip_rings;
       for (int kw = 0; kw < 17 * 17; ++kw) {
         std::vector<RingTile>& v = tg[kw];
         if (v.empty()) continue;
@@ -399,7 +404,7 @@ struct Batch : BatchBase {
     // isothermal temperature output
     D.ignore_active = 1;
     LCHK(launch("pressure_init", 2, k_pressure_init<real>, n_ctiles, FGX_TPB, 0, D));
-    if (ad) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D));
+    if (ad) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
     D.ignore_active = 0;
     // LDS of the azimuthal transport kernel
     int fb_nsec = 0;   // widest ring of the shared-memory azimuthal kernel
@@ -549,10 +554,10 @@ struct Batch : BatchBase {
   int boundary() {
     if (D.open_inner) LCHK(launch("open_boundary", 0, k_open_boundary<real>, nd, FGX_TPB, 0, D));
     if (D.nonreflecting) {
-      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D));
+      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
       LCHK(launch("nonreflecting", 0, k_nonreflecting<real>, nd, FGX_TPB, 0, D));
     }
-    if (D.evanescent) LCHK(launch("evanescent", D.adiabatic ? 8 : 6, k_evanescent<real>, n_ctiles, FGX_TPB, 0, D));
+    if (D.evanescent) LCHK(launch("evanescent", D.adiabatic ? 8 : 6, k_evanescent<real>, n_stiles, FGX_TPB, 0, D));
     return FGX_OK;
   }
   int substep() {
@@ -561,30 +566,31 @@ struct Batch : BatchBase {
     LCHK(launch("substep_begin", 0, k_substep_begin<real>, nd, FGX_TPB, 0, D));
     LCHK(launch("force", 1, k_force_ring<real>, n_rtiles, FGX_TPB, 0, D));
     LCHK(launch("planets", 0, k_planets<real>, nd, FGX_TPB, 0, D));
-    if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_ctiles, FGX_TPB, 0, D));
+    if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_stiles, FGX_TPB, 0, D));
     if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
     LCHK(boundary());
-    LCHK(launch("substep1", ad ? 6 : 5, k_substep1<real>, n_ctiles, FGX_TPB, 0, D));
+    LCHK(launch("potential", 2, k_potential<real>, n_stiles, FGX_TPB, 0, D));
+    LCHK(launch("substep1", ad ? 8 : 7, k_substep1<real>, n_stiles, FGX_TPB, 0, D));
     if (D.self_gravity) {
       LCHK(self_gravity());
-      LCHK(launch("sg_update", 6, k_sg_update<real>, n_ctiles, FGX_TPB, 0, D));
+      LCHK(launch("sg_update", 6, k_sg_update<real>, n_stiles, FGX_TPB, 0, D));
     }
     if (D.viscous) {
       if (visc_dynamic) LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
-      LCHK(launch("visc_tau", 6, k_visc_tau<real>, n_ctiles, FGX_TPB, 0, D, (const real*)D.vradint,
+      LCHK(launch("visc_tau", 6, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vradint,
                   (const real*)D.vthetaint, 0));
     }
-    LCHK(launch("visc_update", 7, k_visc_update<real>, n_ctiles, FGX_TPB, 0, D));
-    LCHK(launch("substep2", ad ? 7 : 5, k_substep2<real>, n_ctiles, FGX_TPB, 0, D));
+    LCHK(launch("visc_update", 7, k_visc_update<real>, n_stiles, FGX_TPB, 0, D));
+    LCHK(launch("substep2", ad ? 7 : 5, k_substep2<real>, n_stiles, FGX_TPB, 0, D));
     LCHK(boundary());
     if (ad) {
       if (visc_dynamic) LCHK(launch("visc_profile", 1, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
-      LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_ctiles, FGX_TPB, 0, D, (const real*)D.vrad, (const real*)D.vtheta, 1));
-      LCHK(launch("substep3", 9, k_substep3<real>, n_ctiles, FGX_TPB, 0, D));
+      LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vrad, (const real*)D.vtheta, 1));
+      LCHK(launch("substep3", 9, k_substep3<real>, n_stiles, FGX_TPB, 0, D));
     }
     LCHK(launch("trans_rad", ad ? 10 : 8, k_trans_rad_strip<real>, n_stiles, FGX_TPB, 0, D, stiles, strip_rings));
     LCHK(trans_theta(ad ? 11 : 9));
-    LCHK(launch("trans_vr", 4, k_trans_vr<real>, n_ctiles, FGX_TPB, 0, D));
+    LCHK(launch("trans_vr", 4, k_trans_vr<real>, n_stiles, FGX_TPB, 0, D));
     LCHK(boundary());
     LCHK(launch("end_substep", 0, k_end_substep<real>, nd, 64, 0, D));
     return FGX_OK;
@@ -592,7 +598,7 @@ struct Batch : BatchBase {
   int step(int n_dt) override {
     for (int k = 0; k < n_dt; ++k) {
       LCHK(launch("begin_dt", 0, k_begin_dt<real>, (nd + 63) / 64, 64, 0, D));
-      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D));
+      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
       int plan = 0;
       for (int guard = 0;; ++guard) {
         if (guard > 10000000) return set_error(FGX_ENUMERIC, "substep loop did not terminate");
@@ -686,7 +692,7 @@ struct Batch : BatchBase {
     if (field == FGX_DENS || field == FGX_ENERGY) {
       D.ignore_active = 1;
       int r = launch("pressure_init", 2, k_pressure_init<real>, n_ctiles, FGX_TPB, 0, D);
-      if (!r && D.adiabatic) r = launch("sound_speed", 3, k_sound_speed<real>, n_ctiles, FGX_TPB, 0, D);
+      if (!r && D.adiabatic) r = launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D);
       D.ignore_active = 0;
       LCHK(r);
       HIPCHK(hipStreamSynchronize(stream));
@@ -943,10 +949,13 @@ int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes) {
   if (!p || !bytes) return set_error(FGX_EPARAM, "null argument");
   const size_t F = (size_t)(p->nrad + 1) * p->nsec, R = (size_t)p->nrad + 1;
   const size_t rs = precision == FGX_F64 ? 8 : 4;
+  // self-gravity: transform rows (nrad*nsec), two kernel transforms
+  // ((2*nrad)*nsec), all complex FP64, and the two acceleration fields
+  const size_t sgb = p->self_gravity ? (size_t)p->nrad * p->nsec * 16 * 5 + F * rs * 2 : 0;
   const int nfields = p->adiabatic ? 23 : 17;
   // fields, ring tables (working precision and the FP64 set), sector tables
-  *bytes = F * rs * nfields + R * rs * 21 + R * 8 * 10 + (size_t)p->nsec * (rs * 3 + 8 * 2) +
-           R * 8 * 2 * FGX_NFORCE_T + 4096;
+  *bytes = F * rs * nfields + F * 8 /* FP64 potential */ + R * rs * 21 + R * 8 * 10 + (size_t)p->nsec * (rs * 3 + 8 * 2) +
+           R * 8 * 2 * FGX_NFORCE_T + sgb + 4096;
   return FGX_OK;
 }
 

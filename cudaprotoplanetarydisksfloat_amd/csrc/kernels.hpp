@@ -39,6 +39,7 @@ struct Dev {
   // ring centres, and the cell geometry of the potential
   const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d, *energymed_d;
   double* vmoy;  // ring-mean vtheta of the CFL condition
+  double* phi;   // potential of the planets and the indirect term, FP64 field
   // self-gravity: transform work rows, kernel transforms, twiddles, ring
   // mean of the radial SG acceleration, column-pair tiles (d, k)
   cxd* sgw;
@@ -52,6 +53,8 @@ struct Dev {
   int* summary;        // [0] active disks, [1] max ncfl
   const CellTile* ctiles;
   const RingTile* rtiles;
+  const CellTile* stiles;   // strips (d, first ring, sector block) of strip_rings rings
+  int strip_rings;
   int max_nsec, ndisk;
   // batch-wide mode flags
   int adiabatic, fast_transport, open_inner, nonreflecting, evanescent, self_gravity;
@@ -153,12 +156,38 @@ __device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, 
   const int rb = c.ro;                                                      \
   (void)nr; (void)rb; (void)fb;
 
+// a strip tile: one thread per sector walks rings [t.i, t.i + strip_rings)
+#define FGX_STRIP_BEGIN                                                     \
+  const CellTile t = D.stiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
+  if (t.d < 0) return;                                                      \
+  const DiskConst& c = D.dc[t.d];                                           \
+  DiskState& s = D.st[t.d];                                                 \
+  if (!D.ignore_active && !s.active) return;                                \
+  const int ns = c.nsec, nr = c.nrad;                                       \
+  const int j = t.jb * FGX_TPB + (int)threadIdx.x;                          \
+  if (j >= ns) return;                                                      \
+  const long long fb = c.fo;                                                \
+  const int rb = c.ro;                                                      \
+  const int i_end = (t.i + D.strip_rings < nr) ? t.i + D.strip_rings : nr;  \
+  (void)rb; (void)s; (void)c;                                               \
+  for (int i = t.i; i < i_end; ++i)
+
 #define AT(a, ii, jj) a[fb + (long long)(ii) * ns + (jj)]
 
 // potential of the planets and the indirect term at cell (i,j) in FP64:
 // FillForcesArraysKernel (Kernels.cu:1322-1349) for all planets in one
 // evaluation; the star's -1/r is handled per ring (its radial difference is
 // the FP64 table gstar, its azimuthal difference vanishes)
+// 1/sqrt(x) in FP64: the hardware estimate refined by two Newton steps
+// (relative error below 1e-16; the estimate alone is good to ~2^-29)
+__device__ __forceinline__ double rsqrt_d(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 template <class real>
 __device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const DiskConst& c, const DiskState& s,
                                                        double rmed, int j) {
@@ -167,7 +196,7 @@ __device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const
   for (int k = 0; k < c.nb; ++k) {
     const double xp = s.pot_x[k], yp = s.pot_y[k], mp = s.pot_m[k];
     const double dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
-    double p = -mp / sqrt(dist + s.pot_sm2[k]);
+    double p = -mp * rsqrt_d(dist + s.pot_sm2[k]);
     if (D.indirect) p += mp * s.pot_inv3[k] * (x * xp + y * yp);
     pot += p;
   }
@@ -514,8 +543,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_planets(Dev<real> D) {
 // CorrectVthetaKernel (Kernels.cu:1352-1359)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_correct_vtheta(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  AT(D.vtheta, i, j) -= (real)s.domega * D.rmed[rb + i];
+  FGX_STRIP_BEGIN {
+    AT(D.vtheta, i, j) -= (real)s.domega * D.rmed[rb + i];
+  }
 }
 
 // ============================================================ viscosity nu
@@ -543,10 +573,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_profile(Dev<real> D, int updat
 // ComputeSoundSpeedKernel, adiabatic branch (Kernels.cu:190-191)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_sound_speed(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const real g = (real)c.adiabaticindex;
-  AT(D.cs, i, j) = sqrt(g * (g - (real)1) * AT(D.energy, i, j) / AT(D.dens, i, j));
+  FGX_STRIP_BEGIN {
+    const real g = (real)c.adiabaticindex;
+    AT(D.cs, i, j) = sqrt(g * (g - (real)1) * AT(D.energy, i, j) / AT(D.dens, i, j));
+  }
 }
 
 // ================================================================ Substep1
@@ -557,51 +587,51 @@ __global__ void __launch_bounds__(FGX_TPB) k_sound_speed(Dev<real> D) {
 // fields and FP64 ring tables; vthetaint is stored as a residual of vref.
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const double dt = s.dt;
-  const double om = s.omega_frame;
-  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-  const double rmed = D.rmed_d[rb + i];
-  // pressure in FP64 from the FP32 state: Sigma*cs^2 with the exact
-  // isothermal cs^2 of the ring, or (gamma-1)*e
-  const double g1 = c.adiabaticindex - 1.0;
-  auto press = [&](int ii, int jj) -> double {
-    if (D.adiabatic) return g1 * (double)AT(D.energy, ii, jj);
-    return (double)AT(D.dens, ii, jj) * D.cs2_d[rb + ii];
-  };
-  const real sig = AT(D.dens, i, j);
-  const double p = press(i, j);
-  const double phi = potential_planets_at(D, c, s, rmed, j);
-  if (D.write_pressure) AT(D.pressure, i, j) = (real)p;
-  // radial
-  if (i > 0) {
-    const double rm1 = D.rmed_d[rb + i - 1];
-    const real sigm = AT(D.dens, i - 1, j);
-    const double pm = press(i - 1, j);
-    const double phim = potential_planets_at(D, c, s, rm1, j);
-    const double idr = D.invdiffrmed_d[rb + i];
-    const double gradp = (p - pm) * 2.0 / ((double)sig + (double)sigm) * idr;
-    const double gradphi = (phi - phim) * idr + D.gstar[rb + i];
-    double vt2 = 2.0 * D.vref[rb + i] + 2.0 * D.vref[rb + i - 1] +
-                 (double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) +
-                 (double)AT(D.vtheta, i - 1, j) + (double)AT(D.vtheta, i - 1, jp);
-    vt2 = vt2 / 4.0 + D.rinf_d[rb + i] * om;
-    vt2 = vt2 * vt2;
-    AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * D.invrinf_d[rb + i]));
-  } else {
-    AT(D.vradint, i, j) = 0;
+  FGX_STRIP_BEGIN {
+    const double dt = s.dt;
+    const double om = s.omega_frame;
+    const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+    const double rmed = D.rmed_d[rb + i];
+    // pressure in FP64 from the FP32 state: Sigma*cs^2 with the exact
+    // isothermal cs^2 of the ring, or (gamma-1)*e
+    const double g1 = c.adiabaticindex - 1.0;
+    auto press = [&](int ii, int jj) -> double {
+      if (D.adiabatic) return g1 * (double)AT(D.energy, ii, jj);
+      return (double)AT(D.dens, ii, jj) * D.cs2_d[rb + ii];
+    };
+    const real sig = AT(D.dens, i, j);
+    const double p = press(i, j);
+    const double* phif = D.phi + fb;
+    const double phi = phif[(long long)i * ns + j];
+    if (D.write_pressure) AT(D.pressure, i, j) = (real)p;
+    // radial
+    if (i > 0) {
+      const real sigm = AT(D.dens, i - 1, j);
+      const double pm = press(i - 1, j);
+      const double phim = phif[(long long)(i - 1) * ns + j];
+      const double idr = D.invdiffrmed_d[rb + i];
+      const double gradp = (p - pm) * 2.0 / ((double)sig + (double)sigm) * idr;
+      const double gradphi = (phi - phim) * idr + D.gstar[rb + i];
+      double vt2 = 2.0 * D.vref[rb + i] + 2.0 * D.vref[rb + i - 1] +
+                   (double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) +
+                   (double)AT(D.vtheta, i - 1, j) + (double)AT(D.vtheta, i - 1, jp);
+      vt2 = vt2 / 4.0 + D.rinf_d[rb + i] * om;
+      vt2 = vt2 * vt2;
+      AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * D.invrinf_d[rb + i]));
+    } else {
+      AT(D.vradint, i, j) = 0;
+    }
+    // azimuthal (the star's potential is the same at j and j-1)
+    const real invdxt = (real)1 / D.dxtheta[rb + i];
+    const real sigl = AT(D.dens, i, jm);
+    const double pl = press(i, jm);
+    const double phil = phif[(long long)i * ns + jm];
+    const real gradp = (real)((p - pl) * 2.0 / ((double)sig + (double)sigl)) * invdxt;
+    const real gradphi = (real)(phi - phil) * invdxt;
+    real v = AT(D.vtheta, i, j) - (real)dt * (gradp + gradphi);
+    v += (real)dt * D.supp_torque[rb + i];
+    AT(D.vthetaint, i, j) = v;
   }
-  // azimuthal (the star's potential is the same at j and j-1)
-  const real invdxt = (real)1 / D.dxtheta[rb + i];
-  const real sigl = AT(D.dens, i, jm);
-  const double pl = press(i, jm);
-  const double phil = potential_planets_at(D, c, s, rmed, jm);
-  const real gradp = (real)((p - pl) * 2.0 / ((double)sig + (double)sigl)) * invdxt;
-  const real gradphi = (real)(phi - phil) * invdxt;
-  real v = AT(D.vtheta, i, j) - (real)dt * (gradp + gradphi);
-  v += (real)dt * D.supp_torque[rb + i];
-  AT(D.vthetaint, i, j) = v;
 }
 
 // ============================================================ viscous terms
@@ -611,36 +641,36 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* __restrict__ vr,
                                                       const real* __restrict__ vt, int write_divv) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-  const real invdphi = (real)((double)ns / (2.0 * kPI));
-  const real irm = D.invrmed[rb + i];
-  const real vr0 = AT(vr, i, j), vr1 = AT(vr, i + 1, j);
-  const real vt0 = AT(vt, i, j), vtp = AT(vt, i, jp);
-  const real drr = (vr1 - vr0) * D.invdiffrsup[rb + i];
-  const real dpp = (vtp - vt0) * invdphi * irm + (real)0.5 * (vr1 + vr0) * irm;
-  real divv = (vr1 * D.rsup[rb + i] - vr0 * D.rinf[rb + i]) * D.invdiffrsup[rb + i] * irm;
-  divv += (vtp - vt0) * invdphi * irm;
-  const real nu = D.visc[rb + i];
-  const real sig = AT(D.dens, i, j);
-  const real third = (real)(1.0 / 3.0);
-  AT(D.taurr, i, j) = (real)2 * sig * nu * (drr - third * divv);
-  AT(D.taupp, i, j) = (real)2 * sig * nu * (dpp - third * divv);
-  if (write_divv) AT(D.divv, i, j) = divv;
-  if (i > 0) {
-    // the angular velocity difference between rings is formed in FP64 from
-    // the residuals and the reference profile
-    const double w0 = (D.vref[rb + i] + (double)vt0) / D.rmed_d[rb + i];
-    const double w1 = (D.vref[rb + i - 1] + (double)AT(vt, i - 1, j)) / D.rmed_d[rb + i - 1];
-    const real drp = (real)0.5 * ((real)(D.rinf_d[rb + i] * (w0 - w1) * D.invdiffrmed_d[rb + i]) +
-                                  (vr0 - AT(vr, i, jm)) * invdphi * D.invrinf[rb + i]);
-    // corner density: column 0 of ring i as the reference indexes it
-    // (Kernels.cu:876), or column j-1 with the quirk fix
-    const real sc = D.fix_taurp ? AT(D.dens, i, jm) : AT(D.dens, i, 0);
-    AT(D.taurp, i, j) = (real)(2.0 * 0.25) * (sig + AT(D.dens, i - 1, j) + sc + AT(D.dens, i - 1, jm)) * nu * drp;
-  } else {
-    AT(D.taurp, i, j) = 0;
+  FGX_STRIP_BEGIN {
+    const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+    const real invdphi = (real)((double)ns / (2.0 * kPI));
+    const real irm = D.invrmed[rb + i];
+    const real vr0 = AT(vr, i, j), vr1 = AT(vr, i + 1, j);
+    const real vt0 = AT(vt, i, j), vtp = AT(vt, i, jp);
+    const real drr = (vr1 - vr0) * D.invdiffrsup[rb + i];
+    const real dpp = (vtp - vt0) * invdphi * irm + (real)0.5 * (vr1 + vr0) * irm;
+    real divv = (vr1 * D.rsup[rb + i] - vr0 * D.rinf[rb + i]) * D.invdiffrsup[rb + i] * irm;
+    divv += (vtp - vt0) * invdphi * irm;
+    const real nu = D.visc[rb + i];
+    const real sig = AT(D.dens, i, j);
+    const real third = (real)(1.0 / 3.0);
+    AT(D.taurr, i, j) = (real)2 * sig * nu * (drr - third * divv);
+    AT(D.taupp, i, j) = (real)2 * sig * nu * (dpp - third * divv);
+    if (write_divv) AT(D.divv, i, j) = divv;
+    if (i > 0) {
+      // the angular velocity difference between rings is formed in FP64 from
+      // the residuals and the reference profile
+      const double w0 = (D.vref[rb + i] + (double)vt0) / D.rmed_d[rb + i];
+      const double w1 = (D.vref[rb + i - 1] + (double)AT(vt, i - 1, j)) / D.rmed_d[rb + i - 1];
+      const real drp = (real)0.5 * ((real)(D.rinf_d[rb + i] * (w0 - w1) * D.invdiffrmed_d[rb + i]) +
+                                    (vr0 - AT(vr, i, jm)) * invdphi * D.invrinf[rb + i]);
+      // corner density: column 0 of ring i as the reference indexes it
+      // (Kernels.cu:876), or column j-1 with the quirk fix
+      const real sc = D.fix_taurp ? AT(D.dens, i, jm) : AT(D.dens, i, 0);
+      AT(D.taurp, i, j) = (real)(2.0 * 0.25) * (sig + AT(D.dens, i - 1, j) + sc + AT(D.dens, i - 1, jm)) * nu * drp;
+    } else {
+      AT(D.taurp, i, j) = 0;
+    }
   }
 }
 
@@ -648,36 +678,36 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
 // (SideEuler.cu:250-272, Kernels.cu:1490-1503)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const real dt = (real)s.dt;
-  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-  const real invdphi = (real)((double)ns / (2.0 * kPI));
-  if (D.viscous && i > 0 && i < nr - 1) {
-    const real trp0 = AT(D.taurp, i, j), trp1 = AT(D.taurp, i + 1, j);
-    AT(D.vthetaint, i, j) +=
-        dt * D.invrmed[rb + i] *
-        ((D.rsup[rb + i] * trp1 - D.rinf[rb + i] * trp0) * D.invdiffrsup[rb + i] +
-         (AT(D.taupp, i, j) - AT(D.taupp, i, jm)) * invdphi + (real)0.5 * (trp0 + trp1)) /
-        ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i, jm)));
-  }
-  if (D.viscous && i > 0) {
-    AT(D.vradint, i, j) +=
-        dt * D.invrinf[rb + i] *
-        ((D.rmed[rb + i] * AT(D.taurr, i, j) - D.rmed[rb + i - 1] * AT(D.taurr, i - 1, j)) * D.invdiffrmed[rb + i] +
-         (AT(D.taurp, i, jp) - AT(D.taurp, i, j)) * invdphi - (real)0.5 * (AT(D.taupp, i, j) + AT(D.taupp, i - 1, j))) /
-        ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
-  }
-  if (!D.evanescent) {
-    // with self-gravity the Keplerian values include -r * <SG_Accr>_ring
-    // (ApplySubKeplerianBoundary, SideEuler.cu:159-166)
-    if (i == 0) {
-      const double v = s.vkep_in - (D.self_gravity ? D.rmed_d[rb] * D.axisg[rb] : 0.0);
-      AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb] * s.omega_frame - D.vref[rb]);
+  FGX_STRIP_BEGIN {
+    const real dt = (real)s.dt;
+    const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+    const real invdphi = (real)((double)ns / (2.0 * kPI));
+    if (D.viscous && i > 0 && i < nr - 1) {
+      const real trp0 = AT(D.taurp, i, j), trp1 = AT(D.taurp, i + 1, j);
+      AT(D.vthetaint, i, j) +=
+          dt * D.invrmed[rb + i] *
+          ((D.rsup[rb + i] * trp1 - D.rinf[rb + i] * trp0) * D.invdiffrsup[rb + i] +
+           (AT(D.taupp, i, j) - AT(D.taupp, i, jm)) * invdphi + (real)0.5 * (trp0 + trp1)) /
+          ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i, jm)));
     }
-    if (i == nr - 1) {
-      const double v = s.vkep_out - (D.self_gravity ? D.rmed_d[rb + nr - 1] * D.axisg[rb + nr - 1] : 0.0);
-      AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb + nr - 1] * s.omega_frame - D.vref[rb + nr - 1]);
+    if (D.viscous && i > 0) {
+      AT(D.vradint, i, j) +=
+          dt * D.invrinf[rb + i] *
+          ((D.rmed[rb + i] * AT(D.taurr, i, j) - D.rmed[rb + i - 1] * AT(D.taurr, i - 1, j)) * D.invdiffrmed[rb + i] +
+           (AT(D.taurp, i, jp) - AT(D.taurp, i, j)) * invdphi - (real)0.5 * (AT(D.taupp, i, j) + AT(D.taupp, i - 1, j))) /
+          ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
+    }
+    if (!D.evanescent) {
+      // with self-gravity the Keplerian values include -r * <SG_Accr>_ring
+      // (ApplySubKeplerianBoundary, SideEuler.cu:159-166)
+      if (i == 0) {
+        const double v = s.vkep_in - (D.self_gravity ? D.rmed_d[rb] * D.axisg[rb] : 0.0);
+        AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb] * s.omega_frame - D.vref[rb]);
+      }
+      if (i == nr - 1) {
+        const double v = s.vkep_out - (D.self_gravity ? D.rmed_d[rb + nr - 1] * D.axisg[rb + nr - 1] : 0.0);
+        AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb + nr - 1] * s.omega_frame - D.vref[rb + nr - 1]);
+      }
     }
   }
 }
@@ -688,34 +718,34 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
 // velocities straight into the state (ActualiseGasVrad/Vtheta become no-ops)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_substep2(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const real dt = (real)s.dt;
-  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-  const real cv2 = (real)(kCVNR * kCVNR);
-  const real sig = AT(D.dens, i, j);
-  const real vri = AT(D.vradint, i, j), vrp = AT(D.vradint, i + 1, j);
-  real dv = vrp - vri;
-  const real qr = dv < 0 ? cv2 * sig * dv * dv : (real)0;
-  const real vti = AT(D.vthetaint, i, j), vtp = AT(D.vthetaint, i, jp), vtm = AT(D.vthetaint, i, jm);
-  dv = vtp - vti;
-  const real qt = dv < 0 ? cv2 * sig * dv * dv : (real)0;
-  const real sigl = AT(D.dens, i, jm);
-  dv = vti - vtm;
-  const real qtl = dv < 0 ? cv2 * sigl * dv * dv : (real)0;
-  const real invdxt = (real)1 / D.dxtheta[rb + i];
-  if (i > 0) {
-    const real sigm = AT(D.dens, i - 1, j);
-    dv = vri - AT(D.vradint, i - 1, j);
-    const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
-    AT(D.vrad, i, j) = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * D.invdiffrmed[rb + i];
-  } else {
-    AT(D.vrad, i, j) = 0;
+  FGX_STRIP_BEGIN {
+    const real dt = (real)s.dt;
+    const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+    const real cv2 = (real)(kCVNR * kCVNR);
+    const real sig = AT(D.dens, i, j);
+    const real vri = AT(D.vradint, i, j), vrp = AT(D.vradint, i + 1, j);
+    real dv = vrp - vri;
+    const real qr = dv < 0 ? cv2 * sig * dv * dv : (real)0;
+    const real vti = AT(D.vthetaint, i, j), vtp = AT(D.vthetaint, i, jp), vtm = AT(D.vthetaint, i, jm);
+    dv = vtp - vti;
+    const real qt = dv < 0 ? cv2 * sig * dv * dv : (real)0;
+    const real sigl = AT(D.dens, i, jm);
+    dv = vti - vtm;
+    const real qtl = dv < 0 ? cv2 * sigl * dv * dv : (real)0;
+    const real invdxt = (real)1 / D.dxtheta[rb + i];
+    if (i > 0) {
+      const real sigm = AT(D.dens, i - 1, j);
+      dv = vri - AT(D.vradint, i - 1, j);
+      const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
+      AT(D.vrad, i, j) = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * D.invdiffrmed[rb + i];
+    } else {
+      AT(D.vrad, i, j) = 0;
+    }
+    AT(D.vtheta, i, j) = vti - dt * (real)2 / (sig + sigl) * (qt - qtl) * invdxt;
+    if (D.adiabatic)
+      AT(D.energyint, i, j) = AT(D.energy, i, j) - dt * qr * (vrp - vri) * D.invdiffrsup[rb + i] -
+                              dt * qt * (vtp - vti) * invdxt;
   }
-  AT(D.vtheta, i, j) = vti - dt * (real)2 / (sig + sigl) * (qt - qtl) * invdxt;
-  if (D.adiabatic)
-    AT(D.energyint, i, j) = AT(D.energy, i, j) - dt * qr * (vrp - vri) * D.invdiffrsup[rb + i] -
-                            dt * qt * (vtp - vti) * invdxt;
 }
 
 // ================================================================ Substep3
@@ -732,29 +762,29 @@ __device__ __forceinline__ real qplus_at(const Dev<real>& D, const DiskConst& c,
 }
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const real dt = (real)s.dt;
-  real qp;
-  if (i > 0) qp = qplus_at(D, c, rb, fb, ns, i, j);
-  else if (D.visc[rb + nr - 1] != (real)0) {
-    const real q1 = qplus_at(D, c, rb, fb, ns, 1, j), q2 = qplus_at(D, c, rb, fb, ns, 2, j);
-    qp = q1 * (real)exp((double)log(q1 / q2) * log(D.rmed_d[rb] / D.rmed_d[rb + 1]) / log(D.rmed_d[rb + 1] / D.rmed_d[rb + 2]));
-  } else qp = 0;
-  AT(D.qplus, i, j) = qp;
-  const real g1 = (real)c.adiabaticindex - (real)1;
-  const real eint = AT(D.energyint, i, j), dvv = AT(D.divv, i, j);
-  real en;
-  if (!D.cooling) {
-    en = (dt * qp + eint) / ((real)1 + g1 * dt * dvv);
-  } else {
-    const real sig = AT(D.dens, i, j);
-    const real tc = D.coolingtime[rb + i], sm = D.sigmamed[rb + i];
-    const real num = D.energymed[rb + i] * dt * sig / sm + tc * eint + dt * tc * (qp - D.qplusmed[rb + i] * sig / sm);
-    const real den = dt + tc + g1 * dt * tc * dvv;
-    en = num / den;
+  FGX_STRIP_BEGIN {
+    const real dt = (real)s.dt;
+    real qp;
+    if (i > 0) qp = qplus_at(D, c, rb, fb, ns, i, j);
+    else if (D.visc[rb + nr - 1] != (real)0) {
+      const real q1 = qplus_at(D, c, rb, fb, ns, 1, j), q2 = qplus_at(D, c, rb, fb, ns, 2, j);
+      qp = q1 * (real)exp((double)log(q1 / q2) * log(D.rmed_d[rb] / D.rmed_d[rb + 1]) / log(D.rmed_d[rb + 1] / D.rmed_d[rb + 2]));
+    } else qp = 0;
+    AT(D.qplus, i, j) = qp;
+    const real g1 = (real)c.adiabaticindex - (real)1;
+    const real eint = AT(D.energyint, i, j), dvv = AT(D.divv, i, j);
+    real en;
+    if (!D.cooling) {
+      en = (dt * qp + eint) / ((real)1 + g1 * dt * dvv);
+    } else {
+      const real sig = AT(D.dens, i, j);
+      const real tc = D.coolingtime[rb + i], sm = D.sigmamed[rb + i];
+      const real num = D.energymed[rb + i] * dt * sig / sm + tc * eint + dt * tc * (qp - D.qplusmed[rb + i] * sig / sm);
+      const real den = dt + tc + g1 * dt * tc * dvv;
+      en = num / den;
+    }
+    AT(D.energy, i, j) = en;
   }
-  AT(D.energy, i, j) = en;
 }
 
 // ============================================================== boundaries
@@ -849,30 +879,30 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
 // EvanescentBoundaryKernel (Kernels.cu:1076-1124)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  const double r = D.rmed_d[rb + i];
-  if (!(r < s.ev_drmin || r > s.ev_drmax)) return;
-  const double dt = s.dt;
-  double lambda = 0, damping;
-  if (r < s.ev_drmin) {
-    damping = (r - s.ev_drmin) / (D.rmed_d[rb] - s.ev_drmin);
-    lambda = damping * damping * 10.0 * dt / s.ev_tin;
+  FGX_STRIP_BEGIN {
+    const double r = D.rmed_d[rb + i];
+    if (!(r < s.ev_drmin || r > s.ev_drmax)) continue;
+    const double dt = s.dt;
+    double lambda = 0, damping;
+    if (r < s.ev_drmin) {
+      damping = (r - s.ev_drmin) / (D.rmed_d[rb] - s.ev_drmin);
+      lambda = damping * damping * 10.0 * dt / s.ev_tin;
+    }
+    if (r > s.ev_drmax) {
+      damping = (r - s.ev_drmax) / (D.rmed_d[rb + nr - 1] - s.ev_drmax);
+      lambda = damping * damping * 10.0 * dt / s.ev_tout;
+    }
+    const double h = aspect_ratio_dev(c, r, s.time - s.time0);
+    double vt0 = sqrt(1.0 / r * (1.0 - (1.0 + c.sigmaslope - 2.0 * c.flaringindex) * h * h * pow(r, 2.0 * c.flaringindex)));
+    vt0 -= r * s.omega_frame;
+    const double vr0 = -3.0 * (double)D.visc[rb + i] / r * (-c.sigmaslope + .5);
+    // relaxation towards the FP64 initial profiles, evaluated in FP64
+    const double inv = 1.0 / (1.0 + lambda);
+    AT(D.vrad, i, j) = (real)(((double)AT(D.vrad, i, j) + lambda * vr0) * inv);
+    AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - D.vref[rb + i])) * inv);
+    AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * D.sigmamed_d[rb + i]) * inv);
+    if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * D.energymed_d[rb + i]) * inv);
   }
-  if (r > s.ev_drmax) {
-    damping = (r - s.ev_drmax) / (D.rmed_d[rb + nr - 1] - s.ev_drmax);
-    lambda = damping * damping * 10.0 * dt / s.ev_tout;
-  }
-  const double h = aspect_ratio_dev(c, r, s.time - s.time0);
-  double vt0 = sqrt(1.0 / r * (1.0 - (1.0 + c.sigmaslope - 2.0 * c.flaringindex) * h * h * pow(r, 2.0 * c.flaringindex)));
-  vt0 -= r * s.omega_frame;
-  const double vr0 = -3.0 * (double)D.visc[rb + i] / r * (-c.sigmaslope + .5);
-  // relaxation towards the FP64 initial profiles, evaluated in FP64
-  const double inv = 1.0 / (1.0 + lambda);
-  AT(D.vrad, i, j) = (real)(((double)AT(D.vrad, i, j) + lambda * vr0) * inv);
-  AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - D.vref[rb + i])) * inv);
-  AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * D.sigmamed_d[rb + i]) * inv);
-  if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * D.energymed_d[rb + i]) * inv);
 }
 
 // ====================================================== azimuthal transport
@@ -1052,10 +1082,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
 // radial half of ComputeVelocitiesKernel (Kernels.cu:1299-1302)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_trans_vr(Dev<real> D) {
-  FGX_CELL_PROLOGUE
-  if (i >= nr) return;
-  if (i == 0) { AT(D.vrad, i, j) = 0; return; }
-  AT(D.vrad, i, j) = (AT(D.t_rmp, i - 1, j) + AT(D.t_rmm, i, j)) / (AT(D.dens, i, j) + AT(D.dens, i - 1, j) + (real)1e-20);
+  FGX_STRIP_BEGIN {
+    if (i == 0) { AT(D.vrad, i, j) = 0; continue; }
+    AT(D.vrad, i, j) = (AT(D.t_rmp, i - 1, j) + AT(D.t_rmm, i, j)) / (AT(D.dens, i, j) + AT(D.dens, i - 1, j) + (real)1e-20);
+  }
 }
 
 // ============================================================ bookkeeping
@@ -1111,6 +1141,14 @@ __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
 
 // ============================================================== diagnostics
 // potential of the current substep into D.work (FillForcesArrays output)
+// FillForcesArraysKernel (Kernels.cu:1322-1349) for the planets and the
+// indirect term, once per cell and substep, in FP64
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_potential(Dev<real> D) {
+  FGX_STRIP_BEGIN {
+    D.phi[fb + (long long)i * ns + j] = potential_planets_at(D, c, s, D.rmed_d[rb + i], j);
+  }
+}
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_potential_field(Dev<real> D) {
   FGX_CELL_PROLOGUE
