@@ -63,6 +63,25 @@ struct Dev {
 };
 
 // ---- small helpers ------------------------------------------------------
+// Data that no thread writes during a kernel (ring and sector tables, the
+// per-disk constants, the per-substep disk state in the kernels that only read
+// it) is read through the constant address space.  Uniform reads of it then
+// use the scalar cache even after the kernel's own stores; through a plain
+// pointer the compiler cannot rule out that a store changed it, issues vector
+// loads instead, and has to drain every outstanding load and store to wait
+// for them.
+// (The host pass of hipcc also parses kernel bodies; there the qualifier is
+// dropped.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FGX_AS4 __attribute__((address_space(4)))
+#else
+#define FGX_AS4
+#endif
+template <class T>
+__device__ __forceinline__ const FGX_AS4 T& cref(const T* p) { return *(const FGX_AS4 T*)p; }
+template <class T>
+__device__ __forceinline__ const FGX_AS4 T* cptr(const T* p) { return (const FGX_AS4 T*)p; }
+#define RT(name) cptr(D.name)
 __device__ __forceinline__ int xcd_tile(int b, int n) {
   // n is a multiple of 8; hardware hands blocks to the 8 XCDs round-robin
   const int per = n >> 3;
@@ -133,8 +152,8 @@ __device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, 
 #define FGX_CELL_PROLOGUE                                                   \
   const CellTile t = D.ctiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
   if (t.d < 0) return;                                                      \
-  const DiskConst& c = D.dc[t.d];                                           \
-  DiskState& s = D.st[t.d];                                                 \
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);                           \
+  const FGX_AS4 DiskState& s = cref(D.st + t.d);                            \
   if (!D.ignore_active && !s.active) return;                                \
   const int ns = c.nsec, nr = c.nrad;                                       \
   const int i = t.i;                                                        \
@@ -147,8 +166,8 @@ __device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, 
 #define FGX_RING_PROLOGUE                                                   \
   const RingTile t = D.rtiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
   if (t.d < 0) return;                                                      \
-  const DiskConst& c = D.dc[t.d];                                           \
-  DiskState& s = D.st[t.d];                                                 \
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);                           \
+  const FGX_AS4 DiskState& s = cref(D.st + t.d);                            \
   if (!D.ignore_active && !s.active) return;                                \
   const int ns = c.nsec, nr = c.nrad;                                       \
   const int i = t.i;                                                        \
@@ -160,8 +179,8 @@ __device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, 
 #define FGX_STRIP_BEGIN                                                     \
   const CellTile t = D.stiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
   if (t.d < 0) return;                                                      \
-  const DiskConst& c = D.dc[t.d];                                           \
-  DiskState& s = D.st[t.d];                                                 \
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);                           \
+  const FGX_AS4 DiskState& s = cref(D.st + t.d);                            \
   if (!D.ignore_active && !s.active) return;                                \
   const int ns = c.nsec, nr = c.nrad;                                       \
   const int j = t.jb * FGX_TPB + (int)threadIdx.x;                          \
@@ -191,7 +210,7 @@ __device__ __forceinline__ double rsqrt_d(double x) {
 template <class real>
 __device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const DiskConst& c, const DiskState& s,
                                                        double rmed, int j) {
-  const double x = rmed * D.cosj_d[c.so + j], y = rmed * D.sinj_d[c.so + j];
+  const double x = rmed * RT(cosj_d)[c.so + j], y = rmed * RT(sinj_d)[c.so + j];
   double pot = 0;
   for (int k = 0; k < c.nb; ++k) {
     const double xp = s.pot_x[k], yp = s.pot_y[k], mp = s.pot_m[k];
@@ -214,15 +233,15 @@ __global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
   __shared__ double redd[FGX_TPB / 64];
   __shared__ real redr[FGX_TPB / 64];
   const long long row = fb + (long long)i * ns;
-  const double vr0 = D.vref[rb + i];
+  const double vr0 = RT(vref)[rb + i];
   double acc = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)D.vtheta[row + j];
   acc = block_sum(acc, redd);
   const double mres = acc / (double)ns;   // ring mean of the residual
   if (threadIdx.x == 0) D.vmoy[rb + i] = vr0 + mres;
   if (i == 0) return;
-  const real dxrad = D.rsup[rb + i] - D.rinf[rb + i];
-  const real dxt = D.dxtheta[rb + i];
+  const real dxrad = RT(rsup)[rb + i] - RT(rinf)[rb + i];
+  const real dxt = RT(dxtheta)[rb + i];
   const real mind = dxrad < dxt ? dxrad : dxt;
   real mn = (real)1e30;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
@@ -272,7 +291,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
       mn = v < mn ? v : mn;
     }
     for (int i = threadIdx.x; i < nr - 1; i += FGX_TPB) {
-      const double a = D.vmoy[rb + i] / D.rmed_d[rb + i] - D.vmoy[rb + i + 1] / D.rmed_d[rb + i + 1];
+      const double a = D.vmoy[rb + i] / RT(rmed_d)[rb + i] - D.vmoy[rb + i + 1] / RT(rmed_d)[rb + i + 1];
       const real v = (real)(2.0 * kPI * kCFLSECURITY / (double)c.nsec / fabs(a));
       mn = v < mn ? v : mn;
     }
@@ -317,7 +336,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
   }
   s.ntarget = nt;
   // ApplySubKeplerianBoundary values (SideEuler.cu:250-268), FP64
-  const double r0 = D.rmed_d[rb], rn = D.rmed_d[rb + nr - 1];
+  const double r0 = RT(rmed_d)[rb], rn = RT(rmed_d)[rb + nr - 1];
   const double fac = 1.0 + c.sigmaslope - 2.0 * c.flaringindex;
   const double h0 = aspect_ratio_dev(c, r0, tt), hn = aspect_ratio_dev(c, rn, tt);
   s.vkep_in = 1.0 / r0 * (1.0 - fac * h0 * h0 * pow(r0, 2.0 * c.flaringindex));
@@ -337,7 +356,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_force_ring(Dev<real> D) {
   FGX_RING_PROLOGUE
   __shared__ double redd[FGX_TPB / 64];
   const long long row = fb + (long long)i * ns;
-  const real rmed = D.rmed[rb + i], surf = D.surf[rb + i];
+  const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
   double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
   const int nt = s.ntarget;
   for (int q = 0; q < nt; ++q) {
@@ -346,7 +365,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_force_ring(Dev<real> D) {
     double fx = 0, fy = 0;
     for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
       const real cm = surf * D.dens[row + j];
-      const real dx = rmed * D.cosj[c.so + j] - tx, dy = rmed * D.sinj[c.so + j] - ty;
+      const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
       const real d2 = dx * dx + dy * dy;
       const real dist2 = d2 + sm2;
       const real inv3 = (real)1 / dist2 / sqrt(dist2);
@@ -544,7 +563,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_planets(Dev<real> D) {
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_correct_vtheta(Dev<real> D) {
   FGX_STRIP_BEGIN {
-    AT(D.vtheta, i, j) -= (real)s.domega * D.rmed[rb + i];
+    AT(D.vtheta, i, j) -= (real)s.domega * RT(rmed)[rb + i];
   }
 }
 
@@ -564,7 +583,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_profile(Dev<real> D, int updat
     if (threadIdx.x == 0) D.csmean[rb + i] = (real)acc;
   }
   if (threadIdx.x == 0) {
-    const double nu = fviscosity_dev(c, alpha, D.rmed_d[rb + i], (double)D.csmean[rb + i], s.time - s.time0);
+    const double nu = fviscosity_dev(c, alpha, RT(rmed_d)[rb + i], (double)D.csmean[rb + i], s.time - s.time0);
     D.visc[rb + i] = (real)nu;
   }
 }
@@ -591,13 +610,13 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
     const double dt = s.dt;
     const double om = s.omega_frame;
     const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-    const double rmed = D.rmed_d[rb + i];
+    const double rmed = RT(rmed_d)[rb + i];
     // pressure in FP64 from the FP32 state: Sigma*cs^2 with the exact
     // isothermal cs^2 of the ring, or (gamma-1)*e
     const double g1 = c.adiabaticindex - 1.0;
     auto press = [&](int ii, int jj) -> double {
       if (D.adiabatic) return g1 * (double)AT(D.energy, ii, jj);
-      return (double)AT(D.dens, ii, jj) * D.cs2_d[rb + ii];
+      return (double)AT(D.dens, ii, jj) * RT(cs2_d)[rb + ii];
     };
     const real sig = AT(D.dens, i, j);
     const double p = press(i, j);
@@ -609,27 +628,27 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
       const real sigm = AT(D.dens, i - 1, j);
       const double pm = press(i - 1, j);
       const double phim = phif[(long long)(i - 1) * ns + j];
-      const double idr = D.invdiffrmed_d[rb + i];
+      const double idr = RT(invdiffrmed_d)[rb + i];
       const double gradp = (p - pm) * 2.0 / ((double)sig + (double)sigm) * idr;
-      const double gradphi = (phi - phim) * idr + D.gstar[rb + i];
-      double vt2 = 2.0 * D.vref[rb + i] + 2.0 * D.vref[rb + i - 1] +
+      const double gradphi = (phi - phim) * idr + RT(gstar)[rb + i];
+      double vt2 = 2.0 * RT(vref)[rb + i] + 2.0 * RT(vref)[rb + i - 1] +
                    (double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) +
                    (double)AT(D.vtheta, i - 1, j) + (double)AT(D.vtheta, i - 1, jp);
-      vt2 = vt2 / 4.0 + D.rinf_d[rb + i] * om;
+      vt2 = vt2 / 4.0 + RT(rinf_d)[rb + i] * om;
       vt2 = vt2 * vt2;
-      AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * D.invrinf_d[rb + i]));
+      AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * RT(invrinf_d)[rb + i]));
     } else {
       AT(D.vradint, i, j) = 0;
     }
     // azimuthal (the star's potential is the same at j and j-1)
-    const real invdxt = (real)1 / D.dxtheta[rb + i];
+    const real invdxt = (real)1 / RT(dxtheta)[rb + i];
     const real sigl = AT(D.dens, i, jm);
     const double pl = press(i, jm);
     const double phil = phif[(long long)i * ns + jm];
     const real gradp = (real)((p - pl) * 2.0 / ((double)sig + (double)sigl)) * invdxt;
     const real gradphi = (real)(phi - phil) * invdxt;
     real v = AT(D.vtheta, i, j) - (real)dt * (gradp + gradphi);
-    v += (real)dt * D.supp_torque[rb + i];
+    v += (real)dt * RT(supp_torque)[rb + i];
     AT(D.vthetaint, i, j) = v;
   }
 }
@@ -644,14 +663,14 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
   FGX_STRIP_BEGIN {
     const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
     const real invdphi = (real)((double)ns / (2.0 * kPI));
-    const real irm = D.invrmed[rb + i];
+    const real irm = RT(invrmed)[rb + i];
     const real vr0 = AT(vr, i, j), vr1 = AT(vr, i + 1, j);
     const real vt0 = AT(vt, i, j), vtp = AT(vt, i, jp);
-    const real drr = (vr1 - vr0) * D.invdiffrsup[rb + i];
+    const real drr = (vr1 - vr0) * RT(invdiffrsup)[rb + i];
     const real dpp = (vtp - vt0) * invdphi * irm + (real)0.5 * (vr1 + vr0) * irm;
-    real divv = (vr1 * D.rsup[rb + i] - vr0 * D.rinf[rb + i]) * D.invdiffrsup[rb + i] * irm;
+    real divv = (vr1 * RT(rsup)[rb + i] - vr0 * RT(rinf)[rb + i]) * RT(invdiffrsup)[rb + i] * irm;
     divv += (vtp - vt0) * invdphi * irm;
-    const real nu = D.visc[rb + i];
+    const real nu = RT(visc)[rb + i];
     const real sig = AT(D.dens, i, j);
     const real third = (real)(1.0 / 3.0);
     AT(D.taurr, i, j) = (real)2 * sig * nu * (drr - third * divv);
@@ -660,10 +679,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
     if (i > 0) {
       // the angular velocity difference between rings is formed in FP64 from
       // the residuals and the reference profile
-      const double w0 = (D.vref[rb + i] + (double)vt0) / D.rmed_d[rb + i];
-      const double w1 = (D.vref[rb + i - 1] + (double)AT(vt, i - 1, j)) / D.rmed_d[rb + i - 1];
-      const real drp = (real)0.5 * ((real)(D.rinf_d[rb + i] * (w0 - w1) * D.invdiffrmed_d[rb + i]) +
-                                    (vr0 - AT(vr, i, jm)) * invdphi * D.invrinf[rb + i]);
+      const double w0 = (RT(vref)[rb + i] + (double)vt0) / RT(rmed_d)[rb + i];
+      const double w1 = (RT(vref)[rb + i - 1] + (double)AT(vt, i - 1, j)) / RT(rmed_d)[rb + i - 1];
+      const real drp = (real)0.5 * ((real)(RT(rinf_d)[rb + i] * (w0 - w1) * RT(invdiffrmed_d)[rb + i]) +
+                                    (vr0 - AT(vr, i, jm)) * invdphi * RT(invrinf)[rb + i]);
       // corner density: column 0 of ring i as the reference indexes it
       // (Kernels.cu:876), or column j-1 with the quirk fix
       const real sc = D.fix_taurp ? AT(D.dens, i, jm) : AT(D.dens, i, 0);
@@ -685,15 +704,15 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
     if (D.viscous && i > 0 && i < nr - 1) {
       const real trp0 = AT(D.taurp, i, j), trp1 = AT(D.taurp, i + 1, j);
       AT(D.vthetaint, i, j) +=
-          dt * D.invrmed[rb + i] *
-          ((D.rsup[rb + i] * trp1 - D.rinf[rb + i] * trp0) * D.invdiffrsup[rb + i] +
+          dt * RT(invrmed)[rb + i] *
+          ((RT(rsup)[rb + i] * trp1 - RT(rinf)[rb + i] * trp0) * RT(invdiffrsup)[rb + i] +
            (AT(D.taupp, i, j) - AT(D.taupp, i, jm)) * invdphi + (real)0.5 * (trp0 + trp1)) /
           ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i, jm)));
     }
     if (D.viscous && i > 0) {
       AT(D.vradint, i, j) +=
-          dt * D.invrinf[rb + i] *
-          ((D.rmed[rb + i] * AT(D.taurr, i, j) - D.rmed[rb + i - 1] * AT(D.taurr, i - 1, j)) * D.invdiffrmed[rb + i] +
+          dt * RT(invrinf)[rb + i] *
+          ((RT(rmed)[rb + i] * AT(D.taurr, i, j) - RT(rmed)[rb + i - 1] * AT(D.taurr, i - 1, j)) * RT(invdiffrmed)[rb + i] +
            (AT(D.taurp, i, jp) - AT(D.taurp, i, j)) * invdphi - (real)0.5 * (AT(D.taupp, i, j) + AT(D.taupp, i - 1, j))) /
           ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
     }
@@ -701,12 +720,12 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
       // with self-gravity the Keplerian values include -r * <SG_Accr>_ring
       // (ApplySubKeplerianBoundary, SideEuler.cu:159-166)
       if (i == 0) {
-        const double v = s.vkep_in - (D.self_gravity ? D.rmed_d[rb] * D.axisg[rb] : 0.0);
-        AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb] * s.omega_frame - D.vref[rb]);
+        const double v = s.vkep_in - (D.self_gravity ? RT(rmed_d)[rb] * D.axisg[rb] : 0.0);
+        AT(D.vthetaint, i, j) = (real)(sqrt(v) - RT(rmed_d)[rb] * s.omega_frame - RT(vref)[rb]);
       }
       if (i == nr - 1) {
-        const double v = s.vkep_out - (D.self_gravity ? D.rmed_d[rb + nr - 1] * D.axisg[rb + nr - 1] : 0.0);
-        AT(D.vthetaint, i, j) = (real)(sqrt(v) - D.rmed_d[rb + nr - 1] * s.omega_frame - D.vref[rb + nr - 1]);
+        const double v = s.vkep_out - (D.self_gravity ? RT(rmed_d)[rb + nr - 1] * D.axisg[rb + nr - 1] : 0.0);
+        AT(D.vthetaint, i, j) = (real)(sqrt(v) - RT(rmed_d)[rb + nr - 1] * s.omega_frame - RT(vref)[rb + nr - 1]);
       }
     }
   }
@@ -732,18 +751,18 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep2(Dev<real> D) {
     const real sigl = AT(D.dens, i, jm);
     dv = vti - vtm;
     const real qtl = dv < 0 ? cv2 * sigl * dv * dv : (real)0;
-    const real invdxt = (real)1 / D.dxtheta[rb + i];
+    const real invdxt = (real)1 / RT(dxtheta)[rb + i];
     if (i > 0) {
       const real sigm = AT(D.dens, i - 1, j);
       dv = vri - AT(D.vradint, i - 1, j);
       const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
-      AT(D.vrad, i, j) = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * D.invdiffrmed[rb + i];
+      AT(D.vrad, i, j) = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * RT(invdiffrmed)[rb + i];
     } else {
       AT(D.vrad, i, j) = 0;
     }
     AT(D.vtheta, i, j) = vti - dt * (real)2 / (sig + sigl) * (qt - qtl) * invdxt;
     if (D.adiabatic)
-      AT(D.energyint, i, j) = AT(D.energy, i, j) - dt * qr * (vrp - vri) * D.invdiffrsup[rb + i] -
+      AT(D.energyint, i, j) = AT(D.energy, i, j) - dt * qr * (vrp - vri) * RT(invdiffrsup)[rb + i] -
                               dt * qt * (vtp - vti) * invdxt;
   }
 }
@@ -752,7 +771,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep2(Dev<real> D) {
 // Substep3Kernel + Substep3Kernel2 (Kernels.cu:91-139)
 template <class real>
 __device__ __forceinline__ real qplus_at(const Dev<real>& D, const DiskConst& c, int rb, long long fb, int ns, int i, int j) {
-  const real nu = D.visc[rb + i];
+  const real nu = RT(visc)[rb + i];
   if (nu == (real)0) return 0;
   const real sig = AT(D.dens, i, j);
   const real trr = AT(D.taurr, i, j), trp = AT(D.taurp, i, j), tpp = AT(D.taupp, i, j), dv = AT(D.divv, i, j);
@@ -766,9 +785,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
     const real dt = (real)s.dt;
     real qp;
     if (i > 0) qp = qplus_at(D, c, rb, fb, ns, i, j);
-    else if (D.visc[rb + nr - 1] != (real)0) {
+    else if (RT(visc)[rb + nr - 1] != (real)0) {
       const real q1 = qplus_at(D, c, rb, fb, ns, 1, j), q2 = qplus_at(D, c, rb, fb, ns, 2, j);
-      qp = q1 * (real)exp((double)log(q1 / q2) * log(D.rmed_d[rb] / D.rmed_d[rb + 1]) / log(D.rmed_d[rb + 1] / D.rmed_d[rb + 2]));
+      qp = q1 * (real)exp((double)log(q1 / q2) * log(RT(rmed_d)[rb] / RT(rmed_d)[rb + 1]) / log(RT(rmed_d)[rb + 1] / RT(rmed_d)[rb + 2]));
     } else qp = 0;
     AT(D.qplus, i, j) = qp;
     const real g1 = (real)c.adiabaticindex - (real)1;
@@ -778,8 +797,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
       en = (dt * qp + eint) / ((real)1 + g1 * dt * dvv);
     } else {
       const real sig = AT(D.dens, i, j);
-      const real tc = D.coolingtime[rb + i], sm = D.sigmamed[rb + i];
-      const real num = D.energymed[rb + i] * dt * sig / sm + tc * eint + dt * tc * (qp - D.qplusmed[rb + i] * sig / sm);
+      const real tc = RT(coolingtime)[rb + i], sm = RT(sigmamed)[rb + i];
+      const real num = RT(energymed)[rb + i] * dt * sig / sm + tc * eint + dt * tc * (qp - RT(qplusmed)[rb + i] * sig / sm);
       const real den = dt + tc + g1 * dt * tc * dvv;
       en = num / den;
     }
@@ -796,7 +815,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_open_boundary(Dev<real> D) {
   if (!D.st[d].active) return;
   const int ns = c.nsec, rb = c.ro;
   const long long fb = c.fo;
-  const real sm0 = D.sigmamed[rb];
+  const real sm0 = RT(sigmamed)[rb];
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     AT(D.dens, 0, j) = AT(D.dens, 1, j);
     if (D.adiabatic) AT(D.energy, 0, j) = AT(D.energy, 1, j);
@@ -826,17 +845,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
   a0 = block_sum(a0, redd) / ns; a1 = block_sum(a1, redd) / ns;
   a2 = block_sum(a2, redd) / ns; a3 = block_sum(a3, redd) / ns;
   if (threadIdx.x == 0) {
-    double da = (pow(D.rinf_d[rb + 1], -1.5) - 1.0) / (.5 * (a0 + a1));
-    da *= (D.rmed_d[rb + 1] - D.rmed_d[rb]);
+    double da = (pow(RT(rinf_d)[rb + 1], -1.5) - 1.0) / (.5 * (a0 + a1));
+    da *= (RT(rmed_d)[rb + 1] - RT(rmed_d)[rb]);
     ia_sh[0] = (int)(da / 2.0 / kPI * (double)ns + .5);
-    double da2 = (pow(D.rinf_d[rb + nr - 2], -1.5) - 1.0) / (.5 * (a2 + a3));
-    da2 *= (D.rmed_d[rb + nr - 1] - D.rmed_d[rb + nr - 2]);
+    double da2 = (pow(RT(rinf_d)[rb + nr - 2], -1.5) - 1.0) / (.5 * (a2 + a3));
+    da2 *= (RT(rmed_d)[rb + nr - 1] - RT(rmed_d)[rb + nr - 2]);
     ia_sh[1] = (int)(da2 / 2.0 / kPI * (double)ns + .5);
     s.nr_iangle = ia_sh[0]; s.nr_iangle2 = ia_sh[1];
   }
   __syncthreads();
   const int ia = ia_sh[0], ia2 = ia_sh[1];
-  const real sm1 = D.sigmamed[rb + 1], sm2 = D.sigmamed[rb + nr - 2];
+  const real sm1 = RT(sigmamed)[rb + 1], sm2 = RT(sigmamed)[rb + nr - 2];
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     int jp = j + ia;
     if (jp >= ns) jp -= ns;
@@ -863,9 +882,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
   const double md = block_sum(m0, redd) / ns, md2 = block_sum(m1, redd) / ns;
   const double me = block_sum(e0, redd) / ns, me2 = block_sum(e1, redd) / ns;
   // ring means restored to the initial profile, in FP64
-  const double cd = D.sigmamed_d[rb] - md, cd2 = D.sigmamed_d[rb + nr - 1] - md2;
-  const double ce = D.adiabatic ? D.energymed_d[rb] - me : 0.0;
-  const double ce2 = D.adiabatic ? D.energymed_d[rb + nr - 1] - me2 : 0.0;
+  const double cd = RT(sigmamed_d)[rb] - md, cd2 = RT(sigmamed_d)[rb + nr - 1] - md2;
+  const double ce = D.adiabatic ? RT(energymed_d)[rb] - me : 0.0;
+  const double ce2 = D.adiabatic ? RT(energymed_d)[rb + nr - 1] - me2 : 0.0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     AT(D.dens, 0, j) = (real)((double)AT(D.dens, 0, j) + cd);
     AT(D.dens, nr - 1, j) = (real)((double)AT(D.dens, nr - 1, j) + cd2);
@@ -880,28 +899,28 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
   FGX_STRIP_BEGIN {
-    const double r = D.rmed_d[rb + i];
+    const double r = RT(rmed_d)[rb + i];
     if (!(r < s.ev_drmin || r > s.ev_drmax)) continue;
     const double dt = s.dt;
     double lambda = 0, damping;
     if (r < s.ev_drmin) {
-      damping = (r - s.ev_drmin) / (D.rmed_d[rb] - s.ev_drmin);
+      damping = (r - s.ev_drmin) / (RT(rmed_d)[rb] - s.ev_drmin);
       lambda = damping * damping * 10.0 * dt / s.ev_tin;
     }
     if (r > s.ev_drmax) {
-      damping = (r - s.ev_drmax) / (D.rmed_d[rb + nr - 1] - s.ev_drmax);
+      damping = (r - s.ev_drmax) / (RT(rmed_d)[rb + nr - 1] - s.ev_drmax);
       lambda = damping * damping * 10.0 * dt / s.ev_tout;
     }
     const double h = aspect_ratio_dev(c, r, s.time - s.time0);
     double vt0 = sqrt(1.0 / r * (1.0 - (1.0 + c.sigmaslope - 2.0 * c.flaringindex) * h * h * pow(r, 2.0 * c.flaringindex)));
     vt0 -= r * s.omega_frame;
-    const double vr0 = -3.0 * (double)D.visc[rb + i] / r * (-c.sigmaslope + .5);
+    const double vr0 = -3.0 * (double)RT(visc)[rb + i] / r * (-c.sigmaslope + .5);
     // relaxation towards the FP64 initial profiles, evaluated in FP64
     const double inv = 1.0 / (1.0 + lambda);
     AT(D.vrad, i, j) = (real)(((double)AT(D.vrad, i, j) + lambda * vr0) * inv);
-    AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - D.vref[rb + i])) * inv);
-    AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * D.sigmamed_d[rb + i]) * inv);
-    if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * D.energymed_d[rb + i]) * inv);
+    AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - RT(vref)[rb + i])) * inv);
+    AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * RT(sigmamed_d)[rb + i]) * inv);
+    if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * RT(energymed_d)[rb + i]) * inv);
   }
 }
 
@@ -979,8 +998,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)D.vtheta[row + j];
   acc = block_sum(acc, redd);
   const double mres = acc / (double)ns;             // ring mean of the residual
-  const real vmed = (real)(D.vref[rb + i] + mres);  // VMed
-  const real rmed = D.rmed[rb + i], irm = D.invrmed[rb + i];
+  const real vmed = (real)(RT(vref)[rb + i] + mres);  // VMed
+  const real rmed = RT(rmed)[rb + i], irm = RT(invrmed)[rb + i];
   real ntilde, nround;
   const int nsh = fargo_shift_dev<real>(vmed, irm, dtd, ns, ntilde, nround);
   const real invdt = (real)1 / dt;
@@ -988,10 +1007,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   const real vconst = (ntilde - nround) * rmed * invdt * dpinvns;
   const real v2 = D.fast_transport ? vconst : (real)0;
   if (threadIdx.x == 0) { D.vmed[rb + i] = vmed; D.nshift[rb + i] = nsh; }
-  const real dxt = D.dxtheta[rb + i];
+  const real dxt = RT(dxtheta)[rb + i];
   const real invdxt = (real)1 / dxt;
-  const real dxrad = (D.rsup[rb + i] - D.rinf[rb + i]) * dt;
-  const real invsurf = (real)1 / D.surf[rb + i];
+  const real dxrad = (RT(rsup)[rb + i] - RT(rinf)[rb + i]) * dt;
+  const real invsurf = (real)1 / RT(surf)[rb + i];
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     sD1[j] = D.t_dens[row + j];
     real v = (real)((double)D.vtheta[row + j] - mres);   // VthetaRes
@@ -1067,8 +1086,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   // (Kernels.cu:1303-1304), evaluated in the unshifted frame and rotated
   // the angular momenta are residuals of density * l0 (k_trans_rad), so
   // (TMP + TMM)/(S + 1e-15)/R - R*Omega = vref + residual below, exactly
-  const double rd = D.rmed_d[rb + i];
-  const double l0 = (D.vref[rb + i] + rd * s.omega_frame) * rd;
+  const double rd = RT(rmed_d)[rb + i];
+  const double l0 = (RT(vref)[rb + i] + rd * s.omega_frame) * rd;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     int js = j - nsm;
     js += (js < 0) ? ns : 0;
@@ -1146,14 +1165,14 @@ __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_potential(Dev<real> D) {
   FGX_STRIP_BEGIN {
-    D.phi[fb + (long long)i * ns + j] = potential_planets_at(D, c, s, D.rmed_d[rb + i], j);
+    D.phi[fb + (long long)i * ns + j] = potential_planets_at(D, c, s, RT(rmed_d)[rb + i], j);
   }
 }
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_potential_field(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
-  const double r = D.rmed_d[rb + i];
+  const double r = RT(rmed_d)[rb + i];
   AT(D.work, i, j) = (real)(potential_planets_at(D, c, s, r, j) - 1.0 / r);
 }
 // ComputeTemperatureFieldKernel (Kernels.cu:211-223) into D.work
@@ -1189,13 +1208,13 @@ __global__ void __launch_bounds__(FGX_TPB) k_force_full(Dev<real> D, int disk, d
   __shared__ double redd[FGX_TPB / 64];
   const double a = sqrt(x * x + y * y);
   const double rh = pow(mass / 3., 1. / 3.) * a + 1e-15;
-  const real rmed = D.rmed[rb + i], surf = D.surf[rb + i];
+  const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
   const bool inner = (double)rmed < (double)(float)a;
   double f[22];
   for (int k = 0; k < 22; ++k) f[k] = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     const real cm = surf * AT(D.dens, i, j);
-    const real dx = rmed * D.cosj[c.so + j] - (real)x, dy = rmed * D.sinj[c.so + j] - (real)y;
+    const real dx = rmed * RT(cosj)[c.so + j] - (real)x, dy = rmed * RT(sinj)[c.so + j] - (real)y;
     const real d2 = dx * dx + dy * dy;
     const real dist2 = d2 + (real)(sm * sm);
     const real inv3 = (real)1 / dist2 / sqrt(dist2);
@@ -1227,10 +1246,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_mdcp(Dev<real> D, int disk, double 
   const int ns = c.nsec, rb = c.ro;
   const long long fb = c.fo;
   __shared__ double redd[FGX_TPB / 64];
-  const real rmed = D.rmed[rb + i], surf = D.surf[rb + i];
+  const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
   double acc = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    const real dx = rmed * D.cosj[c.so + j] - (real)xpl, dy = rmed * D.sinj[c.so + j] - (real)ypl;
+    const real dx = rmed * RT(cosj)[c.so + j] - (real)xpl, dy = rmed * RT(sinj)[c.so + j] - (real)ypl;
     if ((double)sqrt(dx * dx + dy * dy) < rhill) acc += (double)(surf * AT(D.dens, i, j));
   }
   acc = block_sum(acc, redd);

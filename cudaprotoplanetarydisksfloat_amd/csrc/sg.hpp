@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_rows_fwd(Dev<real> D) {
   if (i >= nr) return;
   cxd* a = reinterpret_cast<cxd*>(fgx_smem);
   cxd* b = a + ns;
-  const double q = D.rmed_d[rb + i] / D.rmed_d[rb];
+  const double q = RT(rmed_d)[rb + i] / RT(rmed_d)[rb];
   const double sq = sqrt(q);
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     const double var = (double)AT(D.dens, i, j) * sq;
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_rows_inv(Dev<real> D) {
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) a[j] = w[j];
   __syncthreads();
   const cxd* out = lds_fft(a, b, ns, 1, c.fac_ns, c.nf_ns, 1.0, D.sgtw + c.tw_ns);
-  const double q = D.rmed_d[rb + i] / D.rmed_d[rb];
+  const double q = RT(rmed_d)[rb + i] / RT(rmed_d)[rb];
   const double norm = c.sg_rstep * c.sg_tstep / ((double)(2 * nr) * (double)ns);
   const double nrr = norm / sqrt(q), ntt = norm / (q * sqrt(q));
   const double self = kG * c.sg_rstep * c.sg_tstep / c.sg_eps;
@@ -207,10 +207,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_update(Dev<real> D) {
   FGX_STRIP_BEGIN {
     const double dt = s.dt;
     if (i > 0) {
-      const double ri = D.rinf_d[rb + i];
-      const double inc = dt * ((ri - D.rmed_d[rb + i - 1]) * (double)AT(D.sg_accr, i, j) +
-                               (D.rmed_d[rb + i] - ri) * (double)AT(D.sg_accr, i - 1, j)) *
-                         D.invdiffrmed_d[rb + i];
+      const double ri = RT(rinf_d)[rb + i];
+      const double inc = dt * ((ri - RT(rmed_d)[rb + i - 1]) * (double)AT(D.sg_accr, i, j) +
+                               (RT(rmed_d)[rb + i] - ri) * (double)AT(D.sg_accr, i - 1, j)) *
+                         RT(invdiffrmed_d)[rb + i];
       AT(D.vradint, i, j) = (real)((double)AT(D.vradint, i, j) + inc);
     }
     // (j-1)%nsec at j = 0 is -1 in the reference: linear index i*nsec-1, the

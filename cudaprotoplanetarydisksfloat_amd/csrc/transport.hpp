@@ -64,29 +64,47 @@ struct RadRing {
 template <class real>
 struct RadFields {
   const real *dens, *vrad, *vtheta, *energy;
-  const double *rmed_d, *vref;
+  const FGX_AS4 double *rmed_d, *vref;
 };
 
+// the raw values of ring k a thread needs (clamped rings only feed zero
+// slopes / zero fluxes)
 template <class real>
-__device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, double om,
-                                         bool ad, RadRing<real>& r) {
-  k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);   // clamped rings only feed zero slopes / zero fluxes
+struct RadRaw {
+  real sg, vr0, vr1, vtp, vtj, e;
+};
+template <class real>
+__device__ __forceinline__ void rad_fetch(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, bool ad,
+                                          RadRaw<real>& w) {
+  k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
   const int o = k * ns;
-  const real sg = f.dens[o + j];
-  const real vr0 = f.vrad[o + j], vr1 = f.vrad[o + ns + j];
+  w.sg = f.dens[o + j];
+  w.vr0 = f.vrad[o + j];
+  w.vr1 = f.vrad[o + ns + j];
+  w.vtp = f.vtheta[o + jp];
+  w.vtj = f.vtheta[o + j];
+  w.e = ad ? f.energy[o + j] : (real)0;
+}
+
+template <class real>
+__device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int k, double om, const RadRaw<real>& w,
+                                           RadRing<real>& r) {
+  k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
+  const real sg = w.sg;
   const double rk = f.rmed_d[k];
   const double l0 = (f.vref[k] + rk * om) * rk;
-  const double lp = l0 + (double)f.vtheta[o + jp] * rk, lm = l0 + (double)f.vtheta[o + j] * rk;
+  const double lp = l0 + (double)w.vtp * rk, lm = l0 + (double)w.vtj * rk;
   // DivisePolarGrid: quantity / (density + 1e-20), as a product with one
   // reciprocal per cell; the density's own work is exactly 1 wherever the
   // 1e-20 is below its rounding (density >= 1e-12), as in the quotient
   const real inv = vl_rcp(sg + (real)1e-20);
-  const real w1 = (sg >= (real)1e-12) ? (real)1 : sg / (sg + (real)1e-20);
-  r.sg = sg; r.vr0 = vr0; r.vr1 = vr1;
+  real w1 = (real)1;
+  if (sg < (real)1e-12) w1 = sg / (sg + (real)1e-20);
+  r.sg = sg; r.vr0 = w.vr0; r.vr1 = w.vr1;
   r.q[0] = sg;
   r.q[1] = w1;
-  r.q[2] = (sg * vr1) * inv;
-  r.q[3] = (sg * vr0) * inv;
+  r.q[2] = (sg * w.vr1) * inv;
+  r.q[3] = (sg * w.vr0) * inv;
   const double dd = (double)sg;
   r.tp = dd * lp; r.tm = dd * lm;
   if constexpr (sizeof(real) == 4) {
@@ -95,8 +113,16 @@ __device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int n
   } else {
     r.q[4] = (real)(r.tp / (dd + 1e-20)); r.q[5] = (real)(r.tm / (dd + 1e-20));
   }
-  r.e = ad ? f.energy[o + j] : (real)0;
-  r.q[6] = r.e * inv;
+  r.e = w.e;
+  r.q[6] = w.e * inv;
+}
+
+template <class real>
+__device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, double om,
+                                         bool ad, RadRing<real>& r) {
+  RadRaw<real> w;
+  rad_fetch(f, ns, nr, k, j, jp, ad, w);
+  rad_derive(f, nr, k, om, w, r);
 }
 
 // van Leer slope of ring k from rings k-1, k, k+1 (StarRadKernel); zero on
@@ -126,11 +152,11 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   const double om = s.omega_frame;
   const real dt = (real)s.dt;
   const real dth = (real)(2.0 * kPI / (double)ns);
-  const real* rmed = D.rmed + rb;
-  const real* idr = D.invdiffrmed + rb;
+  const auto rmed = RT(rmed) + rb;
+  const auto idr = RT(invdiffrmed) + rb;
   const int nq = ad ? 7 : 6;
-  const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0), D.rmed_d + rb,
-                          D.vref + rb};
+  const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0), RT(rmed_d) + rb,
+                          RT(vref) + rb};
   real* const o_dens = D.t_dens + fb;
   real* const o_rmp = D.t_rmp + fb;
   real* const o_rmm = D.t_rmm + fb;
@@ -165,19 +191,25 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
       st[q] = (v > 0) ? rm1.q[q] + (rmed[i0] - rmed[i0 - 1 < 0 ? 0 : i0 - 1] - v * dt) * (real)0.5 * slM[q]
                       : ra.q[q] - (rmed[i0 + 1] - rmed[i0] + v * dt) * (real)0.5 * slA[q];
     }
-    const real a = dt * dth * D.rinf[rb + i0];
+    const real a = dt * dth * RT(rinf)[rb + i0];
     flux[0] = 0;
 #pragma unroll
     for (int q = 1; q < 7; ++q) flux[q] = (i0 == 0 || q >= nq) ? (real)0 : a * st[q] * st[0] * v;
   }
-  const real* isurf = D.invsurf + rb;
+  const auto isurf = RT(invsurf) + rb;
+  // ring i+2 is fetched one iteration ahead, before ring i-1's stores: the
+  // vector memory counter retires loads and stores in issue order
+  RadRaw<real> nxt;
+  rad_fetch(F, ns, nr, i0 + 2, j, jp, ad, nxt);
   for (int i = i0; i < i1; ++i) {
+    const RadRaw<real> cur = nxt;
+    if (i + 1 < i1) rad_fetch(F, ns, nr, i + 3, j, jp, ad, nxt);
     RadRing<real> rc;
-    rad_load(F, ns, nr, i + 2, j, jp, om, ad, rc);
+    rad_derive(F, nr, i + 2, om, cur, rc);
     // interface i+1 (zero flux at the outer edge)
     const real v = rb1.vr0;
     const bool edge = (i + 1 == nr);
-    const real a = edge ? (real)0 : dt * dth * D.rinf[rb + i + 1];
+    const real a = edge ? (real)0 : dt * dth * RT(rinf)[rb + i + 1];
     real fnew[7], slB[7];
     real st0 = 0;
     fnew[0] = 0;
@@ -336,11 +368,19 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
     }
   };
 
-  // everything this ring needs first is requested at once
-  real V[K], Dn[K], Qn[K];
+  // every global load of the ring is issued here, before any store: the
+  // vector memory counter retires loads and stores in issue order, so a load
+  // issued after a store could not be waited for without that store
+  const int nq = ad ? 5 : 4, oh = nq - 2;   // oh: ThetaMomP
+  real V[K], Dn[K], Qa[5][K];
   ld_chunk<real, K>(D.vtheta + my, V);
   ld_chunk<real, K>(D.t_dens + my, Dn);
-  ld_chunk<real, K>(D.t_rmp + my, Qn);
+  {
+    const real* srcs[5] = {D.t_rmp, D.t_rmm, ad ? D.t_e : D.t_tmp, ad ? D.t_tmp : D.t_tmm, D.t_tmm};
+#pragma unroll
+    for (int o = 0; o < 5; ++o)
+      if (o < nq) ld_chunk<real, K>(srcs[o] + my, Qa[o]);
+  }
 
   // ring mean of vtheta (FP64 sum of the residuals)
   double acc = 0;
@@ -356,8 +396,8 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
     for (int q = 0; q < W; ++q) acc += redw[q];
   }
   const double mres = acc / (double)ns;
-  const real vmed = (real)(D.vref[rb + i] + mres);
-  const real rmed = D.rmed[rb + i], irm = D.invrmed[rb + i];
+  const real vmed = (real)(RT(vref)[rb + i] + mres);
+  const real rmed = RT(rmed)[rb + i], irm = RT(invrmed)[rb + i];
   real ntilde, nround;
   const int nsh = fargo_shift_dev<real>(vmed, irm, s.dt, ns, ntilde, nround);
   const real dt = (real)s.dt;
@@ -368,10 +408,10 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
   if (threadIdx.x == 0) { D.vmed[rb + i] = vmed; D.nshift[rb + i] = nsh; }
   int nsm = nsh % ns;
   if (nsm < 0) nsm += ns;
-  const real dxt = D.dxtheta[rb + i];
+  const real dxt = RT(dxtheta)[rb + i];
   const real invdxt = (real)1 / dxt;
-  const real dxrad = (D.rsup[rb + i] - D.rinf[rb + i]) * dt;
-  const real invsurf = (real)1 / D.surf[rb + i];
+  const real dxrad = (RT(rsup)[rb + i] - RT(rinf)[rb + i]) * dt;
+  const real invsurf = (real)1 / RT(surf)[rb + i];
   // AdvectSHIFTKernel: out[j] = in[j - Nshift], i.e. sector j lands at
   // j + Nshift
   const int o0 = [&] { int o = j0 + nsm; return o >= ns ? o - ns : o; }();
@@ -491,27 +531,17 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
     invD2[k] = (real)1 / (Dn[k] + (real)1e-20);
   }
   // quantities in the order RadMomP, RadMomM, [energy], ThetaMomP (held),
-  // ThetaMomM (left in Qn), each loaded while the previous one is advected
-  const real* srcs[5] = {D.t_rmp, D.t_rmm, ad ? D.t_e : D.t_tmp, ad ? D.t_tmp : D.t_tmm, D.t_tmm};
+  // ThetaMomM (held)
   real* dsts[3] = {D.t_rmp, D.t_rmm, D.energy};
-  const int nq = ad ? 5 : 4, oh = nq - 2;   // oh: ThetaMomP
-  real Q[K], H[K];
-#pragma nounroll
-  for (int o = 0; o < nq; ++o) {
+  real H[K], Qn[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) Q[k] = Qn[k];
-    if (o + 1 < nq) ld_chunk<real, K>(srcs[o + 1] + my, Qn);
-    both(Q);
-    if (o < oh) {
-      store_shift(dsts[o], Q);
-    } else if (o == oh) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) H[k] = Q[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < K; ++k) Qn[k] = Q[k];
-    }
+  for (int o = 0; o < 5; ++o) {
+    if (o >= nq) break;
+    both(Qa[o]);
+    if (o < oh) store_shift(dsts[o], Qa[o]);
   }
+#pragma unroll
+  for (int k = 0; k < K; ++k) { H[k] = ad ? Qa[3][k] : Qa[2][k]; Qn[k] = ad ? Qa[4][k] : Qa[3][k]; }
   // density, pass 2
   {
     real A[K], F[K];
@@ -525,8 +555,8 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
   // (TMP[j-1] + TMM[j]) / (Sigma[j] + Sigma[j-1] + 1e-15) / R - R*Omega; with
   // the momenta stored as residuals of density * l0 this is vref + the
   // residual below, exactly
-  const double rd = D.rmed_d[rb + i];
-  const double l0 = (D.vref[rb + i] + rd * s.omega_frame) * rd;
+  const double rd = RT(rmed_d)[rb + i];
+  const double l0 = (RT(vref)[rb + i] + rd * s.omega_frame) * rd;
   real hl, dl;
   left2(H[K - 1], Dn[K - 1], hl, dl);
 #pragma unroll
