@@ -511,12 +511,19 @@ struct Batch : BatchBase {
 
   // ------------------------------------------------------------------ step
   // azimuthal transport: one launch per lane width K
+  // g.w: 0 = one full wave per ring, 1 = one partial wave, >= 2 = arcs
   template <int K>
   int theta_k(const ThetaGroup& g) {
     if constexpr (sizeof(real) == 8 && K > 8) {
       return set_error(FGX_EDEVICE, "no FP64 azimuthal kernel for K=%d", g.k);
     } else {
-      return launch("trans_theta", 0, k_trans_theta_arc<real, K>, g.n, 64 * g.w, 0, D, g.tiles, g.w);
+      if (g.w == 0) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 1>, g.n, 64, 0, D, g.tiles, 1);
+      if (g.w == 1) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 0>, g.n, 64, 0, D, g.tiles, 1);
+      if constexpr (K >= 2 && K <= 4) {
+        return launch("trans_theta", 0, k_trans_theta_arc<real, K, 2>, g.n, 64 * g.w, 0, D, g.tiles, g.w);
+      } else {
+        return set_error(FGX_EDEVICE, "no multi-wave azimuthal kernel for K=%d", g.k);
+      }
     }
   }
   int trans_theta(double bytes_per_cell) {

@@ -282,20 +282,23 @@ __device__ __forceinline__ void ld_chunk(const real* __restrict__ p, real (&v)[K
 // rotations, across arcs through two LDS words per wave edge (double
 // buffered, one barrier per exchange).  Every disk's field offset is a
 // multiple of 64 elements, so each lane's chunk is aligned for vector loads.
-template <class real, int K>
-__global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real> D, const RingTile* __restrict__ tiles,
-                                                                       int W) {
+template <class real, int K, int WM>
+__global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<real> D, const RingTile* __restrict__ tiles,
+                                                                        int W) {
+  // WM: 2 = W arcs of one full wave each, 1 = one full wave (nsec = 64*K),
+  // 0 = one wave of T < 64 lanes
+  constexpr bool MULTI = WM == 2;
   __shared__ real ex[2][4][16];
   __shared__ double redw[16];
   const RingTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
-  const DiskConst& c = D.dc[t.d];
-  const DiskState& s = D.st[t.d];
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
+  const FGX_AS4 DiskState& s = cref(D.st + t.d);
   if (!D.ignore_active && !s.active) return;
   const int ns = c.nsec, i = t.i, rb = c.ro;
-  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  const int T = (W > 1) ? 64 : ns / K;
-  const bool on = lane < T;
+  const int w = MULTI ? (int)threadIdx.x >> 6 : 0, lane = (int)threadIdx.x & 63;
+  const int T = (WM == 0) ? ns / K : 64;
+  const bool on = (WM == 0) ? lane < T : true;
   const int wl = (w == 0) ? W - 1 : w - 1, wr = (w + 1 == W) ? 0 : w + 1;
   // lanes past the ring read its first chunk (never stored)
   const long long row = c.fo + (long long)i * ns;
@@ -307,8 +310,8 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
   // neighbour lanes along the ring; in(): values of this lane's left and
   // right neighbours for up to three exchanged quantities at once
   auto fix_single = [&](real& l1, real& l2, real& r1, real x1, real x2, real xr) {
-    // W == 1 and T < 64: the wrap comes from lane T-1, resp. lane 0
-    if (T < 64) {
+    // T < 64: the wrap comes from lane T-1, resp. lane 0
+    if constexpr (WM == 0) {
       const real e1 = lane_read(x1, T - 1), e2 = lane_read(x2, T - 1), er = lane_read(xr, 0);
       if (lane == 0) { l1 = e1; l2 = e2; }
       if (lane == T - 1) r1 = er;
@@ -320,7 +323,7 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
     l1 = lane_from_left(x1);
     l2 = lane_from_left(x2);
     r1 = lane_from_right(xr);
-    if (W > 1) {
+    if constexpr (MULTI) {
       if (lane == 63) { ex[xb][0][w] = x1; ex[xb][1][w] = x2; }
       if (lane == 0) ex[xb][2][w] = xr;
       __syncthreads();
@@ -335,7 +338,7 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
   // W == 1)
   auto left_of = [&](real x) -> real {
     real r = lane_from_left(x);
-    if (T < 64) {
+    if constexpr (WM == 0) {
       const real e = lane_read(x, T - 1);
       if (lane == 0) r = e;
     }
@@ -343,12 +346,12 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
   };
   auto right1 = [&](real xr) -> real {
     real r = lane_from_right(xr);
-    if (W > 1) {
+    if constexpr (MULTI) {
       if (lane == 0) ex[xb][3][w] = xr;
       __syncthreads();
       if (lane == 63) r = ex[xb][3][wr];
       xb ^= 1;
-    } else if (T < 64) {
+    } else if constexpr (WM == 0) {
       const real e = lane_read(xr, 0);
       if (lane == T - 1) r = e;
     }
@@ -357,12 +360,12 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
   auto left2 = [&](real x1, real x2, real& l1, real& l2) {
     l1 = lane_from_left(x1);
     l2 = lane_from_left(x2);
-    if (W > 1) {
+    if constexpr (MULTI) {
       if (lane == 63) { ex[xb][0][w] = x1; ex[xb][1][w] = x2; }
       __syncthreads();
       if (lane == 0) { l1 = ex[xb][0][wl]; l2 = ex[xb][1][wl]; }
       xb ^= 1;
-    } else if (T < 64) {
+    } else if constexpr (WM == 0) {
       const real e1 = lane_read(x1, T - 1), e2 = lane_read(x2, T - 1);
       if (lane == 0) { l1 = e1; l2 = e2; }
     }
@@ -389,7 +392,7 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
     for (int k = 0; k < K; ++k) acc += (double)V[k];
   }
   acc = wave_sum(acc);
-  if (W > 1) {
+  if constexpr (MULTI) {
     if (lane == 0) redw[w] = acc;
     __syncthreads();
     acc = 0;
@@ -514,7 +517,7 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     SV1[k] = dxrad * SV1[k] * V[k] * invsurf;
-    invD1[k] = (real)1 / (Dn[k] + (real)1e-20);
+    invD1[k] = vl_rcp(Dn[k] + (real)1e-20);
   }
   {
     real A[K], F[K];
@@ -528,7 +531,7 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     SV2[k] = dxrad * SV2[k] * v2 * invsurf;
-    invD2[k] = (real)1 / (Dn[k] + (real)1e-20);
+    invD2[k] = vl_rcp(Dn[k] + (real)1e-20);
   }
   // quantities in the order RadMomP, RadMomM, [energy], ThetaMomP (held),
   // ThetaMomM (held)
@@ -559,18 +562,31 @@ __global__ void __launch_bounds__(K <= 4 ? 1024 : 64) k_trans_theta_arc(Dev<real
   const double l0 = (RT(vref)[rb + i] + rd * s.omega_frame) * rd;
   real hl, dl;
   left2(H[K - 1], Dn[K - 1], hl, dl);
+  if constexpr (sizeof(real) == 4) {
+    // the quotient is a small residual of vtheta: FP32 with a reciprocal
+    // keeps it far inside the FP32 field's own rounding
+    const real l0e = (real)(l0 * 1e-15);
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const double hm = (double)((k == 0) ? hl : H[k - 1]);
-    const double dm = (double)((k == 0) ? dl : Dn[k - 1]);
-    V[k] = (real)((hm + (double)Qn[k] - l0 * 1e-15) / ((double)Dn[k] + dm + 1e-15) / rd);
+    for (int k = 0; k < K; ++k) {
+      const real hm = (k == 0) ? hl : H[k - 1];
+      const real dm = (k == 0) ? dl : Dn[k - 1];
+      V[k] = (hm + Qn[k] - l0e) * vl_rcp(Dn[k] + dm + (real)1e-15) * irm;
+    }
+  } else {
+    const double ird = 1.0 / rd;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double hm = (double)((k == 0) ? hl : H[k - 1]);
+      const double dm = (double)((k == 0) ? dl : Dn[k - 1]);
+      V[k] = (real)((hm + (double)Qn[k] - l0 * 1e-15) / ((double)Dn[k] + dm + 1e-15) * ird);
+    }
   }
   store_shift(D.vtheta, V);
 }
 
 // split of a ring of nsec sectors: W arcs of 64*K sectors (K in {3, 2, 4},
-// 2 <= W <= 16), else one wave of T <= 64 lanes with nsec = T*K; K = 0 when
-// nothing fits (shared-memory kernel)
+// 2 <= W <= 16), else one wave with nsec = T*K, W = 0 when T = 64 and W = 1
+// when T < 64; K = 0 when nothing fits (shared-memory kernel)
 template <class real>
 inline void theta_split(int nsec, int& K, int& W) {
   static const int ka[] = {3, 2, 4};
@@ -582,7 +598,7 @@ inline void theta_split(int nsec, int& K, int& W) {
   const int n = sizeof(real) == 4 ? 10 : 7;
   W = 1;
   for (int a = 0; a < n; ++a)
-    if (nsec % ks[a] == 0 && nsec / ks[a] <= 64) { K = ks[a]; return; }
+    if (nsec % ks[a] == 0 && nsec / ks[a] <= 64) { K = ks[a]; W = (nsec / ks[a] == 64) ? 0 : 1; return; }
   K = 0;
 }
 
