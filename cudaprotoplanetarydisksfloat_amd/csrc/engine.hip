@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -24,6 +25,7 @@
 #include "kernels.hpp"
 #include "transport.hpp"
 #include "sg.hpp"
+#include "source.hpp"
 
 namespace fgx {
 
@@ -67,6 +69,10 @@ struct Batch : BatchBase {
   // one per lane width K of the register-resident kernel, plus the rings
   // whose sector count fits none (shared-memory kernel)
   int n_stiles = 0, strip_rings = 16, n_sgctiles = 0;
+  // fused source pass (k_source): 57-sector chunks of src_rings rings
+  int n_srctiles = 0, src_rings = 12;
+  bool fused_source = true;
+  const CellTile* srctiles = nullptr;
   size_t sg_shm_row = 0, sg_shm_col = 0;
   const CellTile* stiles = nullptr;
   struct ThetaGroup { int k, w, n; const RingTile* tiles; };
@@ -276,6 +282,17 @@ struct Batch : BatchBase {
     }
     while (st.size() % 8) st.push_back(CellTile{-1, 0, 0, 0});
     n_stiles = (int)st.size();
+    // fused source pass: on by default, FGX_FUSED_SOURCE=0 keeps the
+    // separate kernels; FGX_SOURCE_RINGS sets the strip height
+    if (const char* e = std::getenv("FGX_FUSED_SOURCE")) fused_source = std::atoi(e) != 0;
+    if (const char* e = std::getenv("FGX_SOURCE_RINGS")) src_rings = std::atoi(e);
+    src_rings = std::min(std::max(src_rings, 1), 56);   // k_source stages rs + 8 rings in 64 lanes
+    std::vector<CellTile> srt;
+    for (int d = 0; d < nd; ++d)
+      for (int i0 = 0; i0 < hd[d].nrad; i0 += src_rings)
+        for (int jb = 0; jb * FGX_SRC_OUT < hd[d].nsec; ++jb) srt.push_back(CellTile{d, i0, jb, 0});
+    while (srt.size() % 8) srt.push_back(CellTile{-1, 0, 0, 0});
+    n_srctiles = (int)srt.size();
     while (ct.size() % 8) ct.push_back(CellTile{-1, 0, 0, 0});
     while (rt.size() % 8) rt.push_back(RingTile{-1, 0});
     n_ctiles = (int)ct.size();
@@ -349,6 +366,10 @@ struct Batch : BatchBase {
       stiles = std_t;
       D.stiles = std_t;
       D.strip_rings = strip_rings;
+      CellTile* srt_d;
+      LCHK(dalloc(&srt_d, srt.size()));
+      HIPCHK(hipMemcpyAsync((void*)srt_d, srt.data(), srt.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
+      srctiles = srt_d;
       for (int kw = 0; kw < 17 * 17; ++kw) {
         std::vector<RingTile>& v = tg[kw];
         if (v.empty()) continue;
@@ -576,19 +597,31 @@ struct Batch : BatchBase {
     if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_stiles, FGX_TPB, 0, D));
     if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
     LCHK(boundary());
-    LCHK(launch("potential", 2, k_potential<real>, n_stiles, FGX_TPB, 0, D));
-    LCHK(launch("substep1", ad ? 8 : 7, k_substep1<real>, n_stiles, FGX_TPB, 0, D));
-    if (D.self_gravity) {
-      LCHK(self_gravity());
-      LCHK(launch("sg_update", 6, k_sg_update<real>, n_stiles, FGX_TPB, 0, D));
+    if (fused_source && !D.self_gravity) {
+      // the viscosity profile only depends on the sound speed, which the
+      // source stages do not change, so it can be formed first
+      if (D.viscous && visc_dynamic)
+        LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
+      LCHK(launch("source", ad ? 7 : 5, k_source<real>, n_srctiles, 64, 0, D, srctiles, src_rings));
+      // the new velocities were written to the VradInt / VthetaInt arrays
+      std::swap(D.vrad, D.vradint);
+      std::swap(D.vtheta, D.vthetaint);
+      D.vpar ^= 1;
+    } else {
+      LCHK(launch("potential", 2, k_potential<real>, n_stiles, FGX_TPB, 0, D));
+      LCHK(launch("substep1", ad ? 8 : 7, k_substep1<real>, n_stiles, FGX_TPB, 0, D));
+      if (D.self_gravity) {
+        LCHK(self_gravity());
+        LCHK(launch("sg_update", 6, k_sg_update<real>, n_stiles, FGX_TPB, 0, D));
+      }
+      if (D.viscous) {
+        if (visc_dynamic) LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
+        LCHK(launch("visc_tau", 6, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vradint,
+                    (const real*)D.vthetaint, 0));
+      }
+      LCHK(launch("visc_update", 7, k_visc_update<real>, n_stiles, FGX_TPB, 0, D));
+      LCHK(launch("substep2", ad ? 7 : 5, k_substep2<real>, n_stiles, FGX_TPB, 0, D));
     }
-    if (D.viscous) {
-      if (visc_dynamic) LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
-      LCHK(launch("visc_tau", 6, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vradint,
-                  (const real*)D.vthetaint, 0));
-    }
-    LCHK(launch("visc_update", 7, k_visc_update<real>, n_stiles, FGX_TPB, 0, D));
-    LCHK(launch("substep2", ad ? 7 : 5, k_substep2<real>, n_stiles, FGX_TPB, 0, D));
     LCHK(boundary());
     if (ad) {
       if (visc_dynamic) LCHK(launch("visc_profile", 1, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
@@ -618,6 +651,8 @@ struct Batch : BatchBase {
         if (h_summary[0] == 0) break;
         plan = std::max(1, h_summary[1]);
       }
+      // disks that finished the DT before the batch did
+      if (fused_source && !D.self_gravity) LCHK(launch("vel_fixup", 0, k_vel_fixup<real>, n_stiles, FGX_TPB, 0, D));
     }
     HIPCHK(hipStreamSynchronize(stream));
     if (prof_on) prof_collect();

@@ -124,6 +124,7 @@ struct DiskState {
   int ncfl, active, sub_in_dt, health, rem_est;
   int nr_iangle, nr_iangle2;            // non-reflecting shifts of this call
   int frozen;                           // skipped by every later DT (served request done)
+  int vpar;                             // velocity buffer pair holding this disk's Vrad/Vtheta
 };
 
 // a field tile: ring i of disk d, sectors [jb*256, jb*256+256)

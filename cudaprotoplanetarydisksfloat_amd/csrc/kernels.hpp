@@ -60,6 +60,10 @@ struct Dev {
   int adiabatic, fast_transport, open_inner, nonreflecting, evanescent, self_gravity;
   int cooling, indirect, exclude_hill, fix_taurp, fix_pp, fix_sgwrap, sloppy, corotating;
   int guiding, forced_circular, roche, write_pressure, ignore_active, viscous;
+  // which of the buffer pairs (vrad, vtheta) / (vradint, vthetaint) the
+  // batch's pointers have been swapped to since set-up, modulo 2 (the fused
+  // source pass writes the new velocities into the second pair)
+  int vpar;
 };
 
 // ---- small helpers ------------------------------------------------------
@@ -191,7 +195,45 @@ __device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, 
   (void)rb; (void)s; (void)c;                                               \
   for (int i = t.i; i < i_end; ++i)
 
-#define AT(a, ii, jj) a[fb + (long long)(ii) * ns + (jj)]
+// Field element (ring ii, sector jj) of the tile's disk as a buffer access:
+// the disk's field base goes into a buffer descriptor (scalar registers), the
+// ring offset into the scalar offset and the sector offset into a 32-bit
+// vector offset, so walking rings costs one scalar add and no vector address
+// arithmetic.  Offsets are bytes; a disk's field is far below 2 GiB.
+template <class T>
+struct FieldRef {
+  __amdgpu_buffer_rsrc_t r;
+  int soff;
+  unsigned voff;
+  __device__ __forceinline__ T get() const {
+    if constexpr (sizeof(T) == 4) {
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    } else {
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    }
+  }
+  __device__ __forceinline__ void put(T v) const {
+    if constexpr (sizeof(T) == 4) {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    } else {
+      using u2 = unsigned __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+    }
+  }
+  __device__ __forceinline__ operator T() const { return get(); }
+  __device__ __forceinline__ const FieldRef& operator=(T v) const { put(v); return *this; }
+  __device__ __forceinline__ const FieldRef& operator=(const FieldRef& o) const { put(o.get()); return *this; }
+  __device__ __forceinline__ const FieldRef& operator+=(T v) const { put(get() + v); return *this; }
+  __device__ __forceinline__ const FieldRef& operator-=(T v) const { put(get() - v); return *this; }
+};
+// raw buffer descriptor word 3 for gfx9-family parts: 32-bit data format, no
+// swizzle, no index stride
+#define FGX_BUF_WORD3 0x00020000
+template <class T>
+__device__ __forceinline__ FieldRef<T> field_at(T* base, int soff, unsigned voff) {
+  return FieldRef<T>{__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, 0x7fffffff, FGX_BUF_WORD3), soff, voff};
+}
+#define AT(a, ii, jj) field_at((a) + fb, (int)((ii) * ns * (int)sizeof(*(a))), (unsigned)(jj) * (unsigned)sizeof(*(a)))
 
 // potential of the planets and the indirect term at cell (i,j) in FP64:
 // FillForcesArraysKernel (Kernels.cu:1322-1349) for all planets in one
@@ -232,10 +274,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
   if (c.substeps_forced > 0 || (D.sloppy && s.sub_in_dt > 0)) return;
   __shared__ double redd[FGX_TPB / 64];
   __shared__ real redr[FGX_TPB / 64];
-  const long long row = fb + (long long)i * ns;
   const double vr0 = RT(vref)[rb + i];
   double acc = 0;
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)D.vtheta[row + j];
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)(real)AT(D.vtheta, i, j);
   acc = block_sum(acc, redd);
   const double mres = acc / (double)ns;   // ring mean of the residual
   if (threadIdx.x == 0) D.vmoy[rb + i] = vr0 + mres;
@@ -243,22 +284,27 @@ __global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
   const real dxrad = RT(rsup)[rb + i] - RT(rinf)[rb + i];
   const real dxt = RT(dxtheta)[rb + i];
   const real mind = dxrad < dxt ? dxrad : dxt;
+  // the per-cell quotients by ring lengths as products with the ring's
+  // reciprocals (the substep count only changes if DT/dt sits within an ulp
+  // of an integer, and dt itself only depends on that count)
+  const real imind = (real)1 / mind, idxr = (real)1 / dxrad, idxt = (real)1 / dxt;
+  const real csr = D.adiabatic ? (real)0 : D.cs_ring[rb + i];
   real mn = (real)1e30;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    const real rs = D.vtheta[row + j];
-    const real vr = D.vrad[row + j];
-    const real cs = D.adiabatic ? D.cs[row + j] : D.cs_ring[rb + i];
+    const real rs = AT(D.vtheta, i, j);
+    const real vr = AT(D.vrad, i, j);
+    const real cs = D.adiabatic ? (real)AT(D.cs, i, j) : csr;
     // Vresidual: vtheta - Vmoy (FARGO) or vtheta (standard)
     const real vres = D.fast_transport ? (real)((double)rs - mres) : (real)(vr0 + (double)rs);
-    const real i1 = cs / mind;
-    const real i2 = fabs(vr) / dxrad;
-    const real i3 = fabs(vres) / dxt;
-    real dvr = D.vrad[row + ns + j] - vr;
-    real dvt = D.vtheta[row + (j + 1 == ns ? 0 : j + 1)] - rs;
+    const real i1 = cs * imind;
+    const real i2 = fabs(vr) * idxr;
+    const real i3 = fabs(vres) * idxt;
+    real dvr = (real)AT(D.vrad, i + 1, j) - vr;
+    real dvt = (real)AT(D.vtheta, i, j + 1 == ns ? 0 : j + 1) - rs;
     dvr = (dvr >= 0) ? (real)1e-10 : -dvr;
     dvt = (dvt >= 0) ? (real)1e-10 : -dvt;
-    real i4 = dvr / dxrad;
-    const real i4b = dvt / dxt;
+    real i4 = dvr * idxr;
+    const real i4b = dvt * idxt;
     i4 = i4b > i4 ? i4b : i4;
     i4 *= (real)(4.0 * kCVNR * kCVNR);
     const real dt2 = (real)kCFLSECURITY / sqrt(i1 * i1 + i2 * i2 + i3 * i3 + i4 * i4);
@@ -355,21 +401,22 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_force_ring(Dev<real> D) {
   FGX_RING_PROLOGUE
   __shared__ double redd[FGX_TPB / 64];
-  const long long row = fb + (long long)i * ns;
   const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
   double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
   const int nt = s.ntarget;
   for (int q = 0; q < nt; ++q) {
-    const real tx = (real)s.tx[q], ty = (real)s.ty[q], sm2 = (real)s.tsm2[q], rh2 = (real)s.trh2[q];
+    const real tx = (real)s.tx[q], ty = (real)s.ty[q], sm2 = (real)s.tsm2[q], irh2 = (real)(1.0 / s.trh2[q]);
     const int kk = s.tk[q];
     double fx = 0, fy = 0;
     for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-      const real cm = surf * D.dens[row + j];
+      const real cm = surf * (real)AT(D.dens, i, j);
       const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
       const real d2 = dx * dx + dy * dy;
       const real dist2 = d2 + sm2;
-      const real inv3 = (real)1 / dist2 / sqrt(dist2);
-      const real cut = kk ? (real)1 - exp(-d2 / rh2) : (real)1;
+      // dist2^(-3/2) from the reciprocal square root
+      const real rq = rsqrt(dist2);
+      const real inv3 = rq * rq * rq;
+      const real cut = kk ? (real)1 - exp(-d2 * irh2) : (real)1;
       fx += (double)(cm * dx * inv3 * cut);
       fy += (double)(cm * dy * inv3 * cut);
     }
@@ -620,14 +667,13 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
     };
     const real sig = AT(D.dens, i, j);
     const double p = press(i, j);
-    const double* phif = D.phi + fb;
-    const double phi = phif[(long long)i * ns + j];
+    const double phi = AT(D.phi, i, j);
     if (D.write_pressure) AT(D.pressure, i, j) = (real)p;
     // radial
     if (i > 0) {
       const real sigm = AT(D.dens, i - 1, j);
       const double pm = press(i - 1, j);
-      const double phim = phif[(long long)(i - 1) * ns + j];
+      const double phim = AT(D.phi, i - 1, j);
       const double idr = RT(invdiffrmed_d)[rb + i];
       const double gradp = (p - pm) * 2.0 / ((double)sig + (double)sigm) * idr;
       const double gradphi = (phi - phim) * idr + RT(gstar)[rb + i];
@@ -644,7 +690,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
     const real invdxt = (real)1 / RT(dxtheta)[rb + i];
     const real sigl = AT(D.dens, i, jm);
     const double pl = press(i, jm);
-    const double phil = phif[(long long)i * ns + jm];
+    const double phil = AT(D.phi, i, jm);
     const real gradp = (real)((p - pl) * 2.0 / ((double)sig + (double)sigl)) * invdxt;
     const real gradphi = (real)(phi - phil) * invdxt;
     real v = AT(D.vtheta, i, j) - (real)dt * (gradp + gradphi);
@@ -1114,6 +1160,7 @@ __global__ void k_begin_dt(Dev<real> D) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= D.ndisk) return;
   DiskState& s = D.st[d];
+  s.vpar = D.vpar;   // every disk's velocities are in the current buffers now
   if (s.frozen) { s.active = 0; return; }
   s.dtemp = 0;
   s.sub_in_dt = 0;
@@ -1141,6 +1188,27 @@ __global__ void __launch_bounds__(64) k_end_substep(Dev<real> D) {
   if (!(s.dtemp < 0.999999999 * c.DT)) {
     s.active = 0;
     s.dt_steps++;
+    s.vpar = D.vpar;   // later swaps of the DT leave this disk's velocities behind
+  }
+}
+// end of a DT with the fused source pass: a disk that stopped taking
+// substeps before the batch did holds its velocities in the buffer pair that
+// was current at its last substep; they are copied into the current pair
+// (rings 0..nrad, the ghost ring with the last strip)
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_vel_fixup(Dev<real> D) {
+  const CellTile t = D.stiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  if (t.d < 0) return;
+  if (D.st[t.d].vpar == D.vpar) return;
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
+  const int ns = c.nsec, nr = c.nrad;
+  const int j = t.jb * FGX_TPB + (int)threadIdx.x;
+  if (j >= ns) return;
+  const long long fb = c.fo;
+  const int i_end = (t.i + D.strip_rings < nr) ? t.i + D.strip_rings : nr + 1;
+  for (int i = t.i; i < i_end; ++i) {
+    AT(D.vrad, i, j) = (real)AT(D.vradint, i, j);
+    AT(D.vtheta, i, j) = (real)AT(D.vthetaint, i, j);
   }
 }
 // batch summary for the host loop: active disks and the largest CFL count
@@ -1165,7 +1233,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_potential(Dev<real> D) {
   FGX_STRIP_BEGIN {
-    D.phi[fb + (long long)i * ns + j] = potential_planets_at(D, c, s, RT(rmed_d)[rb + i], j);
+    AT(D.phi, i, j) = potential_planets_at(D, c, s, RT(rmed_d)[rb + i], j);
   }
 }
 template <class real>

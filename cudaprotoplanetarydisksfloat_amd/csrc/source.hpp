@@ -1,0 +1,247 @@
+// source.hpp -- the source half of a substep in one pass over the state:
+// the planets' potential (FillForcesArraysKernel, Kernels.cu:1322-1349),
+// Substep1 (Kernels.cu:52-88), the viscous stresses (ViscousTermsKernelDRP /
+// TAURP, Kernels.cu:843-880), the viscous velocity update with the
+// sub-Keplerian boundary rows (Kernels.cu:142-160, 1490-1503) and Substep2
+// (Kernels.cu:570-621).
+//
+// One wave owns a chunk of FGX_SRC_OUT sectors of a strip of rings plus four
+// extra lanes on each side, and walks the strip outwards running four stages
+// one ring apart:
+//   A  ring k     potential, pressure, Substep1 -> VradInt, VthetaInt
+//   B  ring k-1   stresses from VradInt, VthetaInt
+//   C  ring k-2   viscous update of VradInt, VthetaInt; boundary rows
+//   D  ring k-3   Substep2 -> new Vrad, Vtheta, [EnergyInt]   (stored)
+// A stage's azimuthal neighbours come from the adjacent lanes (DPP), its
+// radial neighbours from the registers of the previous iterations, so the
+// intermediate fields (potential, VradInt, VthetaInt, the stresses) never
+// reach memory.  Each stage needs one more lane on each side than the next,
+// hence the four extra lanes, and one more ring below and above, hence the
+// three extra rings on each end of a strip.  The arithmetic is that of the
+// separate kernels, operation for operation.
+#pragma once
+#include "transport.hpp"
+
+namespace fgx {
+
+#define FGX_SRC_OUT 57   // sectors stored per wave: lanes 4..60
+
+template <class real>
+__global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
+  const CellTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  if (t.d < 0) return;
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
+  const FGX_AS4 DiskState& s = cref(D.st + t.d);
+  if (!D.ignore_active && !s.active) return;
+  const int ns = c.nsec, nr = c.nrad;
+  const long long fb = c.fo;
+  const int rb = c.ro;
+  const int lane = (int)threadIdx.x;
+  // this lane's sector; lanes 4..60 of the chunk are stored
+  const int first = t.jb * FGX_SRC_OUT;
+  int j = (first - 4 + lane) % ns;
+  if (j < 0) j += ns;
+  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+  const bool out = lane >= 4 && lane <= 60 && first + lane - 4 < ns;
+  const int i0 = t.i, i1 = (t.i + rs < nr) ? t.i + rs : nr;
+  const bool visc = D.viscous, ad = D.adiabatic;
+  const double dtd = s.dt, om = s.omega_frame;
+  const real dt = (real)dtd;
+  const double g1 = c.adiabaticindex - 1.0;
+  const real invdphi = (real)((double)ns / (2.0 * kPI));
+  const real third = (real)(1.0 / 3.0);
+  const real cv2 = (real)(kCVNR * kCVNR);
+
+  // the ring coefficients of rings [i0-4, i0+rs+3) staged in LDS, one lane
+  // per ring: inside the loop they are read from LDS into vector registers,
+  // instead of some fifty per-ring values and their table addresses being
+  // held in scalar registers across the loop
+  __shared__ double cd_[7][64];   // cs2_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d
+  __shared__ real cf_[10][64];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed
+  const int rlo = i0 - 4;
+  {
+    const int r = rlo + lane;
+    if (r >= 0 && r < nr && lane < rs + 8) {
+      cd_[0][lane] = RT(cs2_d)[rb + r];
+      cd_[1][lane] = RT(rmed_d)[rb + r];
+      cd_[2][lane] = RT(invdiffrmed_d)[rb + r];
+      cd_[3][lane] = RT(gstar)[rb + r];
+      cd_[4][lane] = RT(vref)[rb + r];
+      cd_[5][lane] = RT(rinf_d)[rb + r];
+      cd_[6][lane] = RT(invrinf_d)[rb + r];
+      cf_[0][lane] = (real)1 / RT(dxtheta)[rb + r];
+      cf_[1][lane] = RT(supp_torque)[rb + r];
+      cf_[2][lane] = RT(invrmed)[rb + r];
+      cf_[3][lane] = RT(invdiffrsup)[rb + r];
+      cf_[4][lane] = RT(rsup)[rb + r];
+      cf_[5][lane] = RT(rinf)[rb + r];
+      cf_[6][lane] = RT(visc)[rb + r];
+      cf_[7][lane] = RT(invrinf)[rb + r];
+      cf_[8][lane] = RT(rmed)[rb + r];
+      cf_[9][lane] = RT(invdiffrmed)[rb + r];
+    }
+  }
+  __syncthreads();
+#define FGX_CD(q, r) cd_[q][(r) - rlo]
+#define FGX_CF(q, r) cf_[q][(r) - rlo]
+
+  // ring k-1 inputs of stage A
+  real sgA1 = 0, vtA1 = 0, vtpA1 = 0;
+  double pA1 = 0, phiA1 = 0;
+  // density and energy of rings k-1 .. k-4 (index 1..4), A's outputs of
+  // rings k-1, k-2, the stresses of rings k-2, k-3, C's outputs of rings
+  // k-3, k-4
+  real sg1 = 0, sg2 = 0, sg3 = 0, sg4 = 0, e1 = 0, e2 = 0, e3 = 0;
+  real vri1 = 0, vri2 = 0, vti1 = 0, vti2 = 0;
+  real trr1 = 0, trr2 = 0, tpp1 = 0, tpp2 = 0, trp1 = 0;
+  real vrc1 = 0, vrc2 = 0, vtc1 = 0;
+
+  auto pressure = [&](real sg, real e, int k) -> double {
+    return ad ? g1 * (double)e : (double)sg * FGX_CD(0, k);
+  };
+  const int ks = (i0 - 3 > 0) ? i0 - 3 : 0;
+  if (ks > 0) {   // prime stage A with ring ks-1
+    const int k = ks - 1;
+    sgA1 = AT(D.dens, k, j);
+    vtA1 = AT(D.vtheta, k, j);
+    vtpA1 = AT(D.vtheta, k, jp);
+    pA1 = pressure(sgA1, ad ? (real)AT(D.energy, k, j) : (real)0, k);
+    phiA1 = potential_planets_at(D, c, s, FGX_CD(1, k), j);
+  }
+  // past the outer edge A and B are skipped and C yields VradInt(nrad) = 0
+  const int ke = i1 + 2;
+  for (int k = ks; k <= ke; ++k) {
+    // ---------------------------------------------------------- stage A
+    real sg0 = 0, e0 = 0, vri0 = 0, vti0 = 0;
+    if (k < nr) {
+      sg0 = AT(D.dens, k, j);
+      const real vr = AT(D.vrad, k, j);
+      const real vt = AT(D.vtheta, k, j), vtp = AT(D.vtheta, k, jp);
+      if (ad) e0 = AT(D.energy, k, j);
+      const double p = pressure(sg0, e0, k);
+      const double phi = potential_planets_at(D, c, s, FGX_CD(1, k), j);
+      if (D.write_pressure && out && k >= i0 && k < i1) AT(D.pressure, k, j) = (real)p;
+      if (k > 0) {
+        const double idr = FGX_CD(2, k);
+        const double gradp = (p - pA1) * 2.0 / ((double)sg0 + (double)sgA1) * idr;
+        const double gradphi = (phi - phiA1) * idr + FGX_CD(3, k);
+        double vt2 = 2.0 * FGX_CD(4, k) + 2.0 * FGX_CD(4, k - 1) + (double)vt + (double)vtp +
+                     (double)vtA1 + (double)vtpA1;
+        vt2 = vt2 / 4.0 + FGX_CD(5, k) * om;
+        vt2 = vt2 * vt2;
+        vri0 = (real)((double)vr + dtd * (-gradp - gradphi + vt2 * FGX_CD(6, k)));
+      }
+      // azimuthal (the star's potential is the same at j and j-1)
+      const real invdxt = FGX_CF(0, k);
+      const real sigl = lane_from_left(sg0);
+      const double pl = lane_from_left(p);
+      const double phil = lane_from_left(phi);
+      const real gradp = (real)((p - pl) * 2.0 / ((double)sg0 + (double)sigl)) * invdxt;
+      const real gradphi = (real)(phi - phil) * invdxt;
+      real v = vt - (real)dtd * (gradp + gradphi);
+      v += (real)dtd * FGX_CF(1, k);
+      vti0 = v;
+      sgA1 = sg0; vtA1 = vt; vtpA1 = vtp; pA1 = p; phiA1 = phi;
+    }
+    // ---------------------------------------------------------- stage B
+    const int b = k - 1;
+    real trr0 = 0, tpp0 = 0, trp0 = 0;
+    if (visc && b >= 0 && b < nr && b >= i0 - 2) {
+      const real irm = FGX_CF(2, b);
+      const real vr0 = vri1, vr1 = vri0;
+      const real vt0 = vti1, vtp = lane_from_right(vti1);
+      const real drr = (vr1 - vr0) * FGX_CF(3, b);
+      const real dpp = (vtp - vt0) * invdphi * irm + (real)0.5 * (vr1 + vr0) * irm;
+      real divv = (vr1 * FGX_CF(4, b) - vr0 * FGX_CF(5, b)) * FGX_CF(3, b) * irm;
+      divv += (vtp - vt0) * invdphi * irm;
+      const real nu = FGX_CF(6, b);
+      const real sig = sg1;
+      trr0 = (real)2 * sig * nu * (drr - third * divv);
+      tpp0 = (real)2 * sig * nu * (dpp - third * divv);
+      if (b > 0) {
+        const double w0 = (FGX_CD(4, b) + (double)vt0) / FGX_CD(1, b);
+        const double w1 = (FGX_CD(4, b - 1) + (double)vti2) / FGX_CD(1, b - 1);
+        const real drp = (real)0.5 * ((real)(FGX_CD(5, b) * (w0 - w1) * FGX_CD(2, b)) +
+                                      (vr0 - lane_from_left(vri1)) * invdphi * FGX_CF(7, b));
+        // corner density: column 0 of ring b as the reference indexes it
+        // (Kernels.cu:876), or column j-1 with the quirk fix
+        const real sc = D.fix_taurp ? lane_from_left(sig) : (real)AT(D.dens, b, 0);
+        trp0 = (real)(2.0 * 0.25) * (sig + sg2 + sc + lane_from_left(sg2)) * nu * drp;
+      }
+    }
+    // ---------------------------------------------------------- stage C
+    const int cr = k - 2;
+    real vrc0 = vri2, vtc0 = vti2;
+    if (cr >= 0 && cr >= i0 - 1) {
+      if (cr >= nr) {
+        vrc0 = 0;
+      } else {
+        if (visc && cr > 0 && cr < nr - 1) {
+          const real sgl = lane_from_left(sg2);
+          vtc0 += dt * FGX_CF(2, cr) *
+                  ((FGX_CF(4, cr) * trp0 - FGX_CF(5, cr) * trp1) * FGX_CF(3, cr) +
+                   (tpp1 - lane_from_left(tpp1)) * invdphi + (real)0.5 * (trp1 + trp0)) /
+                  ((real)0.5 * (sg2 + sgl));
+        }
+        if (visc && cr > 0) {
+          vrc0 += dt * FGX_CF(7, cr) *
+                  ((FGX_CF(8, cr) * trr1 - FGX_CF(8, cr - 1) * trr2) * FGX_CF(9, cr) +
+                   (lane_from_right(trp1) - trp1) * invdphi - (real)0.5 * (tpp1 + tpp2)) /
+                  ((real)0.5 * (sg2 + sg3));
+        }
+        if (!D.evanescent) {
+          if (cr == 0) {
+            const double v = s.vkep_in - (D.self_gravity ? RT(rmed_d)[rb] * D.axisg[rb] : 0.0);
+            vtc0 = (real)(sqrt(v) - RT(rmed_d)[rb] * om - RT(vref)[rb]);
+          }
+          if (cr == nr - 1) {
+            const double v = s.vkep_out - (D.self_gravity ? RT(rmed_d)[rb + nr - 1] * D.axisg[rb + nr - 1] : 0.0);
+            vtc0 = (real)(sqrt(v) - RT(rmed_d)[rb + nr - 1] * om - RT(vref)[rb + nr - 1]);
+          }
+        }
+      }
+    }
+    // ---------------------------------------------------------- stage D
+    const int d = k - 3;
+    if (d >= i0 && d < i1) {
+      const real sig = sg3;
+      const real vri = vrc1, vrp = vrc0;
+      real dv = vrp - vri;
+      const real qr = dv < 0 ? cv2 * sig * dv * dv : (real)0;
+      const real vti = vtc1, vtp = lane_from_right(vtc1), vtm = lane_from_left(vtc1);
+      dv = vtp - vti;
+      const real qt = dv < 0 ? cv2 * sig * dv * dv : (real)0;
+      const real sigl = lane_from_left(sig);
+      dv = vti - vtm;
+      const real qtl = dv < 0 ? cv2 * sigl * dv * dv : (real)0;
+      const real invdxt = FGX_CF(0, d);
+      real vrn = 0;
+      if (d > 0) {
+        const real sigm = sg4;
+        dv = vri - vrc2;
+        const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
+        vrn = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * FGX_CF(9, d);
+      }
+      const real vtn = vti - dt * (real)2 / (sig + sigl) * (qt - qtl) * invdxt;
+      if (out) {
+        // into the spare VradInt / VthetaInt arrays, swapped with Vrad /
+        // Vtheta after the launch: the neighbouring strips still read this
+        // strip's old velocities as their halo
+        AT(D.vradint, d, j) = vrn;
+        AT(D.vthetaint, d, j) = vtn;
+        if (ad)
+          AT(D.energyint, d, j) = e3 - dt * qr * (vrp - vri) * FGX_CF(3, d) - dt * qt * (vtp - vti) * invdxt;
+      }
+    }
+    // ---------------------------------------------------------- shift
+    sg4 = sg3; sg3 = sg2; sg2 = sg1; sg1 = sg0;
+    e3 = e2; e2 = e1; e1 = e0;
+    vri2 = vri1; vri1 = vri0; vti2 = vti1; vti1 = vti0;
+    trr2 = trr1; trr1 = trr0; tpp2 = tpp1; tpp1 = tpp0; trp1 = trp0;
+    vrc2 = vrc1; vrc1 = vrc0; vtc1 = vtc0;
+  }
+}
+#undef FGX_CD
+#undef FGX_CF
+
+}  // namespace fgx

@@ -174,3 +174,31 @@ def test_mass_conserved_closed(gpu):
     disk.step(3)
     m1 = (disk.field("dens") * surf[:, None]).sum()
     assert abs(m1 / m0 - 1) < 1e-13
+
+
+@pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["plain", "plain_log_alpha", "adiabatic_evanescent", "adiabatic_cooling_open",
+                                  "nonreflecting", "two_planets", "guiding_center"])
+def test_fused_source_equals_separate_kernels(gpu, monkeypatch, case, precision):
+    """The one-pass source kernel (potential, Substep1, stresses, viscous
+    update, Substep2) does the separate kernels' arithmetic operation for
+    operation: after CFL-driven steps every field is bit-identical."""
+    par, cfg, keys = _case(case)
+    keys = dict(keys)
+    keys.setdefault("NRAD", 70)   # several strips and a ragged last one
+    p, pl = configs.load(par, cfg, DT=DT_SMALL, **keys)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FGX_FUSED_SOURCE", fused)
+        monkeypatch.setenv("FGX_SOURCE_RINGS", "8")
+        disk = Batch(p, pl, precision)
+        disk.step(3)
+        names = ("dens", "vrad", "vtheta") + (("energy",) if int(p.get("ADIABATIC_F")) else ())
+        out[fused] = {n: disk.field(n) for n in names}
+        out[fused]["substeps"] = disk.info().substeps
+    assert out["1"]["substeps"] == out["0"]["substeps"]
+    for n in out["0"]:
+        if n == "substeps":
+            continue
+        a, b = out["1"][n], out["0"][n]
+        assert np.array_equal(a, b), (case, n, float(np.abs(a - b).max()))
