@@ -119,6 +119,22 @@ def cpu_baseline(loaded, budget_s=20.0):
                                       f"{total} cell-updates in {dt:.1f} s"}
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_batch64_fp32.txt")
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of a kernel class, FETCH_SIZE + WRITE_SIZE from
+    the two rocprofv3 --pmc runs of this workload recorded in profiles/
+    (scaled by a 256 MiB copy kernel's counts); None for other workloads."""
+    if args.workload != "batch" or args.precision != "f32" or not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) >= 5 and f[0] == kernel:
+            return int(float(f[3]) + float(f[4])), os.path.relpath(TRAFFIC_FILE, os.path.dirname(TRAFFIC_FILE) + "/..")
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,8 +199,11 @@ def main():
     d = kern[dom]
     achieved = d["alg_bytes_per_launch"] / (d["avg_us"] * 1e-6) / 1e9
     sub_ms = sum(v["ms"] for v in kern.values())
+    traffic, traffic_src = measured_traffic(dom, args)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": traffic_src,
+                "alg_bytes_per_launch": round(d["alg_bytes_per_launch"]),
                 "avg_us": round(d["avg_us"], 2),
                 "kernels": {k: {"avg_us": round(v["avg_us"], 2), "share": round(v["ms"] / sub_ms, 3),
                                 "GBs": round(v["alg_bytes_per_launch"] / (v["avg_us"] * 1e-6) / 1e9, 1)}
