@@ -338,16 +338,16 @@ struct Batch : BatchBase {
     for (real** r : rings) LCHK(dalloc(r, R));
     LCHK(dalloc(&D.nshift, R));
     {
-      double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d;
+      double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d;
       LCHK(dalloc(&vref, R)); LCHK(dalloc(&gstar, R)); LCHK(dalloc(&rinf_d, R)); LCHK(dalloc(&invrinf_d, R));
       LCHK(dalloc(&cs2_d, R)); D.cs2_d = cs2_d;
       double *sigmamed_d, *energymed_d;
       LCHK(dalloc(&sigmamed_d, R)); LCHK(dalloc(&energymed_d, R));
       D.sigmamed_d = sigmamed_d; D.energymed_d = energymed_d;
-      LCHK(dalloc(&rmed_d, R)); LCHK(dalloc(&invdiffrmed_d, R)); LCHK(dalloc(&D.vmoy, R));
+      LCHK(dalloc(&rmed_d, R)); LCHK(dalloc(&invrmed_d, R)); LCHK(dalloc(&invdiffrmed_d, R)); LCHK(dalloc(&D.vmoy, R));
       LCHK(dalloc(&cosj_d, S)); LCHK(dalloc(&sinj_d, S));
       D.vref = vref; D.gstar = gstar; D.rinf_d = rinf_d; D.invrinf_d = invrinf_d; D.rmed_d = rmed_d;
-      D.invdiffrmed_d = invdiffrmed_d; D.cosj_d = cosj_d; D.sinj_d = sinj_d;
+      D.invdiffrmed_d = invdiffrmed_d; D.cosj_d = cosj_d; D.sinj_d = sinj_d; D.invrmed_d = invrmed_d;
     }
     LCHK(dalloc(&D.cosj, S)); LCHK(dalloc(&D.sinj, S)); LCHK(dalloc(&D.lost, S));
     DiskConst* dcd; DiskState* std_;
@@ -489,17 +489,19 @@ struct Batch : BatchBase {
       const size_t n = (size_t)h.nrad + 1;
       // FP64 tables: reference vtheta (the initial, ring-uniform profile),
       // star gravity between ring centres, geometry
-      std::vector<double> vref(n, 0.0), gstar(n, 0.0), invrinf(n, 0.0), cs2(n, 0.0);
+      std::vector<double> vref(n, 0.0), gstar(n, 0.0), invrinf(n, 0.0), cs2(n, 0.0), invrmed(n, 0.0);
       for (int i = 0; i < h.nrad; ++i) cs2[i] = h.cs_ring[i] * h.cs_ring[i];
       LCHK(upload_d(D.cs2_d + o, cs2));
       LCHK(upload_d(D.sigmamed_d + o, h.sigmamed)); LCHK(upload_d(D.energymed_d + o, h.energymed));
       for (int i = 0; i < h.nrad; ++i) vref[i] = h.vtheta[(size_t)i * h.nsec];
       for (int i = 1; i < h.nrad; ++i) gstar[i] = (1.0 / h.rmed[i - 1] - 1.0 / h.rmed[i]) / (h.rmed[i] - h.rmed[i - 1]);
       for (int i = 0; i < h.nrad; ++i) invrinf[i] = 1.0 / h.rinf[i];
+      for (int i = 0; i < h.nrad; ++i) invrmed[i] = 1.0 / h.rmed[i];
       vref_h[d] = vref;
       LCHK(upload_d(D.vref + o, vref)); LCHK(upload_d(D.gstar + o, gstar));
       LCHK(upload_d(D.rinf_d + o, h.rinf)); LCHK(upload_d(D.invrinf_d + o, invrinf));
       LCHK(upload_d(D.rmed_d + o, h.rmed)); LCHK(upload_d(D.invdiffrmed_d + o, h.invdiffrmed));
+      LCHK(upload_d(D.invrmed_d + o, invrmed));
       LCHK(upload_d(D.cosj_d + dch[d].so, h.cosj)); LCHK(upload_d(D.sinj_d + dch[d].so, h.sinj));
       LCHK(upload_vec(D.rinf + o, h.rinf, n)); LCHK(upload_vec(D.rsup + o, h.rsup, n));
       LCHK(upload_vec(D.rmed + o, h.rmed, n)); LCHK(upload_vec(D.surf + o, h.surf, n));

@@ -37,7 +37,8 @@ struct Dev {
   // the reference azimuthal velocity profile (vtheta and the angular momenta
   // are stored as FP32 residuals from it), the star's radial gravity between
   // ring centres, and the cell geometry of the potential
-  const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d, *energymed_d;
+  const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d,
+      *energymed_d;
   double* vmoy;  // ring-mean vtheta of the CFL condition
   double* phi;   // potential of the planets and the indirect term, FP64 field
   // self-gravity: transform work rows, kernel transforms, twiddles, ring
@@ -86,6 +87,27 @@ __device__ __forceinline__ const FGX_AS4 T& cref(const T* p) { return *(const FG
 template <class T>
 __device__ __forceinline__ const FGX_AS4 T* cptr(const T* p) { return (const FGX_AS4 T*)p; }
 #define RT(name) cptr(D.name)
+
+// quotients of FP32 field values as products with the hardware reciprocal
+// (1 ulp), exact division in FP64
+template <class real>
+__device__ __forceinline__ real rcp_of(real x) {
+  if constexpr (sizeof(real) == 4) return __builtin_amdgcn_rcpf(x);
+  else return (real)1 / x;
+}
+template <class real>
+__device__ __forceinline__ real qdiv(real a, real b) {
+  if constexpr (sizeof(real) == 4) return a * __builtin_amdgcn_rcpf(b);
+  else return a / b;
+}
+// Substep1's azimuthal pressure gradient (times dx_theta): the FP64 pressure
+// difference over the mean density; the quotient is rounded to the field
+// precision anyway, so FP32 forms it with a reciprocal
+template <class real>
+__device__ __forceinline__ real azim_gradp(double p, double pl, real sig, real sigl) {
+  if constexpr (sizeof(real) == 4) return (real)(p - pl) * (real)2 * __builtin_amdgcn_rcpf(sig + sigl);
+  else return (real)((p - pl) * 2.0 / ((double)sig + (double)sigl));
+}
 __device__ __forceinline__ int xcd_tile(int b, int n) {
   // n is a multiple of 8; hardware hands blocks to the 8 XCDs round-robin
   const int per = n >> 3;
@@ -691,7 +713,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
     const real sigl = AT(D.dens, i, jm);
     const double pl = press(i, jm);
     const double phil = AT(D.phi, i, jm);
-    const real gradp = (real)((p - pl) * 2.0 / ((double)sig + (double)sigl)) * invdxt;
+    const real gradp = azim_gradp(p, pl, sig, sigl) * invdxt;
     const real gradphi = (real)(phi - phil) * invdxt;
     real v = AT(D.vtheta, i, j) - (real)dt * (gradp + gradphi);
     v += (real)dt * RT(supp_torque)[rb + i];
@@ -725,8 +747,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
     if (i > 0) {
       // the angular velocity difference between rings is formed in FP64 from
       // the residuals and the reference profile
-      const double w0 = (RT(vref)[rb + i] + (double)vt0) / RT(rmed_d)[rb + i];
-      const double w1 = (RT(vref)[rb + i - 1] + (double)AT(vt, i - 1, j)) / RT(rmed_d)[rb + i - 1];
+      const double w0 = (RT(vref)[rb + i] + (double)vt0) * RT(invrmed_d)[rb + i];
+      const double w1 = (RT(vref)[rb + i - 1] + (double)AT(vt, i - 1, j)) * RT(invrmed_d)[rb + i - 1];
       const real drp = (real)0.5 * ((real)(RT(rinf_d)[rb + i] * (w0 - w1) * RT(invdiffrmed_d)[rb + i]) +
                                     (vr0 - AT(vr, i, jm)) * invdphi * RT(invrinf)[rb + i]);
       // corner density: column 0 of ring i as the reference indexes it
@@ -750,17 +772,19 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
     if (D.viscous && i > 0 && i < nr - 1) {
       const real trp0 = AT(D.taurp, i, j), trp1 = AT(D.taurp, i + 1, j);
       AT(D.vthetaint, i, j) +=
-          dt * RT(invrmed)[rb + i] *
-          ((RT(rsup)[rb + i] * trp1 - RT(rinf)[rb + i] * trp0) * RT(invdiffrsup)[rb + i] +
-           (AT(D.taupp, i, j) - AT(D.taupp, i, jm)) * invdphi + (real)0.5 * (trp0 + trp1)) /
-          ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i, jm)));
+          qdiv(dt * RT(invrmed)[rb + i] *
+                   ((RT(rsup)[rb + i] * trp1 - RT(rinf)[rb + i] * trp0) * RT(invdiffrsup)[rb + i] +
+                    (AT(D.taupp, i, j) - AT(D.taupp, i, jm)) * invdphi + (real)0.5 * (trp0 + trp1)),
+               (real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i, jm)));
     }
     if (D.viscous && i > 0) {
       AT(D.vradint, i, j) +=
-          dt * RT(invrinf)[rb + i] *
-          ((RT(rmed)[rb + i] * AT(D.taurr, i, j) - RT(rmed)[rb + i - 1] * AT(D.taurr, i - 1, j)) * RT(invdiffrmed)[rb + i] +
-           (AT(D.taurp, i, jp) - AT(D.taurp, i, j)) * invdphi - (real)0.5 * (AT(D.taupp, i, j) + AT(D.taupp, i - 1, j))) /
-          ((real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
+          qdiv(dt * RT(invrinf)[rb + i] *
+                   ((RT(rmed)[rb + i] * AT(D.taurr, i, j) - RT(rmed)[rb + i - 1] * AT(D.taurr, i - 1, j)) *
+                        RT(invdiffrmed)[rb + i] +
+                    (AT(D.taurp, i, jp) - AT(D.taurp, i, j)) * invdphi -
+                    (real)0.5 * (AT(D.taupp, i, j) + AT(D.taupp, i - 1, j))),
+               (real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
     }
     if (!D.evanescent) {
       // with self-gravity the Keplerian values include -r * <SG_Accr>_ring
@@ -802,11 +826,11 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep2(Dev<real> D) {
       const real sigm = AT(D.dens, i - 1, j);
       dv = vri - AT(D.vradint, i - 1, j);
       const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
-      AT(D.vrad, i, j) = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * RT(invdiffrmed)[rb + i];
+      AT(D.vrad, i, j) = vri - qdiv(dt * (real)2, sig + sigm) * (qr - qrm) * RT(invdiffrmed)[rb + i];
     } else {
       AT(D.vrad, i, j) = 0;
     }
-    AT(D.vtheta, i, j) = vti - dt * (real)2 / (sig + sigl) * (qt - qtl) * invdxt;
+    AT(D.vtheta, i, j) = vti - qdiv(dt * (real)2, sig + sigl) * (qt - qtl) * invdxt;
     if (D.adiabatic)
       AT(D.energyint, i, j) = AT(D.energy, i, j) - dt * qr * (vrp - vri) * RT(invdiffrsup)[rb + i] -
                               dt * qt * (vtp - vti) * invdxt;

@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
   // per ring: inside the loop they are read from LDS into vector registers,
   // instead of some fifty per-ring values and their table addresses being
   // held in scalar registers across the loop
-  __shared__ double cd_[7][64];   // cs2_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d
+  __shared__ double cd_[8][64];   // cs2_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d invrmed_d
   __shared__ real cf_[10][64];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed
   const int rlo = i0 - 4;
   {
@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
       cd_[4][lane] = RT(vref)[rb + r];
       cd_[5][lane] = RT(rinf_d)[rb + r];
       cd_[6][lane] = RT(invrinf_d)[rb + r];
+      cd_[7][lane] = RT(invrmed_d)[rb + r];
       cf_[0][lane] = (real)1 / RT(dxtheta)[rb + r];
       cf_[1][lane] = RT(supp_torque)[rb + r];
       cf_[2][lane] = RT(invrmed)[rb + r];
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
       const real sigl = lane_from_left(sg0);
       const double pl = lane_from_left(p);
       const double phil = lane_from_left(phi);
-      const real gradp = (real)((p - pl) * 2.0 / ((double)sg0 + (double)sigl)) * invdxt;
+      const real gradp = azim_gradp(p, pl, sg0, sigl) * invdxt;
       const real gradphi = (real)(phi - phil) * invdxt;
       real v = vt - (real)dtd * (gradp + gradphi);
       v += (real)dtd * FGX_CF(1, k);
@@ -159,8 +160,8 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
       trr0 = (real)2 * sig * nu * (drr - third * divv);
       tpp0 = (real)2 * sig * nu * (dpp - third * divv);
       if (b > 0) {
-        const double w0 = (FGX_CD(4, b) + (double)vt0) / FGX_CD(1, b);
-        const double w1 = (FGX_CD(4, b - 1) + (double)vti2) / FGX_CD(1, b - 1);
+        const double w0 = (FGX_CD(4, b) + (double)vt0) * FGX_CD(7, b);
+        const double w1 = (FGX_CD(4, b - 1) + (double)vti2) * FGX_CD(7, b - 1);
         const real drp = (real)0.5 * ((real)(FGX_CD(5, b) * (w0 - w1) * FGX_CD(2, b)) +
                                       (vr0 - lane_from_left(vri1)) * invdphi * FGX_CF(7, b));
         // corner density: column 0 of ring b as the reference indexes it
@@ -178,16 +179,16 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
       } else {
         if (visc && cr > 0 && cr < nr - 1) {
           const real sgl = lane_from_left(sg2);
-          vtc0 += dt * FGX_CF(2, cr) *
-                  ((FGX_CF(4, cr) * trp0 - FGX_CF(5, cr) * trp1) * FGX_CF(3, cr) +
-                   (tpp1 - lane_from_left(tpp1)) * invdphi + (real)0.5 * (trp1 + trp0)) /
-                  ((real)0.5 * (sg2 + sgl));
+          vtc0 += qdiv(dt * FGX_CF(2, cr) *
+                           ((FGX_CF(4, cr) * trp0 - FGX_CF(5, cr) * trp1) * FGX_CF(3, cr) +
+                            (tpp1 - lane_from_left(tpp1)) * invdphi + (real)0.5 * (trp1 + trp0)),
+                       (real)0.5 * (sg2 + sgl));
         }
         if (visc && cr > 0) {
-          vrc0 += dt * FGX_CF(7, cr) *
-                  ((FGX_CF(8, cr) * trr1 - FGX_CF(8, cr - 1) * trr2) * FGX_CF(9, cr) +
-                   (lane_from_right(trp1) - trp1) * invdphi - (real)0.5 * (tpp1 + tpp2)) /
-                  ((real)0.5 * (sg2 + sg3));
+          vrc0 += qdiv(dt * FGX_CF(7, cr) *
+                           ((FGX_CF(8, cr) * trr1 - FGX_CF(8, cr - 1) * trr2) * FGX_CF(9, cr) +
+                            (lane_from_right(trp1) - trp1) * invdphi - (real)0.5 * (tpp1 + tpp2)),
+                       (real)0.5 * (sg2 + sg3));
         }
         if (!D.evanescent) {
           if (cr == 0) {
@@ -220,9 +221,9 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
         const real sigm = sg4;
         dv = vri - vrc2;
         const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
-        vrn = vri - dt * (real)2 / (sig + sigm) * (qr - qrm) * FGX_CF(9, d);
+        vrn = vri - qdiv(dt * (real)2, sig + sigm) * (qr - qrm) * FGX_CF(9, d);
       }
-      const real vtn = vti - dt * (real)2 / (sig + sigl) * (qt - qtl) * invdxt;
+      const real vtn = vti - qdiv(dt * (real)2, sig + sigl) * (qt - qtl) * invdxt;
       if (out) {
         // into the spare VradInt / VthetaInt arrays, swapped with Vrad /
         // Vtheta after the launch: the neighbouring strips still read this
