@@ -287,21 +287,53 @@ __device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const
   return pot;
 }
 
-// ===================================================================== CFL
+// ============================================================ CFL, forces
 // ConditionCFLKernel1D + ConditionCFLKernel2D1 + per-ring min of
-// ConditionCFLKernel2D2 (Kernels.cu:1363-1422): one block per ring
+// ConditionCFLKernel2D2 (Kernels.cu:1363-1422) and ComputeForceKernel
+// (Kernels.cu:804-839) for the substep's targets, both over the state at the
+// start of the substep: one wave per ring, ring sums and minima by shuffles
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
-  FGX_RING_PROLOGUE
+__global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D) {
+  const RingTile t = D.rtiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  if (t.d < 0) return;
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
+  const FGX_AS4 DiskState& s = cref(D.st + t.d);
+  if (!D.ignore_active && !s.active) return;
+  const int ns = c.nsec, i = t.i, rb = c.ro;
+  const long long fb = c.fo;
+  const int lane = (int)threadIdx.x;
+  // disk forces on the targets (k = 0 or k = 10 Hill cut), per ring in FP64
+  {
+    const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
+    double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
+    const int nt = s.ntarget;
+    for (int q = 0; q < nt; ++q) {
+      const real tx = (real)s.tx[q], ty = (real)s.ty[q], sm2 = (real)s.tsm2[q], irh2 = (real)(1.0 / s.trh2[q]);
+      const int kk = s.tk[q];
+      double fx = 0, fy = 0;
+      for (int j = lane; j < ns; j += 64) {
+        const real cm = surf * (real)AT(D.dens, i, j);
+        const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
+        const real d2 = dx * dx + dy * dy;
+        const real dist2 = d2 + sm2;
+        // dist2^(-3/2) from the reciprocal square root
+        const real rq = rsqrt(dist2);
+        const real inv3 = rq * rq * rq;
+        const real cut = kk ? (real)1 - exp(-d2 * irh2) : (real)1;
+        fx += (double)(cm * dx * inv3 * cut);
+        fy += (double)(cm * dy * inv3 * cut);
+      }
+      fx = wave_sum(fx);
+      fy = wave_sum(fy);
+      if (lane == 0) { out[2 * q] = fx; out[2 * q + 1] = fy; }
+    }
+  }
   if (c.substeps_forced > 0 || (D.sloppy && s.sub_in_dt > 0)) return;
-  __shared__ double redd[FGX_TPB / 64];
-  __shared__ real redr[FGX_TPB / 64];
   const double vr0 = RT(vref)[rb + i];
   double acc = 0;
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) acc += (double)(real)AT(D.vtheta, i, j);
-  acc = block_sum(acc, redd);
-  const double mres = acc / (double)ns;   // ring mean of the residual
-  if (threadIdx.x == 0) D.vmoy[rb + i] = vr0 + mres;
+  for (int j = lane; j < ns; j += 64) acc += (double)(real)AT(D.vtheta, i, j);
+  const double mres = wave_sum(acc) / (double)ns;   // ring mean of the residual
+  if (lane == 0) D.vmoy[rb + i] = vr0 + mres;
   if (i == 0) return;
   const real dxrad = RT(rsup)[rb + i] - RT(rinf)[rb + i];
   const real dxt = RT(dxtheta)[rb + i];
@@ -312,7 +344,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
   const real imind = (real)1 / mind, idxr = (real)1 / dxrad, idxt = (real)1 / dxt;
   const real csr = D.adiabatic ? (real)0 : D.cs_ring[rb + i];
   real mn = (real)1e30;
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+  for (int j = lane; j < ns; j += 64) {
     const real rs = AT(D.vtheta, i, j);
     const real vr = AT(D.vrad, i, j);
     const real cs = D.adiabatic ? (real)AT(D.cs, i, j) : csr;
@@ -332,8 +364,31 @@ __global__ void __launch_bounds__(FGX_TPB) k_cfl_ring(Dev<real> D) {
     const real dt2 = (real)kCFLSECURITY / sqrt(i1 * i1 + i2 * i2 + i3 * i3 + i4 * i4);
     mn = dt2 < mn ? dt2 : mn;
   }
-  mn = block_min(mn, redr);
-  if (threadIdx.x == 0) D.newdt[rb + i] = mn;
+  mn = wave_min(mn);
+  if (lane == 0) D.newdt[rb + i] = mn;
+}
+
+// force targets of the next substep: the star (Pframeforce.cu via
+// SourceEuler.cu:247) and the planets that feel the disk (Pframeforce.cu:68-92)
+// at their current positions; set where a disk's substep starts its DT and
+// after each of its substeps, so the CFL and force pass can run first
+template <class real>
+__device__ void set_force_targets(const Dev<real>& D, const DiskConst& c, DiskState& s) {
+  s.tx[0] = 0; s.ty[0] = 0; s.tsm2[0] = 0; s.trh2[0] = 1e-30; s.tk[0] = 0;
+  int nt = 1;
+  const double tt = s.time - s.time0;
+  for (int k = 0; k < c.nb; ++k) {
+    if (!c.feel_disk[k]) continue;
+    const double m = c.mass[k], x = s.x[k], y = s.y[k];
+    const double r = sqrt(x * x + y * y);
+    const double sm = D.roche ? r * pow(m / 3., 1. / 3.) * c.rochesmoothing
+                              : c.thicknesssmoothing * aspect_ratio_dev(c, r, tt) * pow(r, 1.0 + c.flaringindex);
+    const double rh = pow(m / 3., 1. / 3.) * r + 1e-15;
+    s.tx[nt] = x; s.ty[nt] = y; s.tsm2[nt] = sm * sm; s.trh2[nt] = rh * rh;
+    s.tk[nt] = D.exclude_hill ? 10 : 0;
+    nt++;
+  }
+  s.ntarget = nt;
 }
 
 // ============================================================ substep begin
@@ -385,24 +440,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
   s.ncfl = n;
   s.dtemp += s.dt;
   if (D.corotating) get_psys_info_dev(D, c, s, 1);
-  // force targets: the star (Pframeforce.cu via SourceEuler.cu:247) and the
-  // planets that feel the disk (Pframeforce.cu:68-92)
-  int nt = 0;
-  s.tx[0] = 0; s.ty[0] = 0; s.tsm2[0] = 0; s.trh2[0] = 1e-30; s.tk[0] = 0;
-  nt = 1;
   const double tt = s.time - s.time0;
-  for (int k = 0; k < c.nb; ++k) {
-    if (!c.feel_disk[k]) continue;
-    const double m = c.mass[k], x = s.x[k], y = s.y[k];
-    const double r = sqrt(x * x + y * y);
-    const double sm = D.roche ? r * pow(m / 3., 1. / 3.) * c.rochesmoothing
-                              : c.thicknesssmoothing * aspect_ratio_dev(c, r, tt) * pow(r, 1.0 + c.flaringindex);
-    const double rh = pow(m / 3., 1. / 3.) * r + 1e-15;
-    s.tx[nt] = x; s.ty[nt] = y; s.tsm2[nt] = sm * sm; s.trh2[nt] = rh * rh;
-    s.tk[nt] = D.exclude_hill ? 10 : 0;
-    nt++;
-  }
-  s.ntarget = nt;
   // ApplySubKeplerianBoundary values (SideEuler.cu:250-268), FP64
   const double r0 = RT(rmed_d)[rb], rn = RT(rmed_d)[rb + nr - 1];
   const double fac = 1.0 + c.sigmaslope - 2.0 * c.flaringindex;
@@ -414,38 +452,6 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
   s.ev_tout = 2.0 * kPI * pow(rn, 1.5);
   s.ev_drmin = r0 * 1.25;
   s.ev_drmax = rn * 0.84;
-}
-
-// ================================================================== forces
-// ComputeForceKernel (Kernels.cu:804-839) for the substep's targets, summed
-// per ring in FP64 (k = 0 or k = 10 Hill cut): one block per ring
-template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_force_ring(Dev<real> D) {
-  FGX_RING_PROLOGUE
-  __shared__ double redd[FGX_TPB / 64];
-  const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
-  double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
-  const int nt = s.ntarget;
-  for (int q = 0; q < nt; ++q) {
-    const real tx = (real)s.tx[q], ty = (real)s.ty[q], sm2 = (real)s.tsm2[q], irh2 = (real)(1.0 / s.trh2[q]);
-    const int kk = s.tk[q];
-    double fx = 0, fy = 0;
-    for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-      const real cm = surf * (real)AT(D.dens, i, j);
-      const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
-      const real d2 = dx * dx + dy * dy;
-      const real dist2 = d2 + sm2;
-      // dist2^(-3/2) from the reciprocal square root
-      const real rq = rsqrt(dist2);
-      const real inv3 = rq * rq * rq;
-      const real cut = kk ? (real)1 - exp(-d2 * irh2) : (real)1;
-      fx += (double)(cm * dx * inv3 * cut);
-      fy += (double)(cm * dy * inv3 * cut);
-    }
-    fx = block_sum(fx, redd);
-    fy = block_sum(fy, redd);
-    if (threadIdx.x == 0) { out[2 * q] = fx; out[2 * q + 1] = fy; }
-  }
 }
 
 // ================================================================= planets
@@ -1189,6 +1195,7 @@ __global__ void k_begin_dt(Dev<real> D) {
   s.dtemp = 0;
   s.sub_in_dt = 0;
   s.active = 1;
+  set_force_targets(D, D.dc[d], s);
 }
 // end of a substep (SourceEuler.cu:297-298) and the DT loop condition
 // (SourceEuler.cu:230); LostMass accumulation (TransportEuler.cu:71)
@@ -1213,6 +1220,8 @@ __global__ void __launch_bounds__(64) k_end_substep(Dev<real> D) {
     s.active = 0;
     s.dt_steps++;
     s.vpar = D.vpar;   // later swaps of the DT leave this disk's velocities behind
+  } else {
+    set_force_targets(D, c, s);
   }
 }
 // end of a DT with the fused source pass: a disk that stopped taking
