@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <cstdint>
 #include <memory>
 #include <string>
 #include <vector>
@@ -54,7 +55,22 @@ struct BatchBase {
   virtual const HostDisk& host(int d) const = 0;
   virtual int size() const = 0;
   virtual int freeze(int disk, int on) = 0;
+  virtual int checkpoint(int disk, const char* path) = 0;
+  virtual int restore(int disk, const char* path) = 0;
 };
+
+// checkpoint file: header, per-disk state record, ring profiles carried from
+// substep to substep, fields in the engine's representation, FNV-1a 64 sum
+struct CkptHeader {
+  char magic[8];             // "FGXCKPT1"
+  int32_t real_bytes, nrad, nsec, adiabatic, nfields, nrings;
+  uint32_t state_bytes, pad;
+};
+inline uint64_t fnv1a64(const char* p, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  for (size_t k = 0; k < n; ++k) { h ^= (unsigned char)p[k]; h *= 1099511628211ull; }
+  return h;
+}
 
 template <class real>
 struct Batch : BatchBase {
@@ -858,6 +874,102 @@ struct Batch : BatchBase {
     return FGX_OK;
   }
   int profile_reset() override { prof.clear(); return FGX_OK; }
+  // ----------------------------------------------------------- checkpoints
+  // A disk's evolving state in the engine's own representation (vtheta as
+  // its residual, the FP64 planets and clocks of the state record, the ring
+  // profiles that carry over between substeps), so that a restore into a disk
+  // of the same parameters continues bit-identically.  Between DTs every
+  // disk's velocities are in the current buffer pair.
+  std::vector<real*> ckpt_fields() {
+    std::vector<real*> f = {D.dens, D.vrad, D.vtheta};
+    if (D.adiabatic) f.push_back(D.energy);
+    return f;
+  }
+  int checkpoint(int d, const char* path) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    if (!path) return set_error(FGX_EPARAM, "null path");
+    const HostDisk& h = hd[d];
+    const size_t nf = (size_t)(h.nrad + 1) * h.nsec, nrr = (size_t)h.nrad + 1;
+    const std::vector<real*> fields = ckpt_fields();
+    real* rings[2] = {D.csmean, D.visc};
+    CkptHeader hdr{};
+    std::memcpy(hdr.magic, "FGXCKPT1", 8);
+    hdr.real_bytes = (int32_t)sizeof(real); hdr.nrad = h.nrad; hdr.nsec = h.nsec;
+    hdr.adiabatic = D.adiabatic; hdr.nfields = (int32_t)fields.size(); hdr.nrings = 2;
+    hdr.state_bytes = (uint32_t)sizeof(DiskState);
+    std::vector<char> buf(sizeof hdr + sizeof(DiskState) + (2 * nrr + fields.size() * nf) * sizeof(real));
+    char* q = buf.data();
+    std::memcpy(q, &hdr, sizeof hdr); q += sizeof hdr;
+    DiskState st;
+    LCHK(get_state(d, st));
+    std::memcpy(q, &st, sizeof st); q += sizeof st;
+    for (real* r : rings) {
+      HIPCHK(hipMemcpyAsync(q, r + dch[d].ro, nrr * sizeof(real), hipMemcpyDeviceToHost, stream));
+      q += nrr * sizeof(real);
+    }
+    for (real* f : fields) {
+      HIPCHK(hipMemcpyAsync(q, f + dch[d].fo, nf * sizeof(real), hipMemcpyDeviceToHost, stream));
+      q += nf * sizeof(real);
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    const uint64_t sum = fnv1a64(buf.data(), buf.size());
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return set_error(FGX_EIO, "cannot write checkpoint '%s'", path);
+    const bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size() && std::fwrite(&sum, 8, 1, fp) == 1;
+    if (std::fclose(fp) != 0 || !ok) return set_error(FGX_EIO, "short write of checkpoint '%s'", path);
+    return FGX_OK;
+  }
+  int restore(int d, const char* path) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    if (!path) return set_error(FGX_EPARAM, "null path");
+    const HostDisk& h = hd[d];
+    const size_t nf = (size_t)(h.nrad + 1) * h.nsec, nrr = (size_t)h.nrad + 1;
+    const std::vector<real*> fields = ckpt_fields();
+    real* rings[2] = {D.csmean, D.visc};
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return set_error(FGX_EIO, "cannot open checkpoint '%s'", path);
+    std::vector<char> buf;
+    char chunk[1 << 16];
+    for (size_t n; (n = std::fread(chunk, 1, sizeof chunk, fp)) > 0;) buf.insert(buf.end(), chunk, chunk + n);
+    std::fclose(fp);
+    const size_t want = sizeof(CkptHeader) + sizeof(DiskState) + (2 * nrr + fields.size() * nf) * sizeof(real) + 8;
+    CkptHeader hdr{};
+    if (buf.size() >= sizeof hdr) std::memcpy(&hdr, buf.data(), sizeof hdr);
+    if (buf.size() < sizeof hdr || std::memcmp(hdr.magic, "FGXCKPT1", 8) != 0)
+      return set_error(FGX_EIO, "'%s' is not a checkpoint", path);
+    if (hdr.real_bytes != (int32_t)sizeof(real) || hdr.nrad != h.nrad || hdr.nsec != h.nsec ||
+        hdr.adiabatic != D.adiabatic || hdr.nfields != (int32_t)fields.size() || hdr.nrings != 2 ||
+        hdr.state_bytes != (uint32_t)sizeof(DiskState))
+      return set_error(FGX_EPARAM, "checkpoint '%s' (%d x %d, %d-byte fields, adiabatic %d) does not fit disk %d "
+                       "(%d x %d, %d-byte fields, adiabatic %d)", path, hdr.nrad, hdr.nsec, hdr.real_bytes,
+                       hdr.adiabatic, d, h.nrad, h.nsec, (int)sizeof(real), D.adiabatic);
+    if (buf.size() != want) return set_error(FGX_EIO, "checkpoint '%s' has %zu bytes, expected %zu", path, buf.size(), want);
+    uint64_t sum;
+    std::memcpy(&sum, buf.data() + buf.size() - 8, 8);
+    if (sum != fnv1a64(buf.data(), buf.size() - 8)) return set_error(FGX_EIO, "checkpoint '%s' fails its checksum", path);
+    const char* q = buf.data() + sizeof hdr;
+    DiskState st;
+    std::memcpy(&st, q, sizeof st); q += sizeof st;
+    st.vpar = D.vpar;   // the velocities go into this batch's current buffer pair
+    HIPCHK(hipMemcpyAsync(D.st + d, &st, sizeof st, hipMemcpyHostToDevice, stream));
+    for (real* r : rings) {
+      HIPCHK(hipMemcpyAsync(r + dch[d].ro, q, nrr * sizeof(real), hipMemcpyHostToDevice, stream));
+      q += nrr * sizeof(real);
+    }
+    for (real* f : fields) {
+      HIPCHK(hipMemcpyAsync(f + dch[d].fo, q, nf * sizeof(real), hipMemcpyHostToDevice, stream));
+      q += nf * sizeof(real);
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    // fields derived from the state (pressure, and the adiabatic sound speed)
+    D.ignore_active = 1;
+    int r = launch("pressure_init", 2, k_pressure_init<real>, n_ctiles, FGX_TPB, 0, D);
+    if (!r && D.adiabatic) r = launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D);
+    D.ignore_active = 0;
+    LCHK(r);
+    HIPCHK(hipStreamSynchronize(stream));
+    return FGX_OK;
+  }
   int freeze(int d, int on) override {
     if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
     const int v = on ? 1 : 0;
@@ -923,6 +1035,14 @@ int fgx_batch_destroy(fgx_batch* b) { delete b; return FGX_OK; }
 int fgx_batch_freeze(fgx_batch* b, int disk, int frozen) {
   if (!b) return set_error(FGX_EPARAM, "null handle");
   return b->impl->freeze(disk, frozen);
+}
+int fgx_batch_checkpoint(fgx_batch* b, int disk, const char* path) {
+  if (!b) return set_error(FGX_EPARAM, "null handle");
+  return b->impl->checkpoint(disk, path);
+}
+int fgx_batch_restore(fgx_batch* b, int disk, const char* path) {
+  if (!b) return set_error(FGX_EPARAM, "null handle");
+  return b->impl->restore(disk, path);
 }
 int fgx_batch_size(fgx_batch* b, int* n) {
   if (!b || !n) return set_error(FGX_EPARAM, "null argument");

@@ -176,7 +176,7 @@ static int usage(const char* exe) {
 int fgx_run_main(int argc, char** argv) {
   if (argc == 1) return usage(argv[0]);
   std::string parfile;
-  int disable = 0, sloppy = 0, cb = 0, store = 0, restart = 0, override_out = 0;
+  int disable = 0, sloppy = 0, cb = 0, store = 0, restart = 0, override_out = 0, nb_restart = 0;
   float scaling = 1.0f;
   std::string newout;
   for (int i = 1; i < argc; ++i) {
@@ -189,7 +189,11 @@ int fgx_run_main(int argc, char** argv) {
       if (std::strchr(argv[i], 's')) {
         restart = 1;
         if (++i >= argc) return usage(argv[0]);
-        if (std::atoi(argv[i]) < 0) return usage(argv[0]);
+        nb_restart = std::atoi(argv[i]);
+        if (nb_restart < 0) {
+          std::printf("Incorrect restart number\n");
+          return usage(argv[0]);
+        }
       }
       if (std::strchr(argv[i], 'o')) {
         override_out = 1;
@@ -221,14 +225,36 @@ int fgx_run_main(int argc, char** argv) {
   const int precision = (prec && std::strcmp(prec, "64") == 0) ? FGX_F64 : FGX_F32;
   fgx_disk* d = nullptr;
   if ((r = fgx_disk_create(&P, &pl, precision, &d))) { std::fprintf(stderr, "%s\n", fgx_last_error()); return 1; }
-  if ((r = fgx_disk_write_setup_files(d))) { std::fprintf(stderr, "%s\n", fgx_last_error()); fgx_disk_destroy(d); return 1; }
+  // -s n: the state at output n, written by an earlier run with CHECKPOINT
+  // YES; the loop resumes at that output's iteration, whose log lines that run
+  // has already written (the reference parses -s but restores nothing,
+  // Main.cu:147-155, 175-180)
+  int first = 0;
+  if (restart) {
+    const std::string ck = std::string(P.outputdir) + "restart" + std::to_string(nb_restart) + ".bin";
+    if ((r = fgx_batch_restore(d, 0, ck.c_str()))) {
+      std::fprintf(stderr, "%s\n", fgx_last_error());
+      fgx_disk_destroy(d);
+      return 1;
+    }
+    first = nb_restart * P.ninterm;
+    std::printf("Restarting from output %d (iteration %d)\n", nb_restart, first);
+  } else if ((r = fgx_disk_write_setup_files(d))) {
+    std::fprintf(stderr, "%s\n", fgx_last_error());
+    fgx_disk_destroy(d);
+    return 1;
+  }
   auto t0 = std::chrono::steady_clock::now();
-  int timestep = 0;
-  for (int i = 0; i <= P.ntot; ++i) {
-    if ((r = fgx_disk_write_logs(d, timestep))) break;
+  int timestep = restart ? nb_restart : 0;
+  for (int i = first; i <= P.ntot; ++i) {
+    if (!(restart && i == first) && (r = fgx_disk_write_logs(d, timestep))) break;
     if (P.ninterm * (timestep = (i / P.ninterm)) == i) {
       std::printf("\n*** OUTPUT %d ***\n", timestep);
       if ((r = fgx_disk_write_outputs(d, timestep))) break;
+      if (P.ext_checkpoint) {
+        const std::string ck = std::string(P.outputdir) + "restart" + std::to_string(timestep) + ".bin";
+        if ((r = fgx_batch_checkpoint(d, 0, ck.c_str()))) break;
+      }
     }
     if ((r = fgx_disk_step(d, 1))) break;
   }

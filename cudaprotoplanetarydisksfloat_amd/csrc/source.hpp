@@ -111,14 +111,29 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
   }
   // past the outer edge A and B are skipped and C yields VradInt(nrad) = 0
   const int ke = i1 + 2;
+  // stage A's inputs are fetched one ring ahead, so the wave does not wait
+  // for them at the top of every iteration
+  real nsg = 0, nvr = 0, nvt = 0, nvtp = 0, ne = 0;
+  auto fetch = [&](int k) {
+    if (k < nr) {
+      nsg = AT(D.dens, k, j);
+      nvr = AT(D.vrad, k, j);
+      nvt = AT(D.vtheta, k, j);
+      nvtp = AT(D.vtheta, k, jp);
+      if (ad) ne = AT(D.energy, k, j);
+    }
+  };
+  fetch(ks);
   for (int k = ks; k <= ke; ++k) {
     // ---------------------------------------------------------- stage A
     real sg0 = 0, e0 = 0, vri0 = 0, vti0 = 0;
+    const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne;
+    fetch(k + 1);
     if (k < nr) {
-      sg0 = AT(D.dens, k, j);
-      const real vr = AT(D.vrad, k, j);
-      const real vt = AT(D.vtheta, k, j), vtp = AT(D.vtheta, k, jp);
-      if (ad) e0 = AT(D.energy, k, j);
+      sg0 = csg;
+      const real vr = cvr;
+      const real vt = cvt, vtp = cvtp;
+      if (ad) e0 = ce;
       const double p = pressure(sg0, e0, k);
       const double phi = potential_planets_at(D, c, s, FGX_CD(1, k), j);
       if (D.write_pressure && out && k >= i0 && k < i1) AT(D.pressure, k, j) = (real)p;

@@ -119,6 +119,7 @@ _sigs = {
     "fgx_profile_reset": ([_vp], _i),
     "fgx_run_main": ([_i, ctypes.POINTER(_cp)], _i),
     "fgx_batch_freeze": ([_vp, _i, _i], _i),
+    "fgx_batch_checkpoint": ([_vp, _i, _cp], _i), "fgx_batch_restore": ([_vp, _i, _cp], _i),
     "fgx_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)], _i),
     "fgx_server_submit": ([_vp, _vp, ctypes.POINTER(Planets), _i, _d, ctypes.POINTER(_i)], _i),
     "fgx_server_run": ([_vp], _i),
@@ -252,6 +253,13 @@ class Batch:
     def set_field(self, name, values, disk=0):
         v = np.ascontiguousarray(values, dtype=np.float64).ravel()
         _check(lib.fgx_batch_write_field(self.h, disk, FIELDS[name], F64, v.ctypes.data, v.size))
+
+    def checkpoint(self, path, disk=0):
+        """Complete state of a disk, restorable bit-identically (fgx_batch_checkpoint)."""
+        _check(lib.fgx_batch_checkpoint(self.h, disk, os.fsencode(path)))
+
+    def restore(self, path, disk=0):
+        _check(lib.fgx_batch_restore(self.h, disk, os.fsencode(path)))
 
     # single-disk conveniences (disk 0)
     def ring(self, name):

@@ -88,6 +88,7 @@ typedef struct fgx_params {
   int   ext_fix_sg_wrap;       /* QUIRKFIXSGWRAP: periodic j-1 in SG update  */
   float ext_sg_softening;      /* SGSOFTENING: >0 overrides SG epsilon/H     */
   int   ext_gas_names;         /* GASNAMES: 0 dens/ layout, 1 gasdens*.dat, 2 both */
+  int   ext_checkpoint;        /* CHECKPOINT: fgx-run writes restart<N>.bin at each output */
 } fgx_params;
 
 /* ---- planetary system: the reference's PlanetarySystem (include/Types.cuh:2-15) */
@@ -211,6 +212,15 @@ int fgx_batch_write_field(fgx_batch* b, int disk, int field, int dtype,
  * leave a frozen disk untouched (its tiles are skipped).  No counterpart in
  * the reference; the serving loop retires finished requests with it. */
 int fgx_batch_freeze(fgx_batch* b, int disk, int frozen);
+/* Complete evolving state of disk `disk` (fields in the engine's precision,
+ * vtheta as its residual, planets, clocks, frame, ring profiles carried between
+ * substeps) to `path`, between DTs; fgx_batch_restore loads it into a disk of
+ * the same parameters, which then continues bit-identically.  The reference
+ * parses a restart switch but never restores anything (-s n, src/Main.cu:147-155,
+ * 175-180); fgx-run -s n restores from OUTPUTDIR/restart<n>.bin, written at each
+ * output when CHECKPOINT is YES. */
+int fgx_batch_checkpoint(fgx_batch* b, int disk, const char* path);
+int fgx_batch_restore(fgx_batch* b, int disk, const char* path);
 /* bytes of device memory a disk of these parameters needs */
 int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes);
 

@@ -204,3 +204,14 @@ def test_serving_workload_sampler():
                     n //= 2 if n % 2 == 0 else (3 if n % 3 == 0 else 5)
                 assert n == 1
     assert 20 <= nsg <= 60
+
+
+def test_checkpoint_key():
+    """CHECKPOINT (unknown to the reference, so .par files stay interchangeable)
+    turns on restart<N>.bin at each output of fgx-run."""
+    from cudaprotoplanetarydisksfloat_amd import configs
+    par, _ = configs.plain_disk(nrad=16, nsec=32)
+    assert int(Params(text=par).get("CHECKPOINT")) == 0
+    assert int(Params(text=par + "Checkpoint YES\n").get("CHECKPOINT")) == 1
+    assert int(Params(text=par + "Checkpoint 1\n").get("CHECKPOINT")) == 1
+    assert int(Params(text=par + "Checkpoint NO\n").get("CHECKPOINT")) == 0
