@@ -609,8 +609,7 @@ struct Batch : BatchBase {
   int substep() {
     const int ad = D.adiabatic;
     LCHK(launch("cfl_force", 3, k_cfl_force<real>, n_rtiles, 64, 0, D));
-    LCHK(launch("substep_begin", 0, k_substep_begin<real>, nd, FGX_TPB, 0, D));
-    LCHK(launch("planets", 0, k_planets<real>, nd, FGX_TPB, 0, D));
+    LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D));
     if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_stiles, FGX_TPB, 0, D));
     if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
     LCHK(boundary());

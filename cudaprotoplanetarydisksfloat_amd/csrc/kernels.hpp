@@ -398,13 +398,10 @@ template <class real>
 __device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, DiskState& s, int action);
 
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
-  const int d = blockIdx.x;
+__device__ void substep_begin_block(const Dev<real>& D, int d, real* redr) {
   const DiskConst& c = D.dc[d];
   DiskState& s = D.st[d];
-  if (!s.active) return;
   const int nr = c.nrad, rb = c.ro;
-  __shared__ real redr[FGX_TPB / 64];
   const bool need_cfl = !(c.substeps_forced > 0) && !(D.sloppy && s.sub_in_dt > 0);
   real newdt = (real)1e30;
   if (need_cfl) {
@@ -420,7 +417,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_begin(Dev<real> D) {
     }
     newdt = block_min(mn, redr);
   }
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0) return;   // the rest is serial
   // MassTaper (SourceEuler.cu:231-232)
   double mt = s.time / (c.masstaper * 2.0 * M_PI);
   s.masstaper = (mt > 1.0) ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0);
@@ -490,9 +487,9 @@ __device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, Disk
 
 // DerivMotionRK5 (RungeKutta.cu:9-46)
 template <class real>
-__device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const double* q, double* der, int nb, double dt) {
+__device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const double* q, double* der, double* dist, int nb,
+                              double dt) {
   const double *x = q, *y = q + nb, *vx = q + 2 * nb, *vy = q + 3 * nb;
-  double dist[FGX_MAX_PLANETS];
   for (int i = 0; i < nb; ++i) dist[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
   for (int i = 0; i < nb; ++i) {
     der[i] = vx[i];
@@ -524,13 +521,9 @@ __device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const doub
 // parameters, AdvanceSystemFromDisk, AdvanceSystemRK5 (Cash-Karp,
 // RungeKutta.cu:55-74), corotation and RotatePsys: one block per disk
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_planets(Dev<real> D) {
-  const int d = blockIdx.x;
+__device__ void planets_block(const Dev<real>& D, int d, double* redd, double* tot, double* rk) {
   const DiskConst& c = D.dc[d];
   DiskState& s = D.st[d];
-  if (!s.active) return;
-  __shared__ double redd[FGX_TPB / 64];
-  __shared__ double tot[2 * FGX_NFORCE_T];
   const int nt = s.ntarget;
   for (int q = 0; q < 2 * nt; ++q) {
     double a = 0;
@@ -567,28 +560,31 @@ __global__ void __launch_bounds__(FGX_TPB) k_planets(Dev<real> D) {
   }
   // AdvanceSystemRK5 (Pframeforce.cu:94-157)
   if (nb > 0) {
-    double q0[4 * FGX_MAX_PLANETS], q1[4 * FGX_MAX_PLANETS];
+    // the Cash-Karp stages in LDS (rk: 8 x 4*FGX_MAX_PLANETS doubles plus
+    // FGX_MAX_PLANETS distances) rather than in per-thread scratch memory
+    const int S = 4 * FGX_MAX_PLANETS;
+    double *q0 = rk, *q1 = rk + S;
     if (!D.forced_circular) {
-      double k1[4 * FGX_MAX_PLANETS], k2[4 * FGX_MAX_PLANETS], k3[4 * FGX_MAX_PLANETS],
-          k4[4 * FGX_MAX_PLANETS], k5[4 * FGX_MAX_PLANETS], k6[4 * FGX_MAX_PLANETS];
+      double *k1 = rk + 2 * S, *k2 = rk + 3 * S, *k3 = rk + 4 * S, *k4 = rk + 5 * S, *k5 = rk + 6 * S, *k6 = rk + 7 * S;
+      double* dist = rk + 8 * S;
       for (int k = 0; k < nb; ++k) {
         q0[k] = s.x[k]; q0[k + nb] = s.y[k]; q0[k + 2 * nb] = s.vx[k]; q0[k + 3 * nb] = s.vy[k];
       }
       const int n4 = 4 * nb;
-      deriv_rk5_dev(D, c, q0, k1, nb, dt);
+      deriv_rk5_dev(D, c, q0, k1, dist, nb, dt);
       for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.2 * k1[m];
-      deriv_rk5_dev(D, c, q1, k2, nb, dt);
+      deriv_rk5_dev(D, c, q1, k2, dist, nb, dt);
       for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.075 * k1[m] + 0.225 * k2[m];
-      deriv_rk5_dev(D, c, q1, k3, nb, dt);
+      deriv_rk5_dev(D, c, q1, k3, dist, nb, dt);
       for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.3 * k1[m] - 0.9 * k2[m] + 1.2 * k3[m];
-      deriv_rk5_dev(D, c, q1, k4, nb, dt);
+      deriv_rk5_dev(D, c, q1, k4, dist, nb, dt);
       for (int m = 0; m < n4; ++m)
         q1[m] = q0[m] - 11.0 / 54.0 * k1[m] + 2.5 * k2[m] - 70.0 / 27.0 * k3[m] + 35.0 / 27.0 * k4[m];
-      deriv_rk5_dev(D, c, q1, k5, nb, dt);
+      deriv_rk5_dev(D, c, q1, k5, dist, nb, dt);
       for (int m = 0; m < n4; ++m)
         q1[m] = q0[m] + 1631.0 / 55296.0 * k1[m] + 175.0 / 512.0 * k2[m] + 575.0 / 13824.0 * k3[m] +
                 44275.0 / 110592.0 * k4[m] + 253.0 / 4096.0 * k5[m];
-      deriv_rk5_dev(D, c, q1, k6, nb, dt);
+      deriv_rk5_dev(D, c, q1, k6, dist, nb, dt);
       for (int m = 0; m < n4; ++m)
         q1[m] = q0[m] + 37.0 / 378.0 * k1[m] + 250.0 / 621.0 * k3[m] + 125.0 / 594.0 * k4[m] + 512.0 / 1771.0 * k6[m];
     }
@@ -632,6 +628,23 @@ __global__ void __launch_bounds__(FGX_TPB) k_planets(Dev<real> D) {
     xt = s.vx[k]; yt = s.vy[k];
     s.vx[k] = xt * ca + yt * sa; s.vy[k] = -xt * sa + yt * ca;
   }
+}
+
+// the per-disk serial part of a substep's start: the CFL substep count and dt,
+// MassTaper, boundary constants (substep_begin_block), then the disk forces
+// summed over rings, the potential's planet parameters, AdvanceSystemFromDisk,
+// the RK5 N-body step and the frame (planets_block): one block per disk
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_substep_setup(Dev<real> D) {
+  const int d = blockIdx.x;
+  if (!D.st[d].active) return;
+  __shared__ real redr[FGX_TPB / 64];
+  __shared__ double redd[FGX_TPB / 64];
+  __shared__ double tot[2 * FGX_NFORCE_T];
+  __shared__ double rk[8 * 4 * FGX_MAX_PLANETS + FGX_MAX_PLANETS];
+  substep_begin_block(D, d, redr);
+  __syncthreads();
+  planets_block(D, d, redd, tot, rk);
 }
 
 // CorrectVthetaKernel (Kernels.cu:1352-1359)
