@@ -296,7 +296,7 @@ struct Batch : BatchBase {
       theta_split<real>(h.nsec, K, W);
       for (int i = 0; i < h.nrad; ++i) tg[K * 17 + W].push_back(RingTile{d, i});
     }
-    while (st.size() % 8) st.push_back(CellTile{-1, 0, 0, 0});
+    while (st.size() % FGX_TILE_PAD) st.push_back(CellTile{-1, 0, 0, 0});
     n_stiles = (int)st.size();
     // fused source pass: on by default, FGX_FUSED_SOURCE=0 keeps the
     // separate kernels; FGX_SOURCE_RINGS sets the strip height
@@ -307,10 +307,10 @@ struct Batch : BatchBase {
     for (int d = 0; d < nd; ++d)
       for (int i0 = 0; i0 < hd[d].nrad; i0 += src_rings)
         for (int jb = 0; jb * FGX_SRC_OUT < hd[d].nsec; ++jb) srt.push_back(CellTile{d, i0, jb, 0});
-    while (srt.size() % 8) srt.push_back(CellTile{-1, 0, 0, 0});
+    while (srt.size() % FGX_TILE_PAD) srt.push_back(CellTile{-1, 0, 0, 0});
     n_srctiles = (int)srt.size();
-    while (ct.size() % 8) ct.push_back(CellTile{-1, 0, 0, 0});
-    while (rt.size() % 8) rt.push_back(RingTile{-1, 0});
+    while (ct.size() % FGX_TILE_PAD) ct.push_back(CellTile{-1, 0, 0, 0});
+    while (rt.size() % FGX_TILE_PAD) rt.push_back(RingTile{-1, 0});
     n_ctiles = (int)ct.size();
     n_rtiles = (int)rt.size();
     // device arrays
@@ -326,7 +326,7 @@ struct Batch : BatchBase {
     LCHK(f(&D.sg_accr, sg)); LCHK(f(&D.sg_acct, sg));
     LCHK(dalloc(&D.phi, F));
     if (sg) {
-      while (sgct.size() % 8) sgct.push_back(RingTile{-1, 0});
+      while (sgct.size() % FGX_TILE_PAD) sgct.push_back(RingTile{-1, 0});
       n_sgctiles = (int)sgct.size();
       cxd *w, *kr, *kt, *tw;
       RingTile* sgt;
@@ -389,7 +389,7 @@ struct Batch : BatchBase {
       for (int kw = 0; kw < 17 * 17; ++kw) {
         std::vector<RingTile>& v = tg[kw];
         if (v.empty()) continue;
-        while (v.size() % 8) v.push_back(RingTile{-1, 0});
+        while (v.size() % FGX_TILE_PAD) v.push_back(RingTile{-1, 0});
         RingTile* g;
         LCHK(dalloc(&g, v.size()));
         HIPCHK(hipMemcpyAsync((void*)g, v.data(), v.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));

@@ -108,10 +108,19 @@ __device__ __forceinline__ real azim_gradp(double p, double pl, real sig, real s
   if constexpr (sizeof(real) == 4) return (real)(p - pl) * (real)2 * __builtin_amdgcn_rcpf(sig + sigl);
   else return (real)((p - pl) * 2.0 / ((double)sig + (double)sigl));
 }
+// block -> tile: the hardware hands blocks to the 8 XCDs round-robin (block
+// b to XCD b & 7).  Groups of FGX_XCD_GROUP consecutive tiles (neighbouring
+// strips of a disk, which share halo rings in L2) stay on one XCD, and the
+// groups are dealt to the XCDs in turn, so the tiles of any one disk spread
+// over the whole chip (when only a few disks are still active, a contiguous
+// share per XCD would leave seven of the eight idle).  n is a multiple of
+// 8 * FGX_XCD_GROUP.
+#define FGX_XCD_GROUP 8
+#define FGX_TILE_PAD (8 * FGX_XCD_GROUP)
 __device__ __forceinline__ int xcd_tile(int b, int n) {
-  // n is a multiple of 8; hardware hands blocks to the 8 XCDs round-robin
-  const int per = n >> 3;
-  return (b & 7) * per + (b >> 3);
+  (void)n;
+  const int x = b & 7, slot = b >> 3;
+  return ((slot / FGX_XCD_GROUP) * 8 + x) * FGX_XCD_GROUP + slot % FGX_XCD_GROUP;
 }
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
