@@ -89,6 +89,25 @@ struct Batch : BatchBase {
   int n_srctiles = 0, src_rings = 12;
   bool fused_source = true;
   const CellTile* srctiles = nullptr;
+  // two tilings of the strip kernels: the coarse one above for substeps in
+  // which most of the batch is active, and a fine one (short strips, many
+  // more blocks) for the latency-bound substeps in which only a few disks
+  // still run; the host picks one per substep from the summary's forecast of
+  // active cells.  Both give the same fields bit for bit.
+  struct TileSet {
+    const CellTile *st = nullptr, *src = nullptr;
+    int n_st = 0, strip_rings = 0, n_src = 0, src_rings = 0;
+  } coarse, fine;
+  double fine_fraction = 0.3;    // fine tiling below this share of active cells
+  long long total_cells = 0;
+  std::vector<char> fine_plan;   // tiling of the upcoming substeps
+  int plan_pos = 0;
+  void use_tiles(bool f) {
+    const TileSet& t = f ? fine : coarse;
+    stiles = t.st; n_stiles = t.n_st; strip_rings = t.strip_rings;
+    srctiles = t.src; n_srctiles = t.n_src; src_rings = t.src_rings;
+    D.stiles = t.st; D.strip_rings = t.strip_rings;
+  }
   size_t sg_shm_row = 0, sg_shm_col = 0;
   const CellTile* stiles = nullptr;
   struct ThetaGroup { int k, w, n; const RingTile* tiles; };
@@ -211,7 +230,7 @@ struct Batch : BatchBase {
       if (!p.is_disk) return set_error(FGX_EPARAM, "DISK NO (planets only) is not supported");
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipHostMalloc((void**)&h_summary, 4 * sizeof(int)));
+    HIPCHK(hipHostMalloc((void**)&h_summary, (2 + FGX_SUMMARY_HIST) * sizeof(int)));
     hd.resize(nd);
     for (int d = 0; d < nd; ++d) LCHK(setup_grid_and_density(ps[d], pls[d], hd[d]));
     // self-gravity set-up (FP64, host): kernel transforms, the initial radial
@@ -309,6 +328,19 @@ struct Batch : BatchBase {
         for (int jb = 0; jb * FGX_SRC_OUT < hd[d].nsec; ++jb) srt.push_back(CellTile{d, i0, jb, 0});
     while (srt.size() % FGX_TILE_PAD) srt.push_back(CellTile{-1, 0, 0, 0});
     n_srctiles = (int)srt.size();
+    // the fine tiling: 2-ring strips, 4-ring source strips
+    const int fine_strip = std::min(2, strip_rings), fine_src = std::min(4, src_rings);
+    std::vector<CellTile> stf, srtf;
+    for (int d = 0; d < nd; ++d) {
+      total_cells += (long long)hd[d].nrad * hd[d].nsec;
+      for (int i0 = 0; i0 < hd[d].nrad; i0 += fine_strip)
+        for (int jb = 0; jb * FGX_TPB < hd[d].nsec; ++jb) stf.push_back(CellTile{d, i0, jb, 0});
+      for (int i0 = 0; i0 < hd[d].nrad; i0 += fine_src)
+        for (int jb = 0; jb * FGX_SRC_OUT < hd[d].nsec; ++jb) srtf.push_back(CellTile{d, i0, jb, 0});
+    }
+    while (stf.size() % FGX_TILE_PAD) stf.push_back(CellTile{-1, 0, 0, 0});
+    while (srtf.size() % FGX_TILE_PAD) srtf.push_back(CellTile{-1, 0, 0, 0});
+    if (const char* e = std::getenv("FGX_FINE_FRACTION")) fine_fraction = std::atof(e);
     while (ct.size() % FGX_TILE_PAD) ct.push_back(CellTile{-1, 0, 0, 0});
     while (rt.size() % FGX_TILE_PAD) rt.push_back(RingTile{-1, 0});
     n_ctiles = (int)ct.size();
@@ -371,7 +403,7 @@ struct Batch : BatchBase {
     D.dc = dcd; D.st = std_;
     LCHK(dalloc(&D.fpart, (size_t)n_rtiles * 2 * FGX_NFORCE_T));
     LCHK(dalloc(&D.rpart, (size_t)std::max(max_nrad, 1) * 44));
-    LCHK(dalloc(&D.summary, 4));
+    LCHK(dalloc(&D.summary, 2 + FGX_SUMMARY_HIST));
     CellTile* ctd; RingTile* rtd;
     LCHK(dalloc(&ctd, ct.size())); LCHK(dalloc(&rtd, rt.size()));
     D.ctiles = ctd; D.rtiles = rtd;
@@ -386,6 +418,13 @@ struct Batch : BatchBase {
       LCHK(dalloc(&srt_d, srt.size()));
       HIPCHK(hipMemcpyAsync((void*)srt_d, srt.data(), srt.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
       srctiles = srt_d;
+      coarse = TileSet{std_t, srt_d, n_stiles, strip_rings, n_srctiles, src_rings};
+      CellTile *stf_d, *srtf_d;
+      LCHK(dalloc(&stf_d, stf.size()));
+      LCHK(dalloc(&srtf_d, srtf.size()));
+      HIPCHK(hipMemcpyAsync((void*)stf_d, stf.data(), stf.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync((void*)srtf_d, srtf.data(), srtf.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
+      fine = TileSet{stf_d, srtf_d, (int)stf.size(), fine_strip, (int)srtf.size(), fine_src};
       for (int kw = 0; kw < 17 * 17; ++kw) {
         std::vector<RingTile>& v = tg[kw];
         if (v.empty()) continue;
@@ -608,6 +647,9 @@ struct Batch : BatchBase {
   }
   int substep() {
     const int ad = D.adiabatic;
+    // the tiling the last summary's forecast chose for this substep
+    use_tiles(plan_pos < (int)fine_plan.size() && fine_plan[plan_pos]);
+    ++plan_pos;
     LCHK(launch("cfl_force", 3, k_cfl_force<real>, n_rtiles, 64, 0, D));
     LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D));
     if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_stiles, FGX_TPB, 0, D));
@@ -649,6 +691,7 @@ struct Batch : BatchBase {
     LCHK(launch("trans_vr", 4, k_trans_vr<real>, n_stiles, FGX_TPB, 0, D));
     LCHK(boundary());
     LCHK(launch("end_substep", 0, k_end_substep<real>, nd, 64, 0, D));
+    use_tiles(false);
     return FGX_OK;
   }
   int step(int n_dt) override {
@@ -656,16 +699,27 @@ struct Batch : BatchBase {
       LCHK(launch("begin_dt", 0, k_begin_dt<real>, (nd + 63) / 64, 64, 0, D));
       if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
       int plan = 0;
+      fine_plan.clear();
+      plan_pos = 0;
       for (int guard = 0;; ++guard) {
         if (guard > 10000000) return set_error(FGX_ENUMERIC, "substep loop did not terminate");
         LCHK(substep());
         if (plan > 1) { --plan; continue; }
         LCHK(launch("summary", 0, k_summary<real>, 1, FGX_TPB, 0, D));
-        HIPCHK(hipMemcpyAsync(h_summary, D.summary, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(h_summary, D.summary, (2 + FGX_SUMMARY_HIST) * sizeof(int), hipMemcpyDeviceToHost,
+                              stream));
         HIPCHK(hipStreamSynchronize(stream));
         if (prof_on) prof_collect();
         if (h_summary[0] == 0) break;
         plan = std::max(1, h_summary[1]);
+        // forecast: cells of the disks still active j substeps ahead (units
+        // of 64 cells); short strips where that is a small share of the batch
+        fine_plan.assign(plan, 0);
+        plan_pos = 0;
+        for (int j = 0; j < plan; ++j) {
+          const long long act = 64LL * h_summary[2 + std::min(j, FGX_SUMMARY_HIST - 1)];
+          fine_plan[j] = act < fine_fraction * (double)total_cells;
+        }
       }
       // disks that finished the DT before the batch did
       if (fused_source && !D.self_gravity) LCHK(launch("vel_fixup", 0, k_vel_fixup<real>, n_stiles, FGX_TPB, 0, D));

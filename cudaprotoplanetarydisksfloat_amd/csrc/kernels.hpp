@@ -15,6 +15,7 @@
 #include "fgx_internal.h"
 
 #define FGX_TPB 256
+#define FGX_SUMMARY_HIST 64   // substeps ahead the active-cell forecast covers
 #define FGX_THETA_ARRAYS 8
 
 namespace fgx {
@@ -1270,15 +1271,23 @@ __global__ void __launch_bounds__(FGX_TPB) k_vel_fixup(Dev<real> D) {
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
   __shared__ int red[FGX_TPB / 64];
+  __shared__ int hist[FGX_SUMMARY_HIST];
+  for (int j = threadIdx.x; j < FGX_SUMMARY_HIST; j += FGX_TPB) hist[j] = 0;
+  __syncthreads();
   int a = 0, m = 0;
   for (int d = threadIdx.x; d < D.ndisk; d += FGX_TPB) {
     a += D.st[d].active;
     const int rem = D.st[d].active ? D.st[d].rem_est : 0;
     m = rem > m ? rem : m;
+    // forecast of the cells still active j substeps ahead, in units of 64
+    const int c64 = (D.dc[d].nrad * D.dc[d].nsec + 63) / 64;
+    for (int j = 0; j < rem && j < FGX_SUMMARY_HIST; ++j) atomicAdd(&hist[j], c64);
   }
   a = block_sum(a, red);
   m = -block_min(-m, red);
   if (threadIdx.x == 0) { D.summary[0] = a; D.summary[1] = m; }
+  __syncthreads();
+  for (int j = threadIdx.x; j < FGX_SUMMARY_HIST; j += FGX_TPB) D.summary[2 + j] = hist[j];
 }
 
 // ============================================================== diagnostics

@@ -202,3 +202,23 @@ def test_fused_source_equals_separate_kernels(gpu, monkeypatch, case, precision)
             continue
         a, b = out["1"][n], out["0"][n]
         assert np.array_equal(a, b), (case, n, float(np.abs(a - b).max()))
+
+
+@pytest.mark.parametrize("case", ["plain", "adiabatic_evanescent", "open"])
+def test_fine_tiling_equals_coarse(gpu, monkeypatch, case):
+    """The short-strip tiling used when few disks are still active gives the
+    same fields, bit for bit, as the default tiling (every substep forced one
+    way or the other)."""
+    par, cfg, keys = _case(case)
+    keys = dict(keys)
+    keys.setdefault("NRAD", 66)
+    p, pl = configs.load(par, cfg, DT=DT_SMALL, **keys)
+    out = {}
+    for frac in ("0", "2"):
+        monkeypatch.setenv("FGX_FINE_FRACTION", frac)
+        disk = Batch(p, pl, F32)
+        disk.step(3)
+        names = ("dens", "vrad", "vtheta") + (("energy",) if int(p.get("ADIABATIC_F")) else ())
+        out[frac] = {n: disk.field(n) for n in names}
+    for n in out["0"]:
+        assert np.array_equal(out["0"][n], out["2"][n]), (case, n)
