@@ -321,6 +321,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D) {
       const real tx = (real)s.tx[q], ty = (real)s.ty[q], sm2 = (real)s.tsm2[q], irh2 = (real)(1.0 / s.trh2[q]);
       const int kk = s.tk[q];
       double fx = 0, fy = 0;
+#pragma unroll 4
       for (int j = lane; j < ns; j += 64) {
         const real cm = surf * (real)AT(D.dens, i, j);
         const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
@@ -341,6 +342,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D) {
   if (c.substeps_forced > 0 || (D.sloppy && s.sub_in_dt > 0)) return;
   const double vr0 = RT(vref)[rb + i];
   double acc = 0;
+#pragma unroll 4
   for (int j = lane; j < ns; j += 64) acc += (double)(real)AT(D.vtheta, i, j);
   const double mres = wave_sum(acc) / (double)ns;   // ring mean of the residual
   if (lane == 0) D.vmoy[rb + i] = vr0 + mres;
@@ -354,6 +356,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D) {
   const real imind = (real)1 / mind, idxr = (real)1 / dxrad, idxt = (real)1 / dxt;
   const real csr = D.adiabatic ? (real)0 : D.cs_ring[rb + i];
   real mn = (real)1e30;
+#pragma unroll 4
   for (int j = lane; j < ns; j += 64) {
     const real rs = AT(D.vtheta, i, j);
     const real vr = AT(D.vrad, i, j);
