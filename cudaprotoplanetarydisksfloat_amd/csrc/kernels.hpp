@@ -1203,9 +1203,28 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
 // radial half of ComputeVelocitiesKernel (Kernels.cu:1299-1302)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_trans_vr(Dev<real> D) {
-  FGX_STRIP_BEGIN {
-    if (i == 0) { AT(D.vrad, i, j) = 0; continue; }
-    AT(D.vrad, i, j) = (AT(D.t_rmp, i - 1, j) + AT(D.t_rmm, i, j)) / (AT(D.dens, i, j) + AT(D.dens, i - 1, j) + (real)1e-20);
+  // vr(i) = (RadMomP(i-1) + RadMomM(i)) / (Sigma(i) + Sigma(i-1) + 1e-20),
+  // vr(0) = 0: ring i-1's values are carried in registers and ring i+1's are
+  // fetched before ring i's store
+  const CellTile t = D.stiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  if (t.d < 0) return;
+  const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
+  const FGX_AS4 DiskState& s = cref(D.st + t.d);
+  if (!D.ignore_active && !s.active) return;
+  const int ns = c.nsec, nr = c.nrad;
+  const int j = t.jb * FGX_TPB + (int)threadIdx.x;
+  if (j >= ns) return;
+  const long long fb = c.fo;
+  const int i0 = t.i, i_end = (t.i + D.strip_rings < nr) ? t.i + D.strip_rings : nr;
+  real pm = 0, dm = 0;   // RadMomP and Sigma of ring i-1
+  if (i0 > 0) { pm = AT(D.t_rmp, i0 - 1, j); dm = AT(D.dens, i0 - 1, j); }
+  real nm = AT(D.t_rmm, i0, j), nd = AT(D.dens, i0, j), np = AT(D.t_rmp, i0, j);
+  for (int i = i0; i < i_end; ++i) {
+    const real rmm = nm, den = nd, rmp = np;
+    if (i + 1 < i_end) { nm = AT(D.t_rmm, i + 1, j); nd = AT(D.dens, i + 1, j); np = AT(D.t_rmp, i + 1, j); }
+    AT(D.vrad, i, j) = (i == 0) ? (real)0 : (pm + rmm) / (den + dm + (real)1e-20);
+    pm = rmp;
+    dm = den;
   }
 }
 
