@@ -98,7 +98,9 @@ struct Batch : BatchBase {
     const CellTile *st = nullptr, *src = nullptr;
     int n_st = 0, strip_rings = 0, n_src = 0, src_rings = 0;
   } coarse, fine;
-  double fine_fraction = 0.3;    // fine tiling below this share of active cells
+  // fine tiling below this many active cells: coarse strips cannot fill the
+  // chip with less (256 CUs x 4 SIMDs x a few waves of 64 x 12-16 rings)
+  double fine_cells = 2.0e6;
   long long total_cells = 0;
   std::vector<char> fine_plan;   // tiling of the upcoming substeps
   int plan_pos = 0;
@@ -340,7 +342,7 @@ struct Batch : BatchBase {
     }
     while (stf.size() % FGX_TILE_PAD) stf.push_back(CellTile{-1, 0, 0, 0});
     while (srtf.size() % FGX_TILE_PAD) srtf.push_back(CellTile{-1, 0, 0, 0});
-    if (const char* e = std::getenv("FGX_FINE_FRACTION")) fine_fraction = std::atof(e);
+    if (const char* e = std::getenv("FGX_FINE_CELLS")) fine_cells = std::atof(e);
     while (ct.size() % FGX_TILE_PAD) ct.push_back(CellTile{-1, 0, 0, 0});
     while (rt.size() % FGX_TILE_PAD) rt.push_back(RingTile{-1, 0});
     n_ctiles = (int)ct.size();
@@ -699,7 +701,8 @@ struct Batch : BatchBase {
       LCHK(launch("begin_dt", 0, k_begin_dt<real>, (nd + 63) / 64, 64, 0, D));
       if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
       int plan = 0;
-      fine_plan.clear();
+      // the first substeps of the DT run every disk that is not frozen
+      fine_plan.assign(1, (double)total_cells < fine_cells);
       plan_pos = 0;
       for (int guard = 0;; ++guard) {
         if (guard > 10000000) return set_error(FGX_ENUMERIC, "substep loop did not terminate");
@@ -718,7 +721,7 @@ struct Batch : BatchBase {
         plan_pos = 0;
         for (int j = 0; j < plan; ++j) {
           const long long act = 64LL * h_summary[2 + std::min(j, FGX_SUMMARY_HIST - 1)];
-          fine_plan[j] = act < fine_fraction * (double)total_cells;
+          fine_plan[j] = (double)act < fine_cells;
         }
       }
       // disks that finished the DT before the batch did

@@ -214,11 +214,11 @@ def test_fine_tiling_equals_coarse(gpu, monkeypatch, case):
     keys.setdefault("NRAD", 66)
     p, pl = configs.load(par, cfg, DT=DT_SMALL, **keys)
     out = {}
-    for frac in ("0", "2"):
-        monkeypatch.setenv("FGX_FINE_FRACTION", frac)
+    for frac in ("0", "1e12"):
+        monkeypatch.setenv("FGX_FINE_CELLS", frac)
         disk = Batch(p, pl, F32)
         disk.step(3)
         names = ("dens", "vrad", "vtheta") + (("energy",) if int(p.get("ADIABATIC_F")) else ())
         out[frac] = {n: disk.field(n) for n in names}
     for n in out["0"]:
-        assert np.array_equal(out["0"][n], out["2"][n]), (case, n)
+        assert np.array_equal(out["0"][n], out["1e12"][n]), (case, n)
