@@ -117,6 +117,8 @@ struct Batch : BatchBase {
   size_t F = 0, R = 0, S = 0;
   int* h_summary = nullptr;  // pinned
   int visc_dynamic = 0, alpha = 0, need_force = 1;
+  bool accretion = false;       // some disk's planets accrete (StubbedPhysics)
+  bool outer_source = false;    // some disk has the outer mass source (StubbedPhysics)
   size_t theta_shmem = 0;
   std::vector<std::vector<double>> vref_h;  // reference vtheta profile per disk (nrad+1)
   // profiling
@@ -269,8 +271,13 @@ struct Batch : BatchBase {
       const fgx_params& p = h.P;
       c.nb = h.sys.nb;
       for (int k = 0; k < c.nb; ++k) {
-        c.feel_disk[k] = h.sys.feel_disk[k]; c.feel_others[k] = h.sys.feel_others[k]; c.mass[k] = h.sys.mass[k];
+        c.feel_disk[k] = h.sys.feel_disk[k]; c.feel_others[k] = h.sys.feel_others[k]; c.acc[k] = h.sys.acc[k];
+        if (p.ext_stubbed_physics && h.sys.acc[k] > 1e-10) c.accretes = 1;
       }
+      accretion |= c.accretes != 0;
+      c.outer_source = (p.ext_stubbed_physics && p.outer_source_mass) ? 1 : 0;
+      c.outer_vr = p.imposeddiskdrift * std::pow(h.rinf[h.nrad - 1], -(double)p.sigmaslope);
+      outer_source |= c.outer_source != 0;
       c.DT = p.dt; c.masstaper = p.masstaper; c.sigmaslope = p.sigmaslope; c.flaringindex = p.flaringindex;
       c.aspectratio = p.aspectratio; c.adiabaticindex = p.adiabaticindex;
       c.thicknesssmoothing = p.thicknesssmoothing; c.rochesmoothing = p.rochesmoothing;
@@ -462,6 +469,7 @@ struct Batch : BatchBase {
       const HostDisk& h = hd[d];
       for (int k = 0; k < h.sys.nb; ++k) {
         s.x[k] = h.sys.x[k]; s.y[k] = h.sys.y[k]; s.vx[k] = h.sys.vx[k]; s.vy[k] = h.sys.vy[k];
+        s.mass[k] = h.sys.mass[k];
       }
       s.omega_frame = h.omega_frame;
       s.active = 0;
@@ -645,6 +653,7 @@ struct Batch : BatchBase {
       LCHK(launch("nonreflecting", 0, k_nonreflecting<real>, nd, FGX_TPB, 0, D));
     }
     if (D.evanescent) LCHK(launch("evanescent", D.adiabatic ? 8 : 6, k_evanescent<real>, n_stiles, FGX_TPB, 0, D));
+    if (outer_source) LCHK(launch("outer_source", 0, k_outer_source<real>, nd, FGX_TPB, 0, D));
     return FGX_OK;
   }
   int substep() {
@@ -652,8 +661,16 @@ struct Batch : BatchBase {
     // the tiling the last summary's forecast chose for this substep
     use_tiles(plan_pos < (int)fine_plan.size() && fine_plan[plan_pos]);
     ++plan_pos;
-    LCHK(launch("cfl_force", 3, k_cfl_force<real>, n_rtiles, 64, 0, D));
-    LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D));
+    LCHK(launch("cfl_force", 3, k_cfl_force<real>, n_rtiles, 64, 0, D, 0));
+    if (accretion) {
+      // AccreteOntoPlanets needs the substep's dt and comes before the forces
+      LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D, 1));
+      LCHK(launch("accrete", 0, k_accrete<real>, nd, FGX_TPB, 0, D));
+      LCHK(launch("cfl_force", 1, k_cfl_force<real>, n_rtiles, 64, 0, D, 1));
+      LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D, 2));
+    } else {
+      LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D, 0));
+    }
     if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_stiles, FGX_TPB, 0, D));
     if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
     LCHK(boundary());
@@ -874,7 +891,9 @@ struct Batch : BatchBase {
     DiskState s;
     LCHK(get_state(d, s));
     *out = hd[d].sys;
-    for (int k = 0; k < out->nb; ++k) { out->x[k] = s.x[k]; out->y[k] = s.y[k]; out->vx[k] = s.vx[k]; out->vy[k] = s.vy[k]; }
+    for (int k = 0; k < out->nb; ++k) {
+      out->x[k] = s.x[k]; out->y[k] = s.y[k]; out->vx[k] = s.vx[k]; out->vy[k] = s.vy[k]; out->mass[k] = s.mass[k];
+    }
     return FGX_OK;
   }
   int set_planets(int d, const fgx_planets* in) override {
@@ -882,7 +901,9 @@ struct Batch : BatchBase {
     if (in->nb != hd[d].sys.nb) return set_error(FGX_EPARAM, "planet count differs from the disk's");
     DiskState s;
     LCHK(get_state(d, s));
-    for (int k = 0; k < in->nb; ++k) { s.x[k] = in->x[k]; s.y[k] = in->y[k]; s.vx[k] = in->vx[k]; s.vy[k] = in->vy[k]; }
+    for (int k = 0; k < in->nb; ++k) {
+      s.x[k] = in->x[k]; s.y[k] = in->y[k]; s.vx[k] = in->vx[k]; s.vy[k] = in->vy[k]; s.mass[k] = in->mass[k];
+    }
     HIPCHK(hipMemcpyAsync(D.st + d, &s, sizeof(DiskState), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
     return FGX_OK;
@@ -1112,6 +1133,10 @@ int fgx_disk_info_get(fgx_disk* d, fgx_disk_info* info) {
 int fgx_batch_info(fgx_batch* b, int disk, fgx_disk_info* info) {
   if (!b || !info) return set_error(FGX_EPARAM, "null argument");
   return b->impl->info(disk, info);
+}
+int fgx_batch_planets(fgx_batch* b, int disk, fgx_planets* out) {
+  if (!b || !out) return set_error(FGX_EPARAM, "null argument");
+  return b->impl->planets(disk, out);
 }
 int fgx_disk_read_field(fgx_disk* d, int field, int dtype, void* host, size_t count) {
   if (!d || !host) return set_error(FGX_EPARAM, "null argument");

@@ -83,7 +83,10 @@ struct DiskConst {
   int rt0;            // index of this disk's first ring tile
   int nb;             // planets
   int feel_disk[FGX_MAX_PLANETS], feel_others[FGX_MAX_PLANETS];
-  double mass[FGX_MAX_PLANETS];
+  double acc[FGX_MAX_PLANETS];   // accretion rates (PlanetarySystem acc, Psys.cu:126)
+  int accretes;                  // StubbedPhysics with some rate above 1e-10: FARGO accretion
+  int outer_source;              // StubbedPhysics + OUTERSOURCEMASS: FARGO's outer mass source
+  double outer_vr;               // its radial velocity, IMPOSEDDISKDRIFT * Rinf[nrad-1]^-SIGMASLOPE
   double DT, masstaper, sigmaslope, flaringindex, aspectratio, adiabaticindex;
   double thicknesssmoothing, rochesmoothing, releasedate, releaseradius;
   double transitionradius, transitionratio, transitionwidth, lambdadoubling;
@@ -106,6 +109,7 @@ struct cxd { double x, y; };
 // mutable FP64 state of one disk (planets, time, frame, per-substep scalars)
 struct DiskState {
   double time, time0, dtemp, dt, omega_frame, masstaper;
+  double mass[FGX_MAX_PLANETS];         // planet masses (change only by accretion)
   double x[FGX_MAX_PLANETS], y[FGX_MAX_PLANETS], vx[FGX_MAX_PLANETS], vy[FGX_MAX_PLANETS];
   // potential of this substep (FillForcesArrays, Pframeforce.cu:37-66)
   double pot_x[FGX_MAX_PLANETS], pot_y[FGX_MAX_PLANETS], pot_m[FGX_MAX_PLANETS],

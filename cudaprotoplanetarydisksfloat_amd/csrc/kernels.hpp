@@ -301,19 +301,22 @@ __device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const
 // ConditionCFLKernel1D + ConditionCFLKernel2D1 + per-ring min of
 // ConditionCFLKernel2D2 (Kernels.cu:1363-1422) and ComputeForceKernel
 // (Kernels.cu:804-839) for the substep's targets, both over the state at the
-// start of the substep: one wave per ring, ring sums and minima by shuffles
+// start of the substep: one wave per ring, ring sums and minima by shuffles.
+// A disk whose planets accrete needs its forces after the accretion, which
+// needs the substep's dt: mode 0 skips its forces, mode 1 forms only those.
 template <class real>
-__global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D) {
+__global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   const RingTile t = D.rtiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
   const FGX_AS4 DiskState& s = cref(D.st + t.d);
   if (!D.ignore_active && !s.active) return;
+  if (mode == 1 && !c.accretes) return;
   const int ns = c.nsec, i = t.i, rb = c.ro;
   const long long fb = c.fo;
   const int lane = (int)threadIdx.x;
   // disk forces on the targets (k = 0 or k = 10 Hill cut), per ring in FP64
-  {
+  if (mode == 1 || !c.accretes) {
     const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
     double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
     const int nt = s.ntarget;
@@ -339,7 +342,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D) {
       if (lane == 0) { out[2 * q] = fx; out[2 * q + 1] = fy; }
     }
   }
-  if (c.substeps_forced > 0 || (D.sloppy && s.sub_in_dt > 0)) return;
+  if (mode == 1 || c.substeps_forced > 0 || (D.sloppy && s.sub_in_dt > 0)) return;
   const double vr0 = RT(vref)[rb + i];
   double acc = 0;
 #pragma unroll 4
@@ -392,7 +395,7 @@ __device__ void set_force_targets(const Dev<real>& D, const DiskConst& c, DiskSt
   const double tt = s.time - s.time0;
   for (int k = 0; k < c.nb; ++k) {
     if (!c.feel_disk[k]) continue;
-    const double m = c.mass[k], x = s.x[k], y = s.y[k];
+    const double m = s.mass[k], x = s.x[k], y = s.y[k];
     const double r = sqrt(x * x + y * y);
     const double sm = D.roche ? r * pow(m / 3., 1. / 3.) * c.rochesmoothing
                               : c.thicknesssmoothing * aspect_ratio_dev(c, r, tt) * pow(r, 1.0 + c.flaringindex);
@@ -470,7 +473,7 @@ __device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, Disk
   // GetPsysInfo (Psys.cu:162-232)
   double x = s.x[0], y = s.y[0], vx = s.vx[0], vy = s.vy[0];
   double xc = x, yc = y;
-  const double m = c.mass[0] + 1.;
+  const double m = s.mass[0] + 1.;
   const double h = x * vy - y * vx;
   const double dd = sqrt(x * x + y * y);
   const double Ax = x * vy * vy - y * vx * vy - m * x / dd;
@@ -500,8 +503,8 @@ __device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, Disk
 
 // DerivMotionRK5 (RungeKutta.cu:9-46)
 template <class real>
-__device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const double* q, double* der, double* dist, int nb,
-                              double dt) {
+__device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const double* mass, const double* q, double* der,
+                              double* dist, int nb, double dt) {
   const double *x = q, *y = q + nb, *vx = q + 2 * nb, *vy = q + 3 * nb;
   for (int i = 0; i < nb; ++i) dist[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
   for (int i = 0; i < nb; ++i) {
@@ -511,8 +514,8 @@ __device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const doub
     double ay = -1.0 / dist[i] / dist[i] / dist[i] * y[i];
     for (int j = 0; j < nb; ++j) {
       if (D.indirect) {
-        ax -= c.mass[j] / dist[j] / dist[j] / dist[j] * x[j];
-        ay -= c.mass[j] / dist[j] / dist[j] / dist[j] * y[j];
+        ax -= mass[j] / dist[j] / dist[j] / dist[j] * x[j];
+        ay -= mass[j] / dist[j] / dist[j] / dist[j] * y[j];
       }
       if (j != i && c.feel_others[i]) {
         double d3;
@@ -520,8 +523,8 @@ __device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const doub
           const double dd = sqrt((x[i] - x[j]) * (x[i] - x[j]) + (y[i] - y[j]) * (y[i] - y[j]));
           d3 = dd * dd * dd;
         } else d3 = dist[j] * dist[j] * dist[j];
-        ax += c.mass[j] / d3 * (x[j] - x[i]);
-        ay += c.mass[j] / d3 * (y[j] - y[i]);
+        ax += mass[j] / d3 * (x[j] - x[i]);
+        ay += mass[j] / d3 * (y[j] - y[i]);
       }
     }
     der[i + 2 * nb] = ax;
@@ -554,7 +557,7 @@ __device__ void planets_block(const Dev<real>& D, int d, double* redd, double* t
   const int nb = c.nb;
   // FillForcesArrays parameters (Pframeforce.cu:49-61) at the current positions
   for (int k = 0; k < nb; ++k) {
-    const double xp = s.x[k], yp = s.y[k], mp = c.mass[k] * s.masstaper;
+    const double xp = s.x[k], yp = s.y[k], mp = s.mass[k] * s.masstaper;
     const double pd = sqrt(xp * xp + yp * yp);
     const double rroche = pd * pow((1.0 / 3.0 * mp), 1.0 / 3.0);
     const double sm = D.roche ? rroche * c.rochesmoothing
@@ -584,20 +587,20 @@ __device__ void planets_block(const Dev<real>& D, int d, double* redd, double* t
         q0[k] = s.x[k]; q0[k + nb] = s.y[k]; q0[k + 2 * nb] = s.vx[k]; q0[k + 3 * nb] = s.vy[k];
       }
       const int n4 = 4 * nb;
-      deriv_rk5_dev(D, c, q0, k1, dist, nb, dt);
+      deriv_rk5_dev(D, c, s.mass, q0, k1, dist, nb, dt);
       for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.2 * k1[m];
-      deriv_rk5_dev(D, c, q1, k2, dist, nb, dt);
+      deriv_rk5_dev(D, c, s.mass, q1, k2, dist, nb, dt);
       for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.075 * k1[m] + 0.225 * k2[m];
-      deriv_rk5_dev(D, c, q1, k3, dist, nb, dt);
+      deriv_rk5_dev(D, c, s.mass, q1, k3, dist, nb, dt);
       for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.3 * k1[m] - 0.9 * k2[m] + 1.2 * k3[m];
-      deriv_rk5_dev(D, c, q1, k4, dist, nb, dt);
+      deriv_rk5_dev(D, c, s.mass, q1, k4, dist, nb, dt);
       for (int m = 0; m < n4; ++m)
         q1[m] = q0[m] - 11.0 / 54.0 * k1[m] + 2.5 * k2[m] - 70.0 / 27.0 * k3[m] + 35.0 / 27.0 * k4[m];
-      deriv_rk5_dev(D, c, q1, k5, dist, nb, dt);
+      deriv_rk5_dev(D, c, s.mass, q1, k5, dist, nb, dt);
       for (int m = 0; m < n4; ++m)
         q1[m] = q0[m] + 1631.0 / 55296.0 * k1[m] + 175.0 / 512.0 * k2[m] + 575.0 / 13824.0 * k3[m] +
                 44275.0 / 110592.0 * k4[m] + 253.0 / 4096.0 * k5[m];
-      deriv_rk5_dev(D, c, q1, k6, dist, nb, dt);
+      deriv_rk5_dev(D, c, s.mass, q1, k6, dist, nb, dt);
       for (int m = 0; m < n4; ++m)
         q1[m] = q0[m] + 37.0 / 378.0 * k1[m] + 250.0 / 621.0 * k3[m] + 125.0 / 594.0 * k4[m] + 512.0 / 1771.0 * k6[m];
     }
@@ -616,10 +619,10 @@ __device__ void planets_block(const Dev<real>& D, int d, double* redd, double* t
     if (s.time < c.releasedate) {
       const double x = s.x[0], y = s.y[0], r = sqrt(x * x + y * y), th = atan2(y, x);
       const double rdot = (c.releaseradius - r) / (c.releasedate - s.time);
-      const double om = sqrt((1. + c.mass[0]) / r / r / r);
+      const double om = sqrt((1. + s.mass[0]) / r / r / r);
       const double nrr = r + rdot * dt, den = r - nrr;
       const double dth = (den != 0.0) ? 2. * dt * r * om / den * (sqrt(r / nrr) - 1.) : om * dt;
-      const double vx = rdot, vy = nrr * sqrt((1. + c.mass[0]) / nrr / nrr / nrr);
+      const double vx = rdot, vy = nrr * sqrt((1. + s.mass[0]) / nrr / nrr / nrr);
       s.x[0] = nrr * cos(dth + th); s.y[0] = nrr * sin(dth + th);
       s.vx[0] = vx * cos(dth + th) - vy * sin(dth + th);
       s.vy[0] = vx * sin(dth + th) + vx * cos(dth + th);  // as Pframeforce.cu:154
@@ -646,21 +649,105 @@ __device__ void planets_block(const Dev<real>& D, int d, double* redd, double* t
 // the per-disk serial part of a substep's start: the CFL substep count and dt,
 // MassTaper, boundary constants (substep_begin_block), then the disk forces
 // summed over rings, the potential's planet parameters, AdvanceSystemFromDisk,
-// the RK5 N-body step and the frame (planets_block): one block per disk
+// the RK5 N-body step and the frame (planets_block): one block per disk.
+// part 0 runs both halves, 1 the first, 2 the second (accretion sits between)
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_substep_setup(Dev<real> D) {
+__global__ void __launch_bounds__(FGX_TPB) k_substep_setup(Dev<real> D, int part) {
   const int d = blockIdx.x;
   if (!D.st[d].active) return;
   __shared__ real redr[FGX_TPB / 64];
   __shared__ double redd[FGX_TPB / 64];
   __shared__ double tot[2 * FGX_NFORCE_T];
   __shared__ double rk[8 * 4 * FGX_MAX_PLANETS + FGX_MAX_PLANETS];
-  substep_begin_block(D, d, redr);
+  if (part != 2) substep_begin_block(D, d, redr);
+  if (part == 1) return;
   __syncthreads();
   planets_block(D, d, redd, tot, rk);
 }
 
 // CorrectVthetaKernel (Kernels.cu:1352-1359)
+// AccreteOntoPlanets with FARGO's kernel (Planet.cu:7-47 computes the Roche
+// box and leaves the launch commented out; it runs here with StubbedPhysics):
+// for each accreting planet in turn, the cells of its Roche box closer than
+// 0.75 R_Roche lose facc/3 of their density and those closer than 0.45
+// R_Roche a further 2 facc/3 (facc = dt * acc, W. Kley's scheme); the planet
+// takes that mass and its momentum, the cell velocity formed at the cell
+// centre from the face values.  The angle of the box is real (the reference
+// declares it int, Planet.cu:9) and a box wider than the ring is cut at nsec
+// sectors so that no cell is visited twice.  Afterwards the force targets
+// follow the new masses.  One block per disk; FP64 per cell, FP64 block sums.
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_accrete(Dev<real> D) {
+  const int d = blockIdx.x;
+  const DiskConst& c = D.dc[d];
+  DiskState& s = D.st[d];
+  if (!s.active || !c.accretes) return;
+  __shared__ double redd[FGX_TPB / 64];
+  __shared__ int box[4];
+  const int ns = c.nsec, nr = c.nrad, rb = c.ro;
+  const long long fb = c.fo;
+  for (int k = 0; k < c.nb; ++k) {
+    if (!(c.acc[k] > 1e-10)) continue;
+    const double facc = s.dt * c.acc[k], facc1 = 1.0 / 3.0 * facc, facc2 = 2.0 / 3.0 * facc;
+    const double xp = s.x[k], yp = s.y[k], mp = s.mass[k];
+    const double rp = sqrt(xp * xp + yp * yp);
+    const double rroche = pow(1.0 / 3.0 * mp, 1.0 / 3.0) * rp;
+    if (threadIdx.x == 0) {
+      int imin = 0, imax = nr - 1;
+      while (imin < nr && (double)RT(rsup)[rb + imin] < rp - rroche) imin++;
+      while (imax > 0 && (double)RT(rinf)[rb + imax] > rp + rroche) imax--;
+      const double angle = atan2(yp, xp);
+      const int jmin = (int)((double)ns / 2.0 / kPI * (angle - 2.0 * rroche / rp));
+      int jmax = (int)((double)ns / 2.0 / kPI * (angle + 2.0 * rroche / rp));
+      if (jmax - jmin + 1 > ns) jmax = jmin + ns - 1;
+      box[0] = imin; box[1] = imax; box[2] = jmin; box[3] = jmax;
+    }
+    __syncthreads();
+    const int imin = box[0], jmin = box[2], nj = box[3] - box[2] + 1;
+    const int ncell = (box[1] >= imin && nj > 0) ? (box[1] - imin + 1) * nj : 0;
+    double dm = 0, dpx = 0, dpy = 0;
+    for (int q = threadIdx.x; q < ncell; q += FGX_TPB) {
+      const int i = imin + q / nj;
+      int jf = (jmin + q % nj) % ns;
+      if (jf < 0) jf += ns;
+      const int jp = (jf + 1 == ns) ? 0 : jf + 1;
+      const real rm = RT(rmed)[rb + i];
+      const double xc = (double)(rm * RT(cosj)[c.so + jf]), yc = (double)(rm * RT(sinj)[c.so + jf]);
+      const double dx = xp - xc, dy = yp - yc;
+      const double dist = sqrt(dx * dx + dy * dy);
+      if (!(dist < 0.75 * rroche)) continue;
+      const double rmd = (double)rm;
+      // Vtheta is held as a residual of the ring's reference velocity
+      const double vt = RT(vref)[rb + i] + 0.5 * ((double)(real)AT(D.vtheta, i, jf) + (double)(real)AT(D.vtheta, i, jp)) +
+                        rmd * s.omega_frame;
+      const double vr = 0.5 * ((double)(real)AT(D.vrad, i, jf) + (double)(real)AT(D.vrad, i + 1, jf));
+      const double vx = (vr * xc - vt * yc) / rmd, vy = (vr * yc + vt * xc) / rmd;
+      const double surf = (double)RT(surf)[rb + i];
+      double den = (double)(real)AT(D.dens, i, jf);
+      double m = facc1 * den * surf;
+      den = (double)(real)(den * (1.0 - facc1));
+      dpx += m * vx; dpy += m * vy; dm += m;
+      if (dist < 0.45 * rroche) {
+        m = facc2 * den * surf;
+        den = (double)(real)(den * (1.0 - facc2));
+        dpx += m * vx; dpy += m * vy; dm += m;
+      }
+      AT(D.dens, i, jf) = (real)den;
+    }
+    // the sums also order this planet's density updates before the next one's
+    dm = block_sum(dm, redd);
+    dpx = block_sum(dpx, redd);
+    dpy = block_sum(dpy, redd);
+    if (threadIdx.x == 0) {
+      const double px = mp * s.vx[k] + dpx, py = mp * s.vy[k] + dpy, m = mp + dm;
+      if (c.feel_disk[k]) { s.vx[k] = px / m; s.vy[k] = py / m; }
+      s.mass[k] = m;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) set_force_targets(D, c, s);
+}
+
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_correct_vtheta(Dev<real> D) {
   FGX_STRIP_BEGIN {
@@ -1023,6 +1110,31 @@ __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
     AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - RT(vref)[rb + i])) * inv);
     AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * RT(sigmamed_d)[rb + i]) * inv);
     if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * RT(energymed_d)[rb + i]) * inv);
+  }
+}
+
+// ApplyOuterSourceMass, FARGO's SideEuler.c (the reference leaves it
+// commented out, SideEuler.cu:143-146,181; it runs here with StubbedPhysics):
+// the last ring's mean density is reset to SigmaMed, keeping its azimuthal
+// structure, and its radial velocity to IMPOSEDDISKDRIFT * Rinf^-SIGMASLOPE.
+// One block per disk.
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_outer_source(Dev<real> D) {
+  const int d = blockIdx.x;
+  const DiskConst& c = D.dc[d];
+  const DiskState& s = D.st[d];
+  if (!c.outer_source || (!D.ignore_active && !s.active)) return;
+  __shared__ double redd[FGX_TPB / 64];
+  const int ns = c.nsec, i = c.nrad - 1;
+  const long long fb = c.fo;
+  double a = 0;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) a += (double)(real)AT(D.dens, i, j);
+  a = block_sum(a, redd);
+  const double add = RT(sigmamed_d)[c.ro + i] - a / (double)ns;
+  const real vr = (real)c.outer_vr;
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+    AT(D.dens, i, j) = (real)((double)(real)AT(D.dens, i, j) + add);
+    AT(D.vrad, i, j) = vr;
   }
 }
 

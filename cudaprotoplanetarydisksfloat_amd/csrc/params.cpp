@@ -137,6 +137,7 @@ static void search_variable(fgx_params* p, const char* name, const char* sv) {
   if (std::strcmp(name, "SGSOFTENING") == 0) p->ext_sg_softening = f;
   if (std::strcmp(name, "GASNAMES") == 0) p->ext_gas_names = ti;
   if (std::strcmp(name, "CHECKPOINT") == 0) p->ext_checkpoint = ti != 0 || std::strcmp(sv, "YES") == 0;
+  if (std::strcmp(name, "STUBBEDPHYSICS") == 0) p->ext_stubbed_physics = ti != 0 || std::strcmp(sv, "YES") == 0;
   // the executable's switches -c, -b, -f (Main.cu:122-170) as keys
   if (std::strcmp(name, "SLOPPYCFL") == 0) p->sloppy_cfl = ti;
   if (std::strcmp(name, "CENTRIFUGALBALANCE") == 0) p->centrifugal_balance = ti;
@@ -261,7 +262,7 @@ int params_get_key(const fgx_params* p, const char* key, char* out, size_t n) {
       {"NOISEAMPLITUDE", p->ext_noise_amplitude}, {"NOISESEED", (double)p->ext_noise_seed},
       {"DISKID", (double)p->ext_disk_id}, {"SUBSTEPS", (double)p->ext_substeps},
       {"SGSOFTENING", p->ext_sg_softening}, {"GASNAMES", (double)p->ext_gas_names},
-      {"CHECKPOINT", (double)p->ext_checkpoint},
+      {"CHECKPOINT", (double)p->ext_checkpoint}, {"STUBBEDPHYSICS", (double)p->ext_stubbed_physics},
       // derived flags
       {"LOGGRID", (double)p->log_grid}, {"FASTTRANSPORT", (double)p->fast_transport},
       {"OPENINNER", (double)p->open_inner}, {"NONREFLECTING", (double)p->nonreflecting},

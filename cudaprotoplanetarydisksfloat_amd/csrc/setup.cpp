@@ -143,6 +143,59 @@ int setup_grid_and_density(const fgx_params& P, const fgx_planets& pl, HostDisk&
   return FGX_OK;
 }
 
+// Exact centrifugal balance of -b with FARGO's InitGasVelocities formula (the
+// reference's Pframeforce.cu:202-237 averages the sound speed instead of the
+// pressure, drops the point-mass term and never applies the result,
+// Kernels.cu:783): at each inner interface Radii[i],
+//   v^2 / r = (P[i] - P[i-1]) / (Sigma-bar (Rmed[i] - Rmed[i-1])) + G / (Rmed[i] Rmed[i-1]) - g_sg
+// from the ring-mean pressure, the analytic Sigma and the ring-mean radial SG
+// acceleration interpolated to the interface, frame velocity removed; ring
+// values u follow from (u[i] + u[i-1]) / 2 = v[i] with u[0] from one secant
+// step on the summed second differences (ConstructSequence,
+// SourceEuler.cu:416-425).  NRAD+1 values.
+static std::vector<double> centrifugal_balance(const HostDisk& h, const std::vector<double>& sg_axi) {
+  const fgx_params& P = h.P;
+  const int NR = h.nrad, NS = h.nsec;
+  auto sigma = [&](double r) {
+    double cav = (r < P.cavityradius) ? 1.0 / P.cavityratio : 1.0;
+    return cav * P.scaling_factor * P.sigma0 * std::pow(r, -(double)P.sigmaslope);
+  };
+  std::vector<double> gp(NR), v(NR + 1, 0.0), u(NR + 1, 0.0);
+  for (int i = 0; i < NR; ++i) {
+    double a = 0;
+    for (int j = 0; j < NS; ++j) {
+      const size_t k = (size_t)i * NS + j;
+      // ComputePressureFieldKernel (Kernels.cu:196-208) at t = 0
+      a += P.adiabatic ? (P.adiabaticindex - 1.0) * h.energy[k] : h.dens[k] * h.cs_ring[i] * h.cs_ring[i];
+    }
+    gp[i] = a / (double)NS;
+  }
+  for (int i = 1; i < NR; ++i) {
+    const double r0 = h.rmed[i - 1], r1 = h.rmed[i];
+    v[i] = (gp[i] - gp[i - 1]) / (.5 * (sigma(r1) + sigma(r0))) / (r1 - r0) + kG * (1.0 / r0 - 1.0 / r1) / (r1 - r0);
+  }
+  if (P.self_gravity && !sg_axi.empty())
+    for (int i = 1; i < NR; ++i)
+      v[i] -= ((h.radii[i] - h.rmed[i - 1]) * sg_axi[i] + (h.rmed[i] - h.radii[i]) * sg_axi[i - 1]) /
+              (h.rmed[i] - h.rmed[i - 1]);
+  for (int i = 1; i < NR; ++i) v[i] = std::sqrt(v[i] * h.radii[i]) - h.radii[i] * h.omega_frame;
+  auto sequence = [&]() {
+    for (int i = 1; i < NR; ++i) u[i] = 2.0 * v[i] - u[i - 1];
+    double lapl = 0;
+    for (int i = 1; i < NR - 1; ++i) lapl += std::fabs(u[i + 1] + u[i - 1] - 2.0 * u[i]);
+    return lapl;
+  };
+  const double t1 = u[0] = v[1] + .75 * (v[1] - v[2]);
+  const double r1 = sequence();
+  u[0] += .25 * (v[1] - v[2]);
+  const double t2 = u[0];
+  const double r2 = sequence();
+  u[0] = t1 - r1 / (r2 - r1) * (t2 - t1);
+  sequence();
+  u[NR] = u[NR - 1];
+  return u;
+}
+
 void setup_velocities(HostDisk& h, const std::vector<double>& sg_axi) {
   const fgx_params& P = h.P;
   const int NR = h.nrad, NS = h.nsec;
@@ -180,6 +233,8 @@ void setup_velocities(HostDisk& h, const std::vector<double>& sg_axi) {
       for (int j = 0; j < NS; ++j) h.vtheta[(size_t)i * NS + j] = r * om;
     }
   }
+  std::vector<double> vt_cent;
+  if (P.centrifugal_balance && P.ext_stubbed_physics) vt_cent = centrifugal_balance(h, sg_axi);
   // InitGasVelocitiesKernel (Kernels.cu:759-800)
   for (int i = 0; i <= NR; ++i) {
     double r = (i == NR) ? h.rmed[NR - 1] : h.rmed[i];
@@ -192,6 +247,7 @@ void setup_velocities(HostDisk& h, const std::vector<double>& sg_axi) {
                                                    (1. + P.sigmaslope - 2.0 * P.flaringindex));
       }
       h.vtheta[k] -= h.omega_frame * r;
+      if (!vt_cent.empty()) h.vtheta[k] = vt_cent[i];
       if (i == NR) h.vrad[k] = 0.0;
       else {
         double v = P.imposeddiskdrift * P.sigma0 / h.sigmainf[i] / ri;

@@ -104,6 +104,7 @@ _sigs = {
     "fgx_disk_destroy": ([_vp], _i), "fgx_batch_destroy": ([_vp], _i),
     "fgx_batch_size": ([_vp, ctypes.POINTER(_i)], _i),
     "fgx_batch_info": ([_vp, _i, ctypes.POINTER(DiskInfo)], _i),
+    "fgx_batch_planets": ([_vp, _i, ctypes.POINTER(Planets)], _i),
     "fgx_batch_read_field": ([_vp, _i, _i, _i, _vp, _sz], _i),
     "fgx_batch_write_field": ([_vp, _i, _i, _i, _vp, _sz], _i),
     "fgx_disk_read_ring": ([_vp, _i, _i, _vp, _sz], _i),
@@ -268,9 +269,9 @@ class Batch:
         _check(lib.fgx_disk_read_ring(self.h, RINGS[name], F64, out.ctypes.data, nr + 1))
         return out
 
-    def planets(self):
+    def planets(self, disk=0):
         pl = Planets()
-        _check(lib.fgx_disk_planets(self.h, ctypes.byref(pl)))
+        _check(lib.fgx_batch_planets(self.h, disk, ctypes.byref(pl)))
         return pl
 
     def set_planets(self, pl):

@@ -89,6 +89,12 @@ typedef struct fgx_params {
   float ext_sg_softening;      /* SGSOFTENING: >0 overrides SG epsilon/H     */
   int   ext_gas_names;         /* GASNAMES: 0 dens/ layout, 1 gasdens*.dat, 2 both */
   int   ext_checkpoint;        /* CHECKPOINT: fgx-run writes restart<N>.bin at each output */
+  int   ext_stubbed_physics;   /* STUBBEDPHYSICS: run what the reference parses but leaves  */
+                               /* stubbed, with FARGO's semantics: planet accretion     */
+                               /* (src/Planet.cu:7-47), label advection                 */
+                               /* (src/TransportEuler.cu:28,34,68), -b centrifugal      */
+                               /* balance (src/Pframeforce.cu:202-237) and the outer    */
+                               /* mass source (src/SideEuler.cu:143-146,181)            */
 } fgx_params;
 
 /* ---- planetary system: the reference's PlanetarySystem (include/Types.cuh:2-15) */
@@ -204,6 +210,9 @@ int fgx_batch_step(fgx_batch* b, int n_dt);
 int fgx_batch_size(fgx_batch* b, int* n);
 int fgx_batch_destroy(fgx_batch* b);
 int fgx_batch_info(fgx_batch* b, int disk, fgx_disk_info* info);
+/* planets of disk `disk` (positions, velocities, masses: accretion changes
+ * them with StubbedPhysics), as fgx_disk_planets for one disk */
+int fgx_batch_planets(fgx_batch* b, int disk, fgx_planets* out);
 int fgx_batch_read_field(fgx_batch* b, int disk, int field, int dtype, void* host,
                          size_t count);
 int fgx_batch_write_field(fgx_batch* b, int disk, int field, int dtype,

@@ -462,6 +462,8 @@ struct Disk {
 
   // InitGasVelocities + InitVelocities (Pframeforce.cu:194-272, Kernels.cu:759-800)
   void init_gas_velocities() {
+    std::vector<double> vt_cent;
+    if (P.centrifugal_balance && P.ext_stubbed_physics) vt_cent = centrifugal_balance();
     if (!P.centrifugal_balance && P.self_gravity) {
       // Init_azimutalvelocity_withSG (SourceEuler.cu:428-437, Kernels.cu:1044-1059)
       for (int i = 0; i < NRAD; ++i) {
@@ -496,6 +498,7 @@ struct Disk {
                                                                  (1. + P.sigmaslope - 2.0 * P.flaringindex)));
         }
         Vtheta[id(i, j)] = (real)((double)Vtheta[id(i, j)] - OmegaFrame * r);
+        if (!vt_cent.empty()) Vtheta[id(i, j)] = (real)vt_cent[i];
         if (i == NRAD) Vrad[id(i, j)] = 0;
         else {
           double v = P.imposeddiskdrift * P.sigma0 / (double)SigmaInf[i] / ri;
@@ -506,6 +509,57 @@ struct Disk {
         if (i == 0) { Vrad[id(0, j)] = 0; Vrad[id(NRAD, j)] = 0; }
       }
     }
+  }
+
+  // Exact centrifugal balance of -b, FARGO's InitGasVelocities: at each inner
+  // interface Radii[i], r*Omega^2 = (dP/dr)/Sigma + G/r^2 - g_sg, from the
+  // ring-mean pressure, the analytic Sigma, the point-mass potential and the
+  // ring-mean radial SG acceleration interpolated to the interface; ring
+  // values u[i] then follow from (u[i] + u[i-1])/2 = v[i], with u[0] chosen by
+  // one secant step on the summed second differences of u (ConstructSequence,
+  // SourceEuler.cu:416-425).  The reference's version (Pframeforce.cu:202-237)
+  // averages the sound speed instead of the pressure, drops the G/r^2 term
+  // and is never applied (Kernels.cu:783), so it runs here only with
+  // StubbedPhysics.  Returns NRAD+1 ring values, frame velocity removed.
+  std::vector<double> centrifugal_balance() {
+    std::vector<double> gp(NRAD), v(NRAD + 1, 0.0), u(NRAD + 1, 0.0);
+    for (int i = 0; i < NRAD; ++i) {
+      double a = 0;
+      for (int j = 0; j < NSEC; ++j) a += (double)Pressure[id(i, j)];
+      gp[i] = a / (double)NSEC;
+    }
+    for (int i = 1; i < NRAD; ++i) {
+      const double r0 = Rmed[i - 1], r1 = Rmed[i];
+      v[i] = (gp[i] - gp[i - 1]) / (.5 * (sigma_profile(r1) + sigma_profile(r0))) / (r1 - r0) +
+             G * (1.0 / r0 - 1.0 / r1) / (r1 - r0);
+    }
+    if (P.self_gravity) {
+      std::vector<double> ax(NRAD);
+      for (int i = 0; i < NRAD; ++i) {
+        double a = 0;
+        for (int j = 0; j < NSEC; ++j) a += (double)SG_Accr[id(i, j)];
+        ax[i] = a / (double)NSEC;
+      }
+      for (int i = 1; i < NRAD; ++i)
+        v[i] -= (((double)Radii[i] - (double)Rmed[i - 1]) * ax[i] + ((double)Rmed[i] - (double)Radii[i]) * ax[i - 1]) /
+                ((double)Rmed[i] - (double)Rmed[i - 1]);
+    }
+    for (int i = 1; i < NRAD; ++i) v[i] = std::sqrt(v[i] * (double)Radii[i]) - (double)Radii[i] * OmegaFrame;
+    auto sequence = [&]() {
+      for (int i = 1; i < NRAD; ++i) u[i] = 2.0 * v[i] - u[i - 1];
+      double lapl = 0;
+      for (int i = 1; i < NRAD - 1; ++i) lapl += std::fabs(u[i + 1] + u[i - 1] - 2.0 * u[i]);
+      return lapl;
+    };
+    double t1 = u[0] = v[1] + .75 * (v[1] - v[2]);
+    const double r1 = sequence();
+    u[0] += .25 * (v[1] - v[2]);
+    const double t2 = u[0];
+    const double r2 = sequence();
+    u[0] = t1 - r1 / (r2 - r1) * (t2 - t1);
+    sequence();
+    u[NRAD] = u[NRAD - 1];
+    return u;
   }
 
   // ------------------------------------------------------------------ CFL
@@ -741,6 +795,20 @@ struct Disk {
       nonreflecting_boundary();
     }
     if (P.evanescent) evanescent_boundary(dt);
+    if (P.outer_source_mass && P.ext_stubbed_physics) outer_source_mass();
+  }
+  // ApplyOuterSourceMass, FARGO's SideEuler.c: the reference leaves it
+  // commented out (SideEuler.cu:143-146,181).  The last ring's mean density
+  // is reset to SigmaMed, keeping its azimuthal structure, and its radial
+  // velocity to the imposed drift IMPOSEDDISKDRIFT * Rinf^-SIGMASLOPE.
+  void outer_source_mass() {
+    const int i = NRAD - 1;
+    const real add = SigmaMed[i] - ring_mean(Dens, i);
+    const real vr = (real)(P.imposeddiskdrift * std::pow((double)Rinf[i], -(double)P.sigmaslope));
+    for (int j = 0; j < NSEC; ++j) {
+      Dens[id(i, j)] += add;
+      Vrad[id(i, j)] = vr;
+    }
   }
   // OpenBoundaryKernel (Kernels.cu:624-637)
   void open_boundary() {
@@ -1207,6 +1275,57 @@ struct Disk {
     return s;
   }
 
+  // AccreteOntoPlanets, FARGO's Planet.c (W. Kley's scheme): the reference
+  // computes the Roche box but its kernel launch is commented out
+  // (Planet.cu:7-47).  Gas within 0.75 R_Roche loses facc/3 of its density per
+  // substep, gas within 0.45 R_Roche a further 2 facc/3, facc = dt * acc; the
+  // planet gains that mass and its momentum (cell velocity at the cell
+  // centre from the face-centred Vrad, Vtheta).  The angle is real as in
+  // FARGO (the reference declares it int, Planet.cu:9).  A box wider than the
+  // ring is cut at NSEC sectors so no cell is visited twice.
+  void accrete_onto_planets(double dt) {
+    for (int k = 0; k < sys.nb; ++k) {
+      if (!(sys.acc[k] > 1e-10)) continue;
+      double dM = 0, dPx = 0, dPy = 0;
+      const double facc = dt * sys.acc[k], facc1 = 1.0 / 3.0 * facc, facc2 = 2.0 / 3.0 * facc;
+      const double frac1 = 0.75, frac2 = 0.45;
+      const double xp = sys.x[k], yp = sys.y[k], vxp = sys.vx[k], vyp = sys.vy[k], mp = sys.mass[k];
+      const double rp = std::sqrt(xp * xp + yp * yp);
+      const double rroche = std::pow(1.0 / 3.0 * mp, 1.0 / 3.0) * rp;
+      int imin = 0, imax = NRAD - 1;
+      while (imin < NRAD && (double)Rsup[imin] < rp - rroche) imin++;
+      while (imax > 0 && (double)Rinf[imax] > rp + rroche) imax--;
+      const double angle = std::atan2(yp, xp);
+      const int jmin = (int)((double)NSEC / 2.0 / PI * (angle - 2.0 * rroche / rp));
+      int jmax = (int)((double)NSEC / 2.0 / PI * (angle + 2.0 * rroche / rp));
+      if (jmax - jmin + 1 > NSEC) jmax = jmin + NSEC - 1;
+      for (int i = imin; i <= imax; ++i)
+        for (int j = jmin; j <= jmax; ++j) {
+          const int jf = ((j % NSEC) + NSEC) % NSEC;
+          const size_t l = id(i, jf), lip = id(i + 1, jf), ljp = id(i, jp1(jf));
+          const double xc = (double)(Rmed[i] * cosj[jf]), yc = (double)(Rmed[i] * sinj[jf]);
+          const double dx = xp - xc, dy = yp - yc;
+          const double dist = std::sqrt(dx * dx + dy * dy);
+          const double vt = 0.5 * ((double)Vtheta[l] + (double)Vtheta[ljp]) + (double)Rmed[i] * OmegaFrame;
+          const double vr = 0.5 * ((double)Vrad[l] + (double)Vrad[lip]);
+          const double vx = (vr * xc - vt * yc) / (double)Rmed[i], vy = (vr * yc + vt * xc) / (double)Rmed[i];
+          if (dist < frac1 * rroche) {
+            const double m = facc1 * (double)Dens[l] * (double)Surf[i];
+            Dens[l] = (real)((double)Dens[l] * (1.0 - facc1));
+            dPx += m * vx; dPy += m * vy; dM += m;
+          }
+          if (dist < frac2 * rroche) {
+            const double m = facc2 * (double)Dens[l] * (double)Surf[i];
+            Dens[l] = (real)((double)Dens[l] * (1.0 - facc2));
+            dPx += m * vx; dPy += m * vy; dM += m;
+          }
+        }
+      const double px = mp * vxp + dPx, py = mp * vyp + dPy, m = mp + dM;
+      if (sys.feel_disk[k]) { sys.vx[k] = px / m; sys.vy[k] = py / m; }
+      sys.mass[k] = m;
+    }
+  }
+
   // ---------------------------------------------------------------- AlgoGas
   // AlgoGas (SourceEuler.cu:208-301); returns the number of substeps
   int algo_gas() {
@@ -1227,6 +1346,7 @@ struct Disk {
         n = cfl(DT - dtemp);
         dt = (DT - dtemp) / (double)n;
       }
+      if (P.is_disk && P.ext_stubbed_physics) accrete_onto_planets(dt);
       last_ncfl = n;
       dtemp += dt;
       DOPAx = DOPAy = 0.0;

@@ -139,6 +139,7 @@ int orc_stage(void* hp, const char* name, double dt) {
     } else if (n == "transport") s.transport(dt);
     else if (n == "boundary") s.apply_boundary(dt);
     else if (n == "potential") s.fill_forces_arrays();
+    else if (n == "accrete") s.accrete_onto_planets(dt);
     else return -1;
   });
   return 0;

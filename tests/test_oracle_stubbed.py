@@ -1,0 +1,198 @@
+"""Pins of the oracle's StubbedPhysics branch: FARGO's outer mass source,
+exact centrifugal balance and planet accretion, which the reference parses
+but never runs (src/SideEuler.cu:143-146,181; src/Pframeforce.cu:202-237 with
+src/Kernels.cu:783; src/Planet.cu:7-47).  Without the key every one of them
+must leave the reference's behaviour untouched.  CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cudaprotoplanetarydisksfloat_amd import Planets, configs
+import oracle
+
+
+def make(nrad=24, nsec=48, cfg=None, **keys):
+    par, cfg0 = configs.plain_disk(nrad=nrad, nsec=nsec)
+    p, pl = configs.load(par, cfg if cfg is not None else cfg0, **keys)
+    return p, pl, oracle.Oracle(p, pl, 1)
+
+
+def edges(p):
+    n, rmin, rmax = int(p["NRAD"]), p["RMIN"], p["RMAX"]
+    if p["LOGGRID"]:
+        return rmin * np.exp(np.arange(n + 1) / n * np.log(rmax / rmin))
+    return rmin + np.float32((np.float32(rmax) - np.float32(rmin)) / np.float32(n)) * np.arange(n + 1, dtype=np.float64)
+
+
+# ------------------------------------------------------------ outer mass source
+def _perturbed(o, p):
+    nr, ns = int(p["NRAD"]), int(p["NSEC"])
+    rng = np.random.RandomState(5)
+    d = o.field("dens", ghost=True)
+    d[nr - 1] *= 1 + 0.2 * rng.rand(ns)
+    o.set_field("dens", d)
+    v = o.field("vrad", ghost=True)
+    v[nr - 1] = rng.randn(ns) * 1e-3
+    o.set_field("vrad", v)
+    return d.copy(), v.copy()
+
+
+def test_outer_source_mass_resets_the_last_ring():
+    p, _, o = make(OUTERSOURCEMASS="YES", STUBBEDPHYSICS="YES", IMPOSEDDISKDRIFT=-2e-4, SIGMASLOPE=0.75)
+    nr = int(p["NRAD"])
+    d0, v0 = _perturbed(o, p)
+    o.stage("boundary", 0.01)
+    d1, v1 = o.field("dens", ghost=True), o.field("vrad", ghost=True)
+    sigmamed = o.ring("sigmamed")[nr - 1]
+    # the ring's mean becomes SigmaMed, its azimuthal structure is kept
+    assert d1[nr - 1].mean() == pytest.approx(sigmamed, rel=1e-14)
+    shift = d1[nr - 1] - d0[nr - 1]
+    assert np.ptp(shift) < 1e-15 * sigmamed
+    assert shift[0] == pytest.approx(sigmamed - d0[nr - 1].mean(), rel=1e-12)
+    # radial velocity: the imposed drift at the ring's inner edge
+    rinf = edges(p)[nr - 1]
+    drift = float(np.float32(-2e-4))          # the key is stored as float, as the reference's
+    assert np.allclose(v1[nr - 1], drift * rinf ** -0.75, rtol=1e-13, atol=0)
+    # nothing else moves
+    assert np.array_equal(d1[: nr - 1], d0[: nr - 1]) and np.array_equal(v1[: nr - 1], v0[: nr - 1])
+
+
+def test_outer_source_mass_is_inert_without_stubbed_physics():
+    p, _, o = make(OUTERSOURCEMASS="YES")
+    d0, v0 = _perturbed(o, p)
+    o.stage("boundary", 0.01)
+    assert np.array_equal(o.field("dens", ghost=True), d0)
+    assert np.array_equal(o.field("vrad", ghost=True), v0)
+
+
+# --------------------------------------------------------- centrifugal balance
+def _balance_error(nrad, **keys):
+    p, _, o = make(nrad=nrad, nsec=16, NOISEAMPLITUDE=0, CENTRIFUGALBALANCE=1, STUBBEDPHYSICS="YES", **keys)
+    h, f, s = p["ASPECTRATIO"], p["FLARINGINDEX"], p["SIGMASLOPE"]
+    r = o.ring("rmed")[:nrad]
+    vt = o.field("vtheta")
+    exact = np.sqrt(1 / r) * np.sqrt(1 - h * h * r ** (2 * f) * (1 + s - 2 * f))
+    assert np.ptp(vt, axis=1).max() == 0          # one value per ring
+    return np.abs(vt[:, 0] / exact - 1).max()
+
+
+@pytest.mark.parametrize("flaring,slope", [(0.0, 0.5), (0.25, 1.0)])
+def test_centrifugal_balance_converges_to_the_pressure_supported_orbit(flaring, slope):
+    """For a power-law isothermal disk the balance r Omega^2 = dP/dr / Sigma +
+    G / r^2 has the closed form v^2 = G/r (1 - (1 + slope - 2 flaring) h^2
+    r^(2 flaring)).  The discrete version must approach it at second order; a
+    dropped gravity term gives NaN, a sign slip in the pressure term an O(h^2)
+    error that does not shrink."""
+    e1 = _balance_error(48, FLARINGINDEX=flaring, SIGMASLOPE=slope)
+    e2 = _balance_error(96, FLARINGINDEX=flaring, SIGMASLOPE=slope)
+    assert e1 < 2e-4, e1
+    assert e1 / e2 > 3.0, (e1, e2)
+
+
+def test_centrifugal_balance_is_inert_without_stubbed_physics():
+    p, _, a = make(NOISEAMPLITUDE=0, CENTRIFUGALBALANCE=1)
+    _, _, b = make(NOISEAMPLITUDE=0)
+    assert np.array_equal(a.field("vtheta", ghost=True), b.field("vtheta", ghost=True))
+
+
+def _sg_pair(nrad):
+    par, cfg = configs.sg_disk(nrad=nrad, nsec=64, noise=0)
+    base = dict(CENTRIFUGALBALANCE=1)
+    p, pl = configs.load(par, cfg, **base)
+    ref = oracle.Oracle(p, pl, 1)         # reference init: -b skips the SG velocity, plain Keplerian
+    p2, pl2 = configs.load(par, cfg, STUBBEDPHYSICS="YES", **base)
+    bal = oracle.Oracle(p2, pl2, 1)
+    p3, pl3 = configs.load(par, cfg)
+    sgv = oracle.Oracle(p3, pl3, 1)       # Init_azimutalvelocity_withSG: analytic with the SG ring mean
+    return ref, bal, sgv
+
+
+def test_centrifugal_balance_with_self_gravity_matches_the_sg_rotation_curve():
+    """With self-gravity the balance adds the interpolated ring-mean SG
+    acceleration; its result must approach the reference's analytic SG
+    rotation curve (Init_azimutalvelocity_withSG) at second order, while the
+    reference's own -b path (no SG term) stays O(SG) away from it."""
+    errs = []
+    for nrad in (64, 128):
+        ref, bal, sgv = _sg_pair(nrad)
+        v_sg = sgv.field("vtheta")[:, 0]
+        errs.append(np.abs(bal.field("vtheta")[:, 0] / v_sg - 1)[2:-2].max())
+        gap = np.abs(ref.field("vtheta")[:, 0] / v_sg - 1)[2:-2].max()
+    assert errs[0] / errs[1] > 3.0, errs
+    assert gap > 20 * errs[1], (gap, errs)
+
+
+# ------------------------------------------------------------------ accretion
+ACC_CFG = "Jupiter\t1.0\t0.001\t{acc}\t{feel}\tNO\n"
+
+
+def _accretor(acc=2.0, feel="YES", angle=2.0, nrad=64, nsec=256, **keys):
+    p, pl, o = make(nrad=nrad, nsec=nsec, cfg=ACC_CFG.format(acc=acc, feel=feel), STUBBEDPHYSICS="YES",
+                    FORCEDCIRCULAR="NO", RMIN=0.6, RMAX=1.6, **keys)
+    sysp = o.planets(Planets)
+    sysp.x[0], sysp.y[0] = math.cos(angle), math.sin(angle)
+    sysp.vx[0], sysp.vy[0] = -math.sin(angle), math.cos(angle)
+    o.set_planets(sysp)
+    return p, o
+
+
+@pytest.mark.parametrize("angle", [2.0, -3.1, 0.0])
+def test_accretion_moves_mass_from_the_roche_lobe_to_the_planet(angle):
+    dt, acc = 0.01, 2.0
+    p, o = _accretor(acc=acc, angle=angle)
+    nr, ns = int(p["NRAD"]), int(p["NSEC"])
+    d0 = o.field("dens")
+    pl0 = o.planets(Planets)
+    m0, vx0, vy0 = pl0.mass[0], pl0.vx[0], pl0.vy[0]
+    o.stage("accrete", dt)
+    d1 = o.field("dens")
+    pl1 = o.planets(Planets)
+    surf = o.ring("surf")[:nr]
+    # the planet gains exactly what the gas loses
+    dgas = ((d0 - d1) * surf[:, None]).sum()
+    assert dgas > 0
+    assert pl1.mass[0] - m0 == pytest.approx(dgas, rel=1e-10)
+    # which cells, by how much: Kley's two zones, from the cell centres
+    rm = o.ring("rmed")[:nr]
+    ang = 2 * math.pi * np.arange(ns) / ns
+    xc, yc = rm[:, None] * np.cos(ang)[None, :], rm[:, None] * np.sin(ang)[None, :]
+    rroche = (m0 / 3) ** (1 / 3)
+    dist = np.hypot(xc - math.cos(angle), yc - math.sin(angle)) / rroche
+    ratio = d1 / d0
+    f1, f2 = acc * dt / 3, 2 * acc * dt / 3
+    inner, ring = dist < 0.45, (dist >= 0.45) & (dist < 0.75)
+    assert inner.sum() > 0 and ring.sum() > inner.sum()
+    assert np.allclose(ratio[inner], (1 - f1) * (1 - f2), rtol=1e-15, atol=0)
+    assert np.allclose(ratio[ring], 1 - f1, rtol=1e-15, atol=0)
+    assert np.all(ratio[dist >= 0.75] == 1)
+    # momentum: the accreted gas orbits with the planet, so its velocity
+    # hardly changes (a sign slip in the cell velocity would change it by
+    # ~dM/M of its own size)
+    dv = math.hypot(pl1.vx[0] - vx0, pl1.vy[0] - vy0)
+    assert dv < 0.1 * dgas / m0, (dv, dgas / m0)
+    assert dv > 0
+
+
+def test_accretion_keeps_the_velocity_of_a_planet_that_does_not_feel_the_disk():
+    p, o = _accretor(feel="NO")
+    pl0 = o.planets(Planets)
+    o.stage("accrete", 0.01)
+    pl1 = o.planets(Planets)
+    assert pl1.mass[0] > pl0.mass[0]
+    assert (pl1.vx[0], pl1.vy[0]) == (pl0.vx[0], pl0.vy[0])
+
+
+def test_accretion_is_inert_without_stubbed_physics_or_rate():
+    par, _ = configs.plain_disk(nrad=24, nsec=48)
+    runs = []
+    for cfg, keys in ((ACC_CFG.format(acc=2.0, feel="YES"), {}),
+                      (ACC_CFG.format(acc=0.0, feel="YES"), {"STUBBEDPHYSICS": "YES"}),
+                      (ACC_CFG.format(acc=0.0, feel="YES"), {})):
+        p, pl = configs.load(par, cfg, FORCEDCIRCULAR="NO", **keys)
+        o = oracle.Oracle(p, pl, 1)
+        o.step(2)
+        runs.append((o.field("dens"), o.planets(Planets).mass[0]))
+    for d, m in runs[1:]:
+        assert np.array_equal(d, runs[0][0]) and m == runs[0][1] == 0.001
