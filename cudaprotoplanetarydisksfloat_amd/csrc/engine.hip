@@ -228,7 +228,8 @@ struct Batch : BatchBase {
           p.guiding_center != P0.guiding_center || p.forced_circular != P0.forced_circular ||
           p.roche_smoothing != P0.roche_smoothing || p.viscosity_alpha != P0.viscosity_alpha ||
           p.ext_fix_taurp_corner != P0.ext_fix_taurp_corner || p.ext_fix_pp_distance != P0.ext_fix_pp_distance ||
-          p.is_disk != P0.is_disk)
+          p.is_disk != P0.is_disk ||
+          (p.advect_label && p.ext_stubbed_physics) != (P0.advect_label && P0.ext_stubbed_physics))
         return set_error(FGX_EPARAM, "disk %d of the batch differs from disk 0 in a mode flag", d);
       if (p.corotating && pls[d].nb == 0) return set_error(FGX_EPARAM, "a corotating frame needs a planet");
       if (!p.is_disk) return set_error(FGX_EPARAM, "DISK NO (planets only) is not supported");
@@ -363,6 +364,8 @@ struct Batch : BatchBase {
     LCHK(f(&D.divv, ad)); LCHK(f(&D.qplus, ad)); LCHK(f(&D.energyint, ad)); LCHK(f(&D.pressure, true));
     LCHK(f(&D.t_dens, true)); LCHK(f(&D.t_rmp, true)); LCHK(f(&D.t_rmm, true));
     LCHK(f(&D.t_tmp, true)); LCHK(f(&D.t_tmm, true)); LCHK(f(&D.t_e, ad)); LCHK(f(&D.work, true));
+    const int lab = (P0.advect_label && P0.ext_stubbed_physics) ? 1 : 0;
+    LCHK(f(&D.label, lab)); LCHK(f(&D.t_lab, lab));
     const int sg = P0.self_gravity;
     LCHK(f(&D.sg_accr, sg)); LCHK(f(&D.sg_acct, sg));
     LCHK(dalloc(&D.phi, F));
@@ -453,6 +456,7 @@ struct Batch : BatchBase {
     D.sloppy = P0.sloppy_cfl; D.corotating = P0.corotating; D.guiding = P0.guiding_center;
     D.forced_circular = P0.forced_circular; D.roche = P0.roche_smoothing;
     D.write_pressure = (!ad && P0.write_temperature) ? 1 : 0;
+    D.advect_label = lab;
     D.ignore_active = 0;
     int viscous = 0;
     for (int d = 0; d < nd; ++d)
@@ -485,6 +489,7 @@ struct Batch : BatchBase {
         for (int j = 0; j < hd[d].nsec; ++j) res[(size_t)i * hd[d].nsec + j] -= vref_h[d][i];
       LCHK(upload_field(d, D.vtheta, res));
       if (ad) LCHK(upload_field(d, D.energy, hd[d].energy));
+      if (lab) LCHK(upload_field(d, D.label, hd[d].label));
     }
     // initial pressure (InitEuler's ComputePressureField), used by the
     // isothermal temperature output
@@ -499,7 +504,7 @@ struct Batch : BatchBase {
       theta_split<real>(hd[d].nsec, K, W);
       if (K == 0) fb_nsec = std::max(fb_nsec, hd[d].nsec);
     }
-    theta_shmem = (size_t)FGX_THETA_ARRAYS * fb_nsec * sizeof(real);
+    theta_shmem = (size_t)(FGX_THETA_ARRAYS + lab) * fb_nsec * sizeof(real);
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -602,13 +607,17 @@ struct Batch : BatchBase {
   // g.w: 0 = one full wave per ring, 1 = one partial wave, >= 2 = arcs
   template <int K>
   int theta_k(const ThetaGroup& g) {
+    return D.advect_label ? theta_kl<K, true>(g) : theta_kl<K, false>(g);
+  }
+  template <int K, bool LAB>
+  int theta_kl(const ThetaGroup& g) {
     if constexpr (sizeof(real) == 8 && K > 8) {
       return set_error(FGX_EDEVICE, "no FP64 azimuthal kernel for K=%d", g.k);
     } else {
-      if (g.w == 0) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 1>, g.n, 64, 0, D, g.tiles, 1);
-      if (g.w == 1) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 0>, g.n, 64, 0, D, g.tiles, 1);
+      if (g.w == 0) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 1, LAB>, g.n, 64, 0, D, g.tiles, 1);
+      if (g.w == 1) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 0, LAB>, g.n, 64, 0, D, g.tiles, 1);
       if constexpr (K >= 2 && K <= 4) {
-        return launch("trans_theta", 0, k_trans_theta_arc<real, K, 2>, g.n, 64 * g.w, 0, D, g.tiles, g.w);
+        return launch("trans_theta", 0, k_trans_theta_arc<real, K, 2, LAB>, g.n, 64 * g.w, 0, D, g.tiles, g.w);
       } else {
         return set_error(FGX_EDEVICE, "no multi-wave azimuthal kernel for K=%d", g.k);
       }
@@ -705,7 +714,10 @@ struct Batch : BatchBase {
       LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vrad, (const real*)D.vtheta, 1));
       LCHK(launch("substep3", 9, k_substep3<real>, n_stiles, FGX_TPB, 0, D));
     }
-    LCHK(launch("trans_rad", ad ? 10 : 8, k_trans_rad_strip<real>, n_stiles, FGX_TPB, 0, D, stiles, strip_rings));
+    if (D.advect_label)
+      LCHK(launch("trans_rad", ad ? 12 : 10, k_trans_rad_strip<real, true>, n_stiles, FGX_TPB, 0, D, stiles, strip_rings));
+    else
+      LCHK(launch("trans_rad", ad ? 10 : 8, k_trans_rad_strip<real, false>, n_stiles, FGX_TPB, 0, D, stiles, strip_rings));
     LCHK(trans_theta(ad ? 11 : 9));
     LCHK(launch("trans_vr", 4, k_trans_vr<real>, n_stiles, FGX_TPB, 0, D));
     LCHK(boundary());
@@ -761,6 +773,7 @@ struct Batch : BatchBase {
       case FGX_QPLUS: return D.adiabatic ? D.qplus : nullptr;
       case FGX_SG_ACCR: return D.self_gravity ? D.sg_accr : nullptr;
       case FGX_SG_ACCT: return D.self_gravity ? D.sg_acct : nullptr;
+      case FGX_LABEL: return D.advect_label ? D.label : nullptr;
       default: return nullptr;
     }
   }
@@ -779,7 +792,8 @@ struct Batch : BatchBase {
       D.ignore_active = 0;
       LCHK(r);
       src = D.work;
-    } else if (field == FGX_LABEL) {
+    } else if (field == FGX_LABEL && !src) {
+      // without label advection the label keeps its initial values
       for (size_t k = 0; k < n; ++k) tmp[k] = (real)h.label[k];
       src = nullptr;
     } else if (field == FGX_SOUNDSPEED && !D.adiabatic) {
@@ -960,6 +974,7 @@ struct Batch : BatchBase {
   std::vector<real*> ckpt_fields() {
     std::vector<real*> f = {D.dens, D.vrad, D.vtheta};
     if (D.adiabatic) f.push_back(D.energy);
+    if (D.advect_label) f.push_back(D.label);
     return f;
   }
   int checkpoint(int d, const char* path) override {

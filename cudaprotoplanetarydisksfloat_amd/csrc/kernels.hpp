@@ -26,6 +26,7 @@ struct Dev {
   real *dens, *vrad, *vtheta, *energy, *cs;
   real *vradint, *vthetaint, *taurr, *taupp, *taurp, *divv, *qplus, *energyint, *pressure;
   real *t_dens, *t_rmp, *t_rmm, *t_tmp, *t_tmm, *t_e;
+  real *label, *t_lab;   // advected label and its extensive form after the radial pass
   real *work, *sg_accr, *sg_acct;
   // ring tables, nrad+1 per disk
   real *rinf, *rsup, *rmed, *surf, *invsurf, *invrmed, *invrinf, *invdiffrmed, *invdiffrsup;
@@ -62,6 +63,7 @@ struct Dev {
   int adiabatic, fast_transport, open_inner, nonreflecting, evanescent, self_gravity;
   int cooling, indirect, exclude_hill, fix_taurp, fix_pp, fix_sgwrap, sloppy, corotating;
   int guiding, forced_circular, roche, write_pressure, ignore_active, viscous;
+  int advect_label;   // LABELADVECTION with StubbedPhysics
   // which of the buffer pairs (vrad, vtheta) / (vradint, vthetaint) the
   // batch's pointers have been swapped to since set-up, modulo 2 (the fused
   // source pass writes the new velocities into the second pair)
@@ -1203,6 +1205,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   real* sQ = sD1 + 5 * MS;                         // quantity in flight
   real* sA = sD1 + 6 * MS;                         // work = quantity / density
   real* sH = sD1 + 7 * MS;                         // held ThetaMomP
+  real* sL = sD1 + 8 * MS;                         // held extensive label (label advection)
   __shared__ double redd[FGX_TPB / 64];
   const long long row = fb + (long long)i * ns;
   const double dtd = s.dt;
@@ -1260,20 +1263,22 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   };
   int nsm = nsh % ns;
   if (nsm < 0) nsm += ns;
-  // quantity order: RadMomP, RadMomM, [energy], ThetaMomP (held in sH),
-  // ThetaMomM (left in sQ); every quantity sees the same divisors and
-  // interface densities, so the order is free here
-  const int order[5] = {0, 1, 4, 2, 3};
-  for (int o = 0; o < 5; ++o) {
+  // quantity order: RadMomP, RadMomM, [energy], [extensive label, held in
+  // sL], ThetaMomP (held in sH), ThetaMomM (left in sQ); every quantity sees
+  // the same divisors and interface densities, so the order is free here
+  const int order[6] = {0, 1, 4, 5, 2, 3};
+  for (int o = 0; o < 6; ++o) {
     const int q = order[o];
     if (q == 4 && !D.adiabatic) continue;
-    const real* src = q == 0 ? D.t_rmp : q == 1 ? D.t_rmm : q == 2 ? D.t_tmp : q == 3 ? D.t_tmm : D.t_e;
+    if (q == 5 && !D.advect_label) continue;
+    const real* src = q == 0 ? D.t_rmp : q == 1 ? D.t_rmm : q == 2 ? D.t_tmp : q == 3 ? D.t_tmm : q == 4 ? D.t_e : D.t_lab;
     for (int j = threadIdx.x; j < ns; j += FGX_TPB) sQ[j] = src[row + j];
     __syncthreads();
     pass(sD1, sS1, sV, (real)0);
     pass(sD2, sS2, nullptr, v2);
-    if (q == 2) {
-      for (int j = threadIdx.x; j < ns; j += FGX_TPB) sH[j] = sQ[j];
+    if (q == 2 || q == 5) {
+      real* hold = q == 2 ? sH : sL;
+      for (int j = threadIdx.x; j < ns; j += FGX_TPB) hold[j] = sQ[j];
       __syncthreads();
     } else if (q != 3) {
       // AdvectSHIFTKernel (Kernels.cu:1276-1289): out[j] = in[j - Nshift]
@@ -1307,6 +1312,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
     js += (js < 0) ? ns : 0;
     const int jsm = (js == 0) ? ns - 1 : js - 1;
     D.dens[row + j] = sD1[js];
+    if (D.advect_label) D.label[row + j] = sL[js] / sD1[js];   // ComputeSpeQty (Kernels.cu:1309-1319)
     const double S = (double)sD1[js] + (double)sD1[jsm];
     D.vtheta[row + j] = (real)(((double)sH[jsm] + (double)sQ[js] - l0 * 1e-15) / (S + 1e-15) / rd);
   }

@@ -52,18 +52,19 @@ __device__ __forceinline__ double lane_read(double x, int l) {
 }
 
 // ============================================================ radial sweep
-template <class real>
+template <class real, bool LAB = false>
 struct RadRing {
-  real q[7];      // 0 density (for DensStar), 1..6 work = quantity / (density + 1e-20):
-                  // density, RadMomP, RadMomM, ThetaMomP, ThetaMomM, energy
+  real q[LAB ? 8 : 7];   // 0 density (for DensStar), 1..6 work = quantity / (density + 1e-20):
+                         // density, RadMomP, RadMomM, ThetaMomP, ThetaMomM, energy, [label]
   real sg, vr0, vr1, e;
+  real x;         // extensive label, density * label (label advection)
   double tp, tm;  // angular momenta in FP64 (density * (vref + residual + R*Omega) * R)
 };
 
 // the disk's fields seen from its first element; offsets in 32 bits
 template <class real>
 struct RadFields {
-  const real *dens, *vrad, *vtheta, *energy;
+  const real *dens, *vrad, *vtheta, *energy, *label;
   const FGX_AS4 double *rmed_d, *vref;
 };
 
@@ -71,9 +72,9 @@ struct RadFields {
 // slopes / zero fluxes)
 template <class real>
 struct RadRaw {
-  real sg, vr0, vr1, vtp, vtj, e;
+  real sg, vr0, vr1, vtp, vtj, e, lab;
 };
-template <class real>
+template <class real, bool LAB = false>
 __device__ __forceinline__ void rad_fetch(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, bool ad,
                                           RadRaw<real>& w) {
   k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
@@ -84,11 +85,12 @@ __device__ __forceinline__ void rad_fetch(const RadFields<real>& f, int ns, int 
   w.vtp = f.vtheta[o + jp];
   w.vtj = f.vtheta[o + j];
   w.e = ad ? f.energy[o + j] : (real)0;
+  w.lab = LAB ? f.label[o + j] : (real)0;
 }
 
-template <class real>
+template <class real, bool LAB = false>
 __device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int k, double om, const RadRaw<real>& w,
-                                           RadRing<real>& r) {
+                                           RadRing<real, LAB>& r) {
   k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
   const real sg = w.sg;
   const double rk = f.rmed_d[k];
@@ -115,14 +117,19 @@ __device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int
   }
   r.e = w.e;
   r.q[6] = w.e * inv;
+  if constexpr (LAB) {
+    // ComputeExtQty (Kernels.cu:899-906): the label travels as density * label
+    r.x = sg * w.lab;
+    r.q[7] = r.x * inv;
+  }
 }
 
-template <class real>
+template <class real, bool LAB = false>
 __device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, double om,
-                                         bool ad, RadRing<real>& r) {
+                                         bool ad, RadRing<real, LAB>& r) {
   RadRaw<real> w;
-  rad_fetch(f, ns, nr, k, j, jp, ad, w);
-  rad_derive(f, nr, k, om, w, r);
+  rad_fetch<real, LAB>(f, ns, nr, k, j, jp, ad, w);
+  rad_derive<real, LAB>(f, nr, k, om, w, r);
 }
 
 // van Leer slope of ring k from rings k-1, k, k+1 (StarRadKernel); zero on
@@ -134,8 +141,10 @@ __device__ __forceinline__ real rad_vl(real wm, real w0, real wp, real idm, real
   return (dqp * dqm > 0) ? (real)2 * dqp * dqm * vl_rcp(dqp + dqm) : (real)0;
 }
 
-template <class real>
+// LAB: the label is advected as an eighth quantity (density * label)
+template <class real, bool LAB>
 __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
+  constexpr int NQ = LAB ? 8 : 7;
   const CellTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
   const DiskConst& c = D.dc[t.d];
@@ -154,68 +163,71 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   const real dth = (real)(2.0 * kPI / (double)ns);
   const auto rmed = RT(rmed) + rb;
   const auto idr = RT(invdiffrmed) + rb;
-  const int nq = ad ? 7 : 6;
-  const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0), RT(rmed_d) + rb,
-                          RT(vref) + rb};
+  // quantities in use: 0..5 always, 6 (energy) if adiabatic, 7 (label) if LAB
+  auto qon = [&](int q) { return q < 6 || q == 7 || ad; };
+  const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0),
+                          LAB ? D.label + fb : nullptr, RT(rmed_d) + rb, RT(vref) + rb};
   real* const o_dens = D.t_dens + fb;
   real* const o_rmp = D.t_rmp + fb;
   real* const o_rmm = D.t_rmm + fb;
   real* const o_tmp = D.t_tmp + fb;
   real* const o_tmm = D.t_tmm + fb;
   real* const o_e = D.t_e + (ad ? fb : 0);
+  real* const o_lab = LAB ? D.t_lab + fb : nullptr;
 
-  auto slope_at = [&](int k, const RadRing<real>& m, const RadRing<real>& z, const RadRing<real>& p, int q) -> real {
+  using Ring = RadRing<real, LAB>;
+  auto slope_at = [&](int k, const Ring& m, const Ring& z, const Ring& p, int q) -> real {
     if (k <= 0 || k >= nr - 1) return (real)0;
     return rad_vl(m.q[q], z.q[q], p.q[q], idr[k], idr[k + 1]);
   };
   // prologue: rings i0-2 .. i0+1
-  RadRing<real> rm2, rm1, ra, rb1;
-  rad_load(F, ns, nr, i0 - 2, j, jp, om, ad, rm2);
-  rad_load(F, ns, nr, i0 - 1, j, jp, om, ad, rm1);
-  rad_load(F, ns, nr, i0, j, jp, om, ad, ra);
-  rad_load(F, ns, nr, i0 + 1, j, jp, om, ad, rb1);
-  real slA[7], slM[7], flux[7];
+  Ring rm2, rm1, ra, rb1;
+  rad_load<real, LAB>(F, ns, nr, i0 - 2, j, jp, om, ad, rm2);
+  rad_load<real, LAB>(F, ns, nr, i0 - 1, j, jp, om, ad, rm1);
+  rad_load<real, LAB>(F, ns, nr, i0, j, jp, om, ad, ra);
+  rad_load<real, LAB>(F, ns, nr, i0 + 1, j, jp, om, ad, rb1);
+  real slA[NQ], slM[NQ], flux[NQ];
 #pragma unroll
-  for (int q = 0; q < 7; ++q) {
-    if (q >= nq) { slA[q] = slM[q] = (real)0; continue; }
+  for (int q = 0; q < NQ; ++q) {
+    if (!qon(q)) { slA[q] = slM[q] = (real)0; continue; }
     slM[q] = slope_at(i0 - 1, rm2, rm1, ra, q);
     slA[q] = slope_at(i0, rm1, ra, rb1, q);
   }
   // flux through interface i0 (zero at the inner edge)
   {
     const real v = ra.vr0;
-    real st[7];
+    real st[NQ];
 #pragma unroll
-    for (int q = 0; q < 7; ++q) {
-      if (q >= nq) { st[q] = 0; continue; }
+    for (int q = 0; q < NQ; ++q) {
+      if (!qon(q)) { st[q] = 0; continue; }
       st[q] = (v > 0) ? rm1.q[q] + (rmed[i0] - rmed[i0 - 1 < 0 ? 0 : i0 - 1] - v * dt) * (real)0.5 * slM[q]
                       : ra.q[q] - (rmed[i0 + 1] - rmed[i0] + v * dt) * (real)0.5 * slA[q];
     }
     const real a = dt * dth * RT(rinf)[rb + i0];
     flux[0] = 0;
 #pragma unroll
-    for (int q = 1; q < 7; ++q) flux[q] = (i0 == 0 || q >= nq) ? (real)0 : a * st[q] * st[0] * v;
+    for (int q = 1; q < NQ; ++q) flux[q] = (i0 == 0 || !qon(q)) ? (real)0 : a * st[q] * st[0] * v;
   }
   const auto isurf = RT(invsurf) + rb;
   // ring i+2 is fetched one iteration ahead, before ring i-1's stores: the
   // vector memory counter retires loads and stores in issue order
   RadRaw<real> nxt;
-  rad_fetch(F, ns, nr, i0 + 2, j, jp, ad, nxt);
+  rad_fetch<real, LAB>(F, ns, nr, i0 + 2, j, jp, ad, nxt);
   for (int i = i0; i < i1; ++i) {
     const RadRaw<real> cur = nxt;
-    if (i + 1 < i1) rad_fetch(F, ns, nr, i + 3, j, jp, ad, nxt);
-    RadRing<real> rc;
-    rad_derive(F, nr, i + 2, om, cur, rc);
+    if (i + 1 < i1) rad_fetch<real, LAB>(F, ns, nr, i + 3, j, jp, ad, nxt);
+    Ring rc;
+    rad_derive<real, LAB>(F, nr, i + 2, om, cur, rc);
     // interface i+1 (zero flux at the outer edge)
     const real v = rb1.vr0;
     const bool edge = (i + 1 == nr);
     const real a = edge ? (real)0 : dt * dth * RT(rinf)[rb + i + 1];
-    real fnew[7], slB[7];
+    real fnew[NQ], slB[NQ];
     real st0 = 0;
     fnew[0] = 0;
 #pragma unroll
-    for (int q = 0; q < 7; ++q) {
-      if (q >= nq) { slB[q] = 0; fnew[q] = 0; continue; }
+    for (int q = 0; q < NQ; ++q) {
+      if (!qon(q)) { slB[q] = 0; fnew[q] = 0; continue; }
       slB[q] = slope_at(i + 1, ra, rb1, rc, q);
       const real st = (v > 0) ? ra.q[q] + (rmed[i + 1] - rmed[i] - v * dt) * (real)0.5 * slA[q]
                               : rb1.q[q] - (rmed[i + 2 <= nr ? i + 2 : nr] - rmed[i + 1] + v * dt) * (real)0.5 * slB[q];
@@ -240,9 +252,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - sgx * l0i);
     o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - sgx * l0i);
     if (ad) o_e[oc] = ra.e + (flux[6] - fnew[6]) * is;
+    if constexpr (LAB) o_lab[oc] = ra.x + (flux[7] - fnew[7]) * is;
     // slide the window
 #pragma unroll
-    for (int q = 0; q < 7; ++q) { flux[q] = fnew[q]; slA[q] = slB[q]; }
+    for (int q = 0; q < NQ; ++q) { flux[q] = fnew[q]; slA[q] = slB[q]; }
     ra = rb1;
     rb1 = rc;
   }
@@ -282,7 +295,9 @@ __device__ __forceinline__ void ld_chunk(const real* __restrict__ p, real (&v)[K
 // rotations, across arcs through two LDS words per wave edge (double
 // buffered, one barrier per exchange).  Every disk's field offset is a
 // multiple of 64 elements, so each lane's chunk is aligned for vector loads.
-template <class real, int K, int WM>
+// LAB: the extensive label is carried as one more quantity and the label
+// (ComputeSpeQty, Kernels.cu:1309-1319) formed from the final density
+template <class real, int K, int WM, bool LAB>
 __global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<real> D, const RingTile* __restrict__ tiles,
                                                                         int W) {
   // WM: 2 = W arcs of one full wave each, 1 = one full wave (nsec = 64*K),
@@ -384,6 +399,8 @@ __global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<rea
     for (int o = 0; o < 5; ++o)
       if (o < nq) ld_chunk<real, K>(srcs[o] + my, Qa[o]);
   }
+  real Lq[LAB ? K : 1];
+  if constexpr (LAB) ld_chunk<real, K>(D.t_lab + my, Lq);
 
   // ring mean of vtheta (FP64 sum of the residuals)
   double acc = 0;
@@ -543,6 +560,7 @@ __global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<rea
     both(Qa[o]);
     if (o < oh) store_shift(dsts[o], Qa[o]);
   }
+  if constexpr (LAB) both(Lq);
 #pragma unroll
   for (int k = 0; k < K; ++k) { H[k] = ad ? Qa[3][k] : Qa[2][k]; Qn[k] = ad ? Qa[4][k] : Qa[3][k]; }
   // density, pass 2
@@ -554,6 +572,11 @@ __global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<rea
     apply(Dn, F, SV2);
   }
   store_shift(D.dens, Dn);
+  if constexpr (LAB) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) Lq[k] = Lq[k] / Dn[k];
+    store_shift(D.label, Lq);
+  }
   // azimuthal velocity of ComputeVelocitiesKernel at the face j-1/2:
   // (TMP[j-1] + TMM[j]) / (Sigma[j] + Sigma[j-1] + 1e-15) / R - R*Omega; with
   // the momenta stored as residuals of density * l0 this is vref + the

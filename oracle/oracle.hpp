@@ -112,7 +112,7 @@ struct Disk {
       Potential, VradInt, VthetaInt, VradNew, VthetaNew, EnergyInt, EnergyNew, DensInt,
       TemperInt, DensStar, QRStar, dq, Work, RadMomP, RadMomM, ThetaMomP, ThetaMomM,
       VthetaRes, TempShift, DRR, DPP, DRP, DivV, TAURR, TAUPP, TAURP, Qplus, CellAbscissa,
-      CellOrdinate, SG_Accr, SG_Acct, Vresidual, DT2D;
+      CellOrdinate, SG_Accr, SG_Acct, Vresidual, DT2D, ExtLabel;
   std::vector<real> VMed, Vmoy, newDT, DT1D, LostByDisk;
   std::vector<int> Nshift, NoSplitAdvection;
 
@@ -198,7 +198,7 @@ struct Disk {
                     &DensInt, &TemperInt, &DensStar, &QRStar, &dq, &Work, &RadMomP, &RadMomM,
                     &ThetaMomP, &ThetaMomM, &VthetaRes, &TempShift, &DRR, &DPP, &DRP, &DivV,
                     &TAURR, &TAUPP, &TAURP, &Qplus, &CellAbscissa, &CellOrdinate, &SG_Accr,
-                    &SG_Acct, &Vresidual, &DT2D})
+                    &SG_Acct, &Vresidual, &DT2D, &ExtLabel})
       v->assign(NG, real(0));
   }
 
@@ -1163,8 +1163,15 @@ struct Disk {
     van_leer_theta(V, ThetaMomP, dt);
     van_leer_theta(V, ThetaMomM, dt);
     if (P.adiabatic) van_leer_theta(V, Energy, dt);
+    if (advect_label()) van_leer_theta(V, ExtLabel, dt);
     van_leer_theta(V, Dens, dt);
   }
+  // Label advection (LABELADVECTION YES): the reference advects only the
+  // never-filled ExtLabel radially and leaves the extensive/specific
+  // conversions and the azimuthal pass commented out (TransportEuler.cu:28,
+  // 34,68-69,186,235-236), so the label never moves; with StubbedPhysics it is
+  // advected as FARGO does, as the density-weighted quantity Dens*Label
+  bool advect_label() const { return P.advect_label && P.ext_stubbed_physics; }
   // ComputeConstantResidualKernel (Kernels.cu:1201-1232): integer FARGO shift
   static int fargo_shift(real vmed, real invrmed, double dt, int nsec, real& ntilde_out, real& nround_out) {
     if constexpr (sizeof(real) == 4) {
@@ -1221,6 +1228,7 @@ struct Disk {
     advect_shift(ThetaMomP);
     advect_shift(ThetaMomM);
     if (P.adiabatic) advect_shift(Energy);
+    if (advect_label()) advect_shift(ExtLabel);
     advect_shift(Dens);
   }
   // Transport (TransportEuler.cu:24-35)
@@ -1235,6 +1243,10 @@ struct Disk {
         ThetaMomP[k] = Dens[k] * (Vtheta[id(i, jp1(j))] + Rmed[i] * (real)OmegaFrame) * Rmed[i];
         ThetaMomM[k] = Dens[k] * (Vtheta[k] + Rmed[i] * (real)OmegaFrame) * Rmed[i];
       }
+    // ComputeExtQty (ExtQtyKernel, Kernels.cu:899-906)
+    if (advect_label())
+      for (int i = 0; i < NRAD; ++i)
+        for (int j = 0; j < NSEC; ++j) ExtLabel[id(i, j)] = Dens[id(i, j)] * Label[id(i, j)];
     // OneWindRad (TransportEuler.cu:54-73)
     star_rad(Dens, DensStar, dt);
     DensInt = Dens;
@@ -1243,6 +1255,7 @@ struct Disk {
     van_leer_radial(ThetaMomP, dt, false);
     van_leer_radial(ThetaMomM, dt, false);
     if (P.adiabatic) van_leer_radial(Energy, dt, false);
+    if (advect_label()) van_leer_radial(ExtLabel, dt, false);
     LostMass += van_leer_radial(Dens, dt, true);
     one_wind_theta(dt);
     // ComputeVelocitiesKernel (Kernels.cu:1292-1306)
@@ -1256,6 +1269,10 @@ struct Disk {
         Vtheta[k] = (ThetaMomP[kl] + ThetaMomM[k]) / (Dens[k] + Dens[kl] + (real)1e-15) / Rmed[i] -
                     Rmed[i] * (real)OmegaFrame;
       }
+    // ComputeSpeQty (ComputeSpeQtyKernel, Kernels.cu:1309-1319)
+    if (advect_label())
+      for (int i = 0; i < NRAD; ++i)
+        for (int j = 0; j < NSEC; ++j) Label[id(i, j)] = ExtLabel[id(i, j)] / Dens[id(i, j)];
   }
 
   // CircumPlanetaryMass (SourceEuler.cu:539-553, Kernels.cu:245-261)

@@ -121,3 +121,79 @@ def test_accretion_conserves_mass_on_the_gpu(gpu):
     disk.step(3)
     assert disk.planets().mass[0] > 1e-3
     assert abs(total() / t0 - 1) < 1e-13
+
+
+# ------------------------------------------------------------- label advection
+LAB = {"LABELADVECTION": "YES", **ON}
+
+
+def _label_case(name):
+    if name == "label":
+        par, cfg = configs.plain_disk(nrad=40, nsec=128)
+        return par, cfg, dict(LAB)
+    if name == "label_adiabatic":
+        par, cfg = configs.large_disk(nrad=40, nsec=120)
+        return par, cfg, dict(LAB)
+    if name == "label_multiarc":      # two arcs of one wave each
+        par, cfg = configs.plain_disk(nrad=16, nsec=384)
+        return par, cfg, dict(LAB)
+    if name == "label_lds":           # no arc split fits: shared-memory kernel
+        par, cfg = configs.plain_disk(nrad=16, nsec=1030)
+        return par, cfg, dict(LAB)
+    raise KeyError(name)
+
+
+def _smooth_label(nr, ns, seed=3):
+    rng = np.random.RandomState(seed)
+    i = np.arange(nr + 1)[:, None] / nr
+    j = np.arange(ns)[None, :] * 2 * np.pi / ns
+    lab = 1.0 + 0.5 * np.sin(3 * j + 5 * i) + 0.2 * rng.rand(nr + 1, ns)
+    lab[nr] = 0
+    return lab
+
+
+@pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["label", "label_adiabatic", "label_multiarc", "label_lds"])
+def test_label_advection_ten_steps(gpu, case, precision):
+    par, cfg, keys = _label_case(case)
+    disk, orc, names = pair(par, cfg, precision, DT=DT_SMALL, SUBSTEPS=1, **keys)
+    nr, ns = disk.shape[0]
+    lab = _smooth_label(nr, ns)
+    disk.set_field("label", lab)
+    orc.set_field("label", disk.field("label", ghost=True))
+    l0 = disk.field("label")
+    disk.step(10)
+    orc.step(10)
+    err = compare(disk, orc, names + ("label",))
+    print(case, precision, err)
+    for n, e in err.items():
+        assert e < TOL[precision], (case, n, e)
+    assert np.abs(disk.field("label") - l0).max() > 1e-3   # it moved
+
+
+def test_label_without_stubbed_physics_stays_put(gpu):
+    """The reference's LABELADVECTION YES leaves the label as initialised."""
+    par, cfg = configs.plain_disk(nrad=32, nsec=96)
+    p, pl = configs.load(par, cfg, LABELADVECTION="YES")
+    disk = Batch(p, pl, F32)
+    l0 = disk.field("label")
+    disk.step(2)
+    assert np.array_equal(disk.field("label"), l0)
+    assert l0.max() == 1 and l0.min() == 0          # InitLabel: 1 inside the Hill sphere
+
+
+def test_label_checkpoint_restores_bit_identically(gpu, tmp_path):
+    par, cfg, keys = _label_case("label")
+    p, pl = configs.load(par, cfg, **keys)
+    a = Batch(p, pl, F32)
+    nr, ns = a.shape[0]
+    a.set_field("label", _smooth_label(nr, ns))
+    a.step(2)
+    ck = str(tmp_path / "lab.bin")
+    a.checkpoint(ck)
+    a.step(2)
+    b = Batch(p, pl, F32)
+    b.restore(ck)
+    b.step(2)
+    for n in ("dens", "vtheta", "label"):
+        assert np.array_equal(a.field(n, ghost=True), b.field(n, ghost=True)), n

@@ -196,3 +196,79 @@ def test_accretion_is_inert_without_stubbed_physics_or_rate():
         runs.append((o.field("dens"), o.planets(Planets).mass[0]))
     for d, m in runs[1:]:
         assert np.array_equal(d, runs[0][0]) and m == runs[0][1] == 0.001
+
+
+# ------------------------------------------------------------- label advection
+LAB = {"LABELADVECTION": "YES", "STUBBEDPHYSICS": "YES"}
+
+
+def test_label_pattern_rides_the_fargo_shift():
+    """A label pattern on a ring moving an integer number of sectors per step
+    comes back rolled by that integer, like the density (a missing shift of
+    the extensive label, as commented out in the reference, fails this)."""
+    p, _, o = make(NOISEAMPLITUDE=0, **LAB)
+    nr, ns = int(p["NRAD"]), int(p["NSEC"])
+    dt = 0.01
+    rmed = o.ring("rmed")
+    shift = (np.arange(nr) % 5) + 1
+    vt = np.zeros((nr + 1, ns))
+    vt[:nr] = (shift * 2 * math.pi * rmed[:nr] / (ns * dt))[:, None]
+    j = np.arange(ns)
+    dens = np.zeros((nr + 1, ns))
+    dens[:nr] = 1e-3 * (1.0 + 0.3 * np.sin(2 * math.pi * 3 * j / ns))[None, :] * (1 + 0.01 * np.arange(nr))[:, None]
+    lab = np.zeros((nr + 1, ns))
+    lab[:nr] = (np.cos(2 * math.pi * 2 * j / ns) + 2.0)[None, :] * (1 + 0.1 * np.arange(nr))[:, None]
+    o.set_field("dens", dens)
+    o.set_field("vrad", np.zeros_like(dens))
+    o.set_field("vtheta", vt)
+    o.set_field("label", lab)
+    o.stage("transport", dt)
+    want = np.stack([np.roll(lab[i], shift[i]) for i in range(nr)])
+    assert np.allclose(o.field("label"), want, rtol=1e-13, atol=0)
+
+
+def _label_run(scale=1.0, uniform=False, **keys):
+    p, _, o = make(nrad=32, nsec=96, **LAB, **keys)
+    nr, ns = int(p["NRAD"]), int(p["NSEC"])
+    rng = np.random.RandomState(11)
+    lab = np.zeros((nr + 1, ns))
+    lab[:nr] = 0.5 if uniform else rng.rand(nr, ns)
+    o.set_field("label", scale * lab)
+    return p, o
+
+
+def test_uniform_label_stays_uniform():
+    """Dens*Label is carried with the density's own fluxes, so a uniform
+    label is unchanged to rounding after full steps."""
+    p, o = _label_run(uniform=True)
+    o.step(3)
+    assert np.allclose(o.field("label"), 0.5, rtol=1e-12, atol=0)
+
+
+def test_label_mass_is_conserved_and_the_label_moves():
+    p, o = _label_run()
+    nr = int(p["NRAD"])
+    surf = o.ring("surf")[:nr]
+    l0 = o.field("label")
+    total = lambda: (o.field("dens") * o.field("label") * surf[:, None]).sum()
+    t0 = total()
+    o.step(3)
+    assert total() / t0 - 1 == pytest.approx(0.0, abs=1e-13)
+    assert np.abs(o.field("label") - l0).max() > 1e-3
+
+
+def test_label_transport_is_homogeneous():
+    """Van Leer's limiter is scale invariant: three times the label gives
+    three times the result."""
+    _, a = _label_run()
+    _, b = _label_run(scale=3.0)
+    a.step(2)
+    b.step(2)
+    assert np.allclose(b.field("label"), 3.0 * a.field("label"), rtol=1e-12, atol=0)
+
+
+def test_label_is_inert_without_stubbed_physics():
+    p, _, o = make(nrad=32, nsec=96, LABELADVECTION="YES")
+    l0 = o.field("label", ghost=True)
+    o.step(2)
+    assert np.array_equal(o.field("label", ghost=True), l0)
