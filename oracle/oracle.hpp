@@ -20,6 +20,11 @@
 //   - the SG azimuthal update at ring 0, sector 0 wraps periodically
 //     (Kernels.cu:1039 reads index -1 there).
 // Quirk switches in fgx_params (ext_fix_*) select the corrected forms.
+// With ext_stubbed_physics it also runs what the reference parses but leaves
+// stubbed, with FARGO's semantics: accretion (accrete_onto_planets), the
+// outer mass source (outer_source_mass), exact centrifugal balance
+// (centrifugal_balance) and label advection (advect_label); each is pinned
+// by closed forms and invariants in tests/test_oracle_stubbed.py.
 #pragma once
 #include "../include/fgx.h"
 
