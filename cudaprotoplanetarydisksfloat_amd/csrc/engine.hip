@@ -279,6 +279,15 @@ struct Batch : BatchBase {
       c.outer_source = (p.ext_stubbed_physics && p.outer_source_mass) ? 1 : 0;
       c.outer_vr = p.imposeddiskdrift * std::pow(h.rinf[h.nrad - 1], -(double)p.sigmaslope);
       outer_source |= c.outer_source != 0;
+      {
+        const double r0 = h.rmed[0], rn = h.rmed[h.nrad - 1];
+        c.pow_in_2f = std::pow(r0, 2.0 * (double)p.flaringindex);
+        c.pow_out_2f = std::pow(rn, 2.0 * (double)p.flaringindex);
+        c.ev_tin = 2.0 * kPI * std::pow(r0, 1.5);
+        c.ev_tout = 2.0 * kPI * std::pow(rn, 1.5);
+        c.ev_drmin = r0 * 1.25;
+        c.ev_drmax = rn * 0.84;
+      }
       c.DT = p.dt; c.masstaper = p.masstaper; c.sigmaslope = p.sigmaslope; c.flaringindex = p.flaringindex;
       c.aspectratio = p.aspectratio; c.adiabaticindex = p.adiabaticindex;
       c.thicknesssmoothing = p.thicknesssmoothing; c.rochesmoothing = p.rochesmoothing;

@@ -416,9 +416,8 @@ template <class real>
 __device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, DiskState& s, int action);
 
 template <class real>
-__device__ void substep_begin_block(const Dev<real>& D, int d, real* redr) {
-  const DiskConst& c = D.dc[d];
-  DiskState& s = D.st[d];
+__device__ void substep_begin_block(const Dev<real>& D, int d, DiskState& s, real* redr) {
+  const FGX_AS4 DiskConst& c = cref(D.dc + d);
   const int nr = c.nrad, rb = c.ro;
   const bool need_cfl = !(c.substeps_forced > 0) && !(D.sloppy && s.sub_in_dt > 0);
   real newdt = (real)1e30;
@@ -436,9 +435,14 @@ __device__ void substep_begin_block(const Dev<real>& D, int d, real* redr) {
     newdt = block_min(mn, redr);
   }
   if (threadIdx.x != 0) return;   // the rest is serial
-  // MassTaper (SourceEuler.cu:231-232)
-  double mt = s.time / (c.masstaper * 2.0 * M_PI);
-  s.masstaper = (mt > 1.0) ? 1.0 : pow(sin(mt * M_PI / 2.0), 2.0);
+  // MassTaper (SourceEuler.cu:231-232); the sine only while it ramps up
+  const double mt = s.time / (c.masstaper * 2.0 * M_PI);
+  if (mt > 1.0) {
+    s.masstaper = 1.0;
+  } else {
+    const double sn = sin(mt * M_PI / 2.0);
+    s.masstaper = sn * sn;
+  }
   const double DT = c.DT;
   int n;
   if (c.substeps_forced > 0) {
@@ -456,17 +460,18 @@ __device__ void substep_begin_block(const Dev<real>& D, int d, real* redr) {
   s.dtemp += s.dt;
   if (D.corotating) get_psys_info_dev(D, c, s, 1);
   const double tt = s.time - s.time0;
-  // ApplySubKeplerianBoundary values (SideEuler.cu:250-268), FP64
+  // ApplySubKeplerianBoundary values (SideEuler.cu:250-268), FP64; the
+  // powers of the edge radii are the disk's constants
   const double r0 = RT(rmed_d)[rb], rn = RT(rmed_d)[rb + nr - 1];
   const double fac = 1.0 + c.sigmaslope - 2.0 * c.flaringindex;
   const double h0 = aspect_ratio_dev(c, r0, tt), hn = aspect_ratio_dev(c, rn, tt);
-  s.vkep_in = 1.0 / r0 * (1.0 - fac * h0 * h0 * pow(r0, 2.0 * c.flaringindex));
-  s.vkep_out = 1.0 / rn * (1.0 - fac * hn * hn * pow(rn, 2.0 * c.flaringindex));
-  // EvanescentBoundary constants (SideEuler.cu:227-233)
-  s.ev_tin = 2.0 * kPI * pow(r0, 1.5);
-  s.ev_tout = 2.0 * kPI * pow(rn, 1.5);
-  s.ev_drmin = r0 * 1.25;
-  s.ev_drmax = rn * 0.84;
+  s.vkep_in = 1.0 / r0 * (1.0 - fac * h0 * h0 * c.pow_in_2f);
+  s.vkep_out = 1.0 / rn * (1.0 - fac * hn * hn * c.pow_out_2f);
+  // EvanescentBoundary constants (SideEuler.cu:227-233), formed on the host
+  s.ev_tin = c.ev_tin;
+  s.ev_tout = c.ev_tout;
+  s.ev_drmin = c.ev_drmin;
+  s.ev_drmax = c.ev_drmax;
 }
 
 // ================================================================= planets
@@ -539,9 +544,8 @@ __device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const doub
 // parameters, AdvanceSystemFromDisk, AdvanceSystemRK5 (Cash-Karp,
 // RungeKutta.cu:55-74), corotation and RotatePsys: one block per disk
 template <class real>
-__device__ void planets_block(const Dev<real>& D, int d, double* redd, double* tot, double* rk) {
-  const DiskConst& c = D.dc[d];
-  DiskState& s = D.st[d];
+__device__ void planets_block(const Dev<real>& D, int d, DiskState& s, double* redd, double* tot, double* rk) {
+  const FGX_AS4 DiskConst& c = cref(D.dc + d);
   const int nt = s.ntarget;
   for (int q = 0; q < 2 * nt; ++q) {
     double a = 0;
@@ -561,9 +565,14 @@ __device__ void planets_block(const Dev<real>& D, int d, double* redd, double* t
   for (int k = 0; k < nb; ++k) {
     const double xp = s.x[k], yp = s.y[k], mp = s.mass[k] * s.masstaper;
     const double pd = sqrt(xp * xp + yp * yp);
-    const double rroche = pd * pow((1.0 / 3.0 * mp), 1.0 / 3.0);
-    const double sm = D.roche ? rroche * c.rochesmoothing
-                              : c.thicknesssmoothing * aspect_ratio_dev(c, pd, tt) * pow(pd, 1.0 + c.flaringindex);
+    double sm;
+    if (D.roche) {
+      sm = pd * pow((1.0 / 3.0 * mp), 1.0 / 3.0) * c.rochesmoothing;
+    } else {
+      // pd^(1 + flaring) is pd itself for a flat disk (pow(x, 1) == x)
+      const double pw = c.flaringindex == 0.0 ? pd : pow(pd, 1.0 + c.flaringindex);
+      sm = c.thicknesssmoothing * aspect_ratio_dev(c, pd, tt) * pw;
+    }
     s.pot_x[k] = xp; s.pot_y[k] = yp; s.pot_m[k] = mp;
     s.pot_inv3[k] = 1.0 / pd / pd / pd;
     s.pot_sm2[k] = sm * sm;
@@ -613,9 +622,11 @@ __device__ void planets_block(const Dev<real>& D, int d, double* redd, double* t
         const double x = s.x[k], y = s.y[k], th = atan2(y, x), vx = s.vx[k], vy = s.vy[k];
         const double r = sqrt(x * x + y * y);
         const double om = (-y * vx + x * vy) / r / r, dth = om * dt;
-        s.x[k] = r * cos(th + dth); s.y[k] = r * sin(th + dth);
-        s.vx[k] = vx * cos(dth + th) - vy * sin(dth + th);
-        s.vy[k] = vx * sin(dth + th) + vy * cos(dth + th);
+        double sn, cs;
+        sincos(th + dth, &sn, &cs);
+        s.x[k] = r * cs; s.y[k] = r * sn;
+        s.vx[k] = vx * cs - vy * sn;
+        s.vy[k] = vx * sn + vy * cs;
       }
     }
     if (s.time < c.releasedate) {
@@ -637,9 +648,12 @@ __device__ void planets_block(const Dev<real>& D, int d, double* redd, double* t
     s.domega = onew - s.omega_frame;
     s.omega_frame = onew;
   }
-  // RotatePsys (Psys.cu:234-251), by -OmegaFrame*dt
+  // RotatePsys (Psys.cu:234-251), by -OmegaFrame*dt; a zero angle is the
+  // identity (cos 0 = 1, sin 0 = 0 leave every product exact)
   const double ang = s.omega_frame * dt;
-  const double sa = sin(ang), ca = cos(ang);
+  if (ang == 0.0) return;
+  double sa, ca;
+  sincos(ang, &sa, &ca);
   for (int k = 0; k < nb; ++k) {
     double xt = s.x[k], yt = s.y[k];
     s.x[k] = xt * ca + yt * sa; s.y[k] = -xt * sa + yt * ca;
@@ -661,10 +675,19 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_setup(Dev<real> D, int part
   __shared__ double redd[FGX_TPB / 64];
   __shared__ double tot[2 * FGX_NFORCE_T];
   __shared__ double rk[8 * 4 * FGX_MAX_PLANETS + FGX_MAX_PLANETS];
-  if (part != 2) substep_begin_block(D, d, redr);
-  if (part == 1) return;
+  // the disk's state is staged in LDS: the serial part reads and writes it
+  // field by field, and an LDS round trip is far shorter than a global one
+  __shared__ DiskState ls;
+  constexpr int NW = (int)(sizeof(DiskState) / sizeof(int));
+  for (int q = threadIdx.x; q < NW; q += FGX_TPB) reinterpret_cast<int*>(&ls)[q] = reinterpret_cast<const int*>(D.st + d)[q];
   __syncthreads();
-  planets_block(D, d, redd, tot, rk);
+  if (part != 2) substep_begin_block(D, d, ls, redr);
+  if (part != 1) {
+    __syncthreads();
+    planets_block(D, d, ls, redd, tot, rk);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < NW; q += FGX_TPB) reinterpret_cast<int*>(D.st + d)[q] = reinterpret_cast<const int*>(&ls)[q];
 }
 
 // CorrectVthetaKernel (Kernels.cu:1352-1359)
