@@ -114,6 +114,8 @@ struct Batch : BatchBase {
   const CellTile* stiles = nullptr;
   struct ThetaGroup { int k, w, n; const RingTile* tiles; };
   std::vector<ThetaGroup> theta_groups;
+  ThetaMulti theta_multi{};   // the multi-arc groups, launched together
+  int theta_multi_n = 0, theta_multi_block = 0;
   size_t F = 0, R = 0, S = 0;
   int* h_summary = nullptr;  // pinned
   int visc_dynamic = 0, alpha = 0, need_force = 1;
@@ -453,7 +455,17 @@ struct Batch : BatchBase {
         RingTile* g;
         LCHK(dalloc(&g, v.size()));
         HIPCHK(hipMemcpyAsync((void*)g, v.data(), v.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
-        theta_groups.push_back(ThetaGroup{kw / 17, kw % 17, (int)v.size(), g});
+        const int K = kw / 17, W = kw % 17;
+        if ((K == 2 || K == 3) && W >= 2 && theta_multi.ng < FGX_THETA_MULTI) {
+          ThetaMulti& M = theta_multi;
+          M.k[M.ng] = K; M.w[M.ng] = W; M.tiles[M.ng] = g;
+          M.first[M.ng] = theta_multi_n;
+          theta_multi_n += (int)v.size();
+          M.first[++M.ng] = theta_multi_n;
+          theta_multi_block = std::max(theta_multi_block, 64 * W);
+        } else {
+          theta_groups.push_back(ThetaGroup{K, W, (int)v.size(), g});
+        }
       }
       HIPCHK(hipStreamSynchronize(stream));
     }
@@ -635,7 +647,14 @@ struct Batch : BatchBase {
   int trans_theta(double bytes_per_cell) {
     group_begin("trans_theta", bytes_per_cell);
     int r = FGX_OK;
+    if (theta_multi.ng > 0) {
+      r = D.advect_label ? launch("trans_theta", 0, k_trans_theta_multi<real, true>, theta_multi_n, theta_multi_block, 0,
+                                  D, theta_multi)
+                         : launch("trans_theta", 0, k_trans_theta_multi<real, false>, theta_multi_n, theta_multi_block,
+                                  0, D, theta_multi);
+    }
     for (const ThetaGroup& g : theta_groups) {
+      if (r) break;
       switch (g.k) {
         case 0: r = launch("trans_theta", 0, k_trans_theta<real>, g.n, FGX_TPB, theta_shmem, D, g.tiles); break;
         case 1: r = theta_k<1>(g); break;

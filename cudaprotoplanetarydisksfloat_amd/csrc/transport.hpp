@@ -298,14 +298,12 @@ __device__ __forceinline__ void ld_chunk(const real* __restrict__ p, real (&v)[K
 // LAB: the extensive label is carried as one more quantity and the label
 // (ComputeSpeQty, Kernels.cu:1309-1319) formed from the final density
 template <class real, int K, int WM, bool LAB>
-__global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<real> D, const RingTile* __restrict__ tiles,
-                                                                        int W) {
+__device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTile t, int W) {
   // WM: 2 = W arcs of one full wave each, 1 = one full wave (nsec = 64*K),
   // 0 = one wave of T < 64 lanes
   constexpr bool MULTI = WM == 2;
   __shared__ real ex[2][4][16];
   __shared__ double redw[16];
-  const RingTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
   const FGX_AS4 DiskState& s = cref(D.st + t.d);
@@ -605,6 +603,36 @@ __global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<rea
     }
   }
   store_shift(D.vtheta, V);
+}
+
+template <class real, int K, int WM, bool LAB>
+__global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<real> D, const RingTile* __restrict__ tiles,
+                                                                        int W) {
+  theta_arc_ring<real, K, WM, LAB>(D, tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)], W);
+}
+
+// The multi-arc ring groups of a batch (K = 2 or 3 sectors per lane, W = 2..16
+// arcs) in one launch: group g owns blocks [first[g], first[g+1]), a multiple
+// of 64 (so each block keeps its XCD under the grouped map), and the block
+// has 64 * max W threads; the waves past a ring's W arcs end at once (a
+// barrier waits only for the waves that have not ended).
+#define FGX_THETA_MULTI 8
+struct ThetaMulti {
+  int ng;
+  int first[FGX_THETA_MULTI + 1];
+  int k[FGX_THETA_MULTI], w[FGX_THETA_MULTI];
+  const RingTile* tiles[FGX_THETA_MULTI];
+};
+template <class real, bool LAB>
+__global__ void __launch_bounds__(1024) k_trans_theta_multi(Dev<real> D, ThetaMulti M) {
+  const int b = (int)blockIdx.x;
+  int g = 0;
+  while (g + 1 < M.ng && b >= M.first[g + 1]) ++g;
+  const int W = M.w[g];
+  if ((int)threadIdx.x >= 64 * W) return;
+  const RingTile t = M.tiles[g][xcd_tile(b - M.first[g], M.first[g + 1] - M.first[g])];
+  if (M.k[g] == 3) theta_arc_ring<real, 3, 2, LAB>(D, t, W);
+  else theta_arc_ring<real, 2, 2, LAB>(D, t, W);
 }
 
 // split of a ring of nsec sectors: W arcs of 64*K sectors (K in {3, 2, 4},
