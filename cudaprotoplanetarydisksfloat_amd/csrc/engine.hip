@@ -325,6 +325,7 @@ struct Batch : BatchBase {
         tiles16 += std::ceil(hd[d].nrad / 16.0) * std::ceil(hd[d].nsec / (double)FGX_TPB);
       strip_rings = 16;
       while (strip_rings > 2 && tiles16 * 16.0 / strip_rings < 2048) strip_rings /= 2;
+      if (const char* e = std::getenv("FGX_STRIP_RINGS")) strip_rings = std::min(std::max(std::atoi(e), 2), 64);
     }
     std::vector<CellTile> st;
     std::vector<std::vector<RingTile>> tg(17 * 17);   // index K*17 + W (K = 0: shared-memory kernel)

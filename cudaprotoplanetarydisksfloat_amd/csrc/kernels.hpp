@@ -1433,24 +1433,40 @@ __global__ void __launch_bounds__(FGX_TPB) k_vel_fixup(Dev<real> D) {
 // batch summary for the host loop: active disks and the largest CFL count
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
+  static_assert(FGX_SUMMARY_HIST == 64, "the forecast is scanned by one wave");
   __shared__ int red[FGX_TPB / 64];
-  __shared__ int hist[FGX_SUMMARY_HIST];
-  for (int j = threadIdx.x; j < FGX_SUMMARY_HIST; j += FGX_TPB) hist[j] = 0;
+  __shared__ int hist[FGX_SUMMARY_HIST + 1];   // difference array of the forecast
+  for (int j = threadIdx.x; j <= FGX_SUMMARY_HIST; j += FGX_TPB) hist[j] = 0;
   __syncthreads();
   int a = 0, m = 0;
   for (int d = threadIdx.x; d < D.ndisk; d += FGX_TPB) {
     a += D.st[d].active;
     const int rem = D.st[d].active ? D.st[d].rem_est : 0;
     m = rem > m ? rem : m;
-    // forecast of the cells still active j substeps ahead, in units of 64
+    // forecast of the cells still active j substeps ahead, in units of 64:
+    // the disk adds its cells to entries [0, rem)
     const int c64 = (D.dc[d].nrad * D.dc[d].nsec + 63) / 64;
-    for (int j = 0; j < rem && j < FGX_SUMMARY_HIST; ++j) atomicAdd(&hist[j], c64);
+    const int r = rem < FGX_SUMMARY_HIST ? rem : FGX_SUMMARY_HIST;
+    if (r > 0) {
+      atomicAdd(&hist[0], c64);
+      atomicAdd(&hist[r], -c64);
+    }
   }
   a = block_sum(a, red);
   m = -block_min(-m, red);
   if (threadIdx.x == 0) { D.summary[0] = a; D.summary[1] = m; }
   __syncthreads();
-  for (int j = threadIdx.x; j < FGX_SUMMARY_HIST; j += FGX_TPB) D.summary[2 + j] = hist[j];
+  if (threadIdx.x < 64) {
+    // inclusive scan of the differences across the wave
+    const int lane = (int)threadIdx.x;
+    int v = hist[lane];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(v, o, 64);
+      if (lane >= o) v += u;
+    }
+    D.summary[2 + lane] = v;
+  }
 }
 
 // ============================================================== diagnostics
