@@ -360,7 +360,11 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   // of an integer, and dt itself only depends on that count)
   const real imind = (real)1 / mind, idxr = (real)1 / dxrad, idxt = (real)1 / dxt;
   const real csr = D.adiabatic ? (real)0 : D.cs_ring[rb + i];
-  real mn = (real)1e30;
+  // the per-cell limit CFLSECURITY / sqrt(x) only falls as x grows (sqrt and
+  // the quotient are both correctly rounded, hence monotone), so the ring's
+  // least limit is the limit of the ring's largest x, bit for bit; NaN x are
+  // skipped by both forms, and the 1e30 start survives an empty maximum
+  real xmax = 0;
 #pragma unroll 4
   for (int j = lane; j < ns; j += 64) {
     const real rs = AT(D.vtheta, i, j);
@@ -379,11 +383,14 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
     const real i4b = dvt * idxt;
     i4 = i4b > i4 ? i4b : i4;
     i4 *= (real)(4.0 * kCVNR * kCVNR);
-    const real dt2 = (real)kCFLSECURITY / sqrt(i1 * i1 + i2 * i2 + i3 * i3 + i4 * i4);
-    mn = dt2 < mn ? dt2 : mn;
+    const real x = i1 * i1 + i2 * i2 + i3 * i3 + i4 * i4;
+    xmax = x > xmax ? x : xmax;
   }
-  mn = wave_min(mn);
-  if (lane == 0) D.newdt[rb + i] = mn;
+  xmax = -wave_min(-xmax);
+  if (lane == 0) {
+    const real dt2 = (real)kCFLSECURITY / sqrt(xmax);
+    D.newdt[rb + i] = dt2 < (real)1e30 ? dt2 : (real)1e30;
+  }
 }
 
 // force targets of the next substep: the star (Pframeforce.cu via
