@@ -115,6 +115,7 @@ struct Batch : BatchBase {
   struct ThetaGroup { int k, w, n; const RingTile* tiles; };
   std::vector<ThetaGroup> theta_groups;
   ThetaMulti theta_multi{};   // the multi-arc groups, launched together
+  bool theta_arcs = true;     // FGX_THETA_ARCS=0: one wave per ring wherever it fits
   int theta_multi_n = 0, theta_multi_block = 0;
   size_t F = 0, R = 0, S = 0;
   int* h_summary = nullptr;  // pinned
@@ -329,12 +330,13 @@ struct Batch : BatchBase {
     }
     std::vector<CellTile> st;
     std::vector<std::vector<RingTile>> tg(17 * 17);   // index K*17 + W (K = 0: shared-memory kernel)
+    if (const char* e = std::getenv("FGX_THETA_ARCS")) theta_arcs = std::atoi(e) != 0;
     for (int d = 0; d < nd; ++d) {
       const HostDisk& h = hd[d];
       for (int i0 = 0; i0 < h.nrad; i0 += strip_rings)
         for (int jb = 0; jb * FGX_TPB < h.nsec; ++jb) st.push_back(CellTile{d, i0, jb, 0});
       int K = 0, W = 0;
-      theta_split<real>(h.nsec, K, W);
+      theta_split<real>(h.nsec, K, W, theta_arcs);
       for (int i = 0; i < h.nrad; ++i) tg[K * 17 + W].push_back(RingTile{d, i});
     }
     while (st.size() % FGX_TILE_PAD) st.push_back(CellTile{-1, 0, 0, 0});
@@ -523,7 +525,7 @@ struct Batch : BatchBase {
     int fb_nsec = 0;   // widest ring of the shared-memory azimuthal kernel
     for (int d = 0; d < nd; ++d) {
       int K = 0, W = 0;
-      theta_split<real>(hd[d].nsec, K, W);
+      theta_split<real>(hd[d].nsec, K, W, theta_arcs);
       if (K == 0) fb_nsec = std::max(fb_nsec, hd[d].nsec);
     }
     theta_shmem = (size_t)(FGX_THETA_ARRAYS + lab) * fb_nsec * sizeof(real);

@@ -284,9 +284,8 @@ __device__ __forceinline__ double rsqrt_d(double x) {
 }
 
 template <class real>
-__device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const DiskConst& c, const DiskState& s,
-                                                       double rmed, int j) {
-  const double x = rmed * RT(cosj_d)[c.so + j], y = rmed * RT(sinj_d)[c.so + j];
+__device__ __forceinline__ double potential_planets_xy(const Dev<real>& D, const DiskConst& c, const DiskState& s,
+                                                       double x, double y) {
   double pot = 0;
   for (int k = 0; k < c.nb; ++k) {
     const double xp = s.pot_x[k], yp = s.pot_y[k], mp = s.pot_m[k];
@@ -297,6 +296,11 @@ __device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const
   }
   pot -= s.ind_x * x + s.ind_y * y;
   return pot;
+}
+template <class real>
+__device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const DiskConst& c, const DiskState& s,
+                                                       double rmed, int j) {
+  return potential_planets_xy(D, c, s, rmed * RT(cosj_d)[c.so + j], rmed * RT(sinj_d)[c.so + j]);
 }
 
 // ============================================================ CFL, forces

@@ -97,6 +97,12 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
   real trr1 = 0, trr2 = 0, tpp1 = 0, tpp2 = 0, trp1 = 0;
   real vrc1 = 0, vrc2 = 0, vtc1 = 0;
 
+  // the sector's FP64 cosine and sine, loaded once: loaded inside the loop,
+  // each ring would wait for them and so for every load in flight
+  const double cj = RT(cosj_d)[c.so + j], sj = RT(sinj_d)[c.so + j];
+  // consumed here, so that the loop does not wait for them (and, loads
+  // retiring in order, for everything behind them) on every ring
+  asm volatile("" ::"v"(cj), "v"(sj));
   auto pressure = [&](real sg, real e, int k) -> double {
     return ad ? g1 * (double)e : (double)sg * FGX_CD(0, k);
   };
@@ -107,27 +113,35 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
     vtA1 = AT(D.vtheta, k, j);
     vtpA1 = AT(D.vtheta, k, jp);
     pA1 = pressure(sgA1, ad ? (real)AT(D.energy, k, j) : (real)0, k);
-    phiA1 = potential_planets_at(D, c, s, FGX_CD(1, k), j);
+    phiA1 = potential_planets_xy(D, c, s, FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
   }
   // past the outer edge A and B are skipped and C yields VradInt(nrad) = 0
   const int ke = i1 + 2;
-  // stage A's inputs are fetched one ring ahead, so the wave does not wait
-  // for them at the top of every iteration
-  real nsg = 0, nvr = 0, nvt = 0, nvtp = 0, ne = 0;
+  // Stage A's inputs are fetched one ring ahead, so that the wave does not
+  // wait for them at the top of every iteration.  The fetch also carries
+  // the density of column 0 of the ring (the corner density of the
+  // reference's TAURP, used by stage B one ring later): a load used in the
+  // iteration that issues it would make the wave wait for every load in
+  // flight, since loads retire in issue order.  Rings past the outer edge
+  // load the last ring (their values are never used): a fixed number of
+  // loads per iteration lets the compiler wait for exactly the ring it needs.
+  real nsg, nvr, nvt, nvtp, ne, nc0;
   auto fetch = [&](int k) {
-    if (k < nr) {
-      nsg = AT(D.dens, k, j);
-      nvr = AT(D.vrad, k, j);
-      nvt = AT(D.vtheta, k, j);
-      nvtp = AT(D.vtheta, k, jp);
-      if (ad) ne = AT(D.energy, k, j);
-    }
+    const int kk = k < nr ? k : nr - 1;
+    nsg = AT(D.dens, kk, j);
+    nvr = AT(D.vrad, kk, j);
+    nvt = AT(D.vtheta, kk, j);
+    nvtp = AT(D.vtheta, kk, jp);
+    ne = ad ? (real)AT(D.energy, kk, j) : (real)0;
+    nc0 = AT(D.dens, kk, 0);
   };
+  real c0A1 = 0;   // column-0 density of ring k-1
+  if (ks > 0) c0A1 = AT(D.dens, ks - 1, 0);
   fetch(ks);
   for (int k = ks; k <= ke; ++k) {
     // ---------------------------------------------------------- stage A
     real sg0 = 0, e0 = 0, vri0 = 0, vti0 = 0;
-    const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne;
+    const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne, cc0 = nc0;
     fetch(k + 1);
     if (k < nr) {
       sg0 = csg;
@@ -135,7 +149,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
       const real vt = cvt, vtp = cvtp;
       if (ad) e0 = ce;
       const double p = pressure(sg0, e0, k);
-      const double phi = potential_planets_at(D, c, s, FGX_CD(1, k), j);
+      const double phi = potential_planets_xy(D, c, s, FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
       if (D.write_pressure && out && k >= i0 && k < i1) AT(D.pressure, k, j) = (real)p;
       if (k > 0) {
         const double idr = FGX_CD(2, k);
@@ -181,7 +195,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
                                       (vr0 - lane_from_left(vri1)) * invdphi * FGX_CF(7, b));
         // corner density: column 0 of ring b as the reference indexes it
         // (Kernels.cu:876), or column j-1 with the quirk fix
-        const real sc = D.fix_taurp ? lane_from_left(sig) : (real)AT(D.dens, b, 0);
+        const real sc = D.fix_taurp ? lane_from_left(sig) : c0A1;
         trp0 = (real)(2.0 * 0.25) * (sig + sg2 + sc + lane_from_left(sg2)) * nu * drp;
       }
     }
@@ -250,6 +264,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
       }
     }
     // ---------------------------------------------------------- shift
+    c0A1 = cc0;
     sg4 = sg3; sg3 = sg2; sg2 = sg1; sg1 = sg0;
     e3 = e2; e2 = e1; e1 = e0;
     vri2 = vri1; vri1 = vri0; vti2 = vti1; vti1 = vti0;

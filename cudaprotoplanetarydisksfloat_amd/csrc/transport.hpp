@@ -639,10 +639,12 @@ __global__ void __launch_bounds__(1024) k_trans_theta_multi(Dev<real> D, ThetaMu
 // 2 <= W <= 16), else one wave with nsec = T*K, W = 0 when T = 64 and W = 1
 // when T < 64; K = 0 when nothing fits (shared-memory kernel)
 template <class real>
-inline void theta_split(int nsec, int& K, int& W) {
+inline void theta_split(int nsec, int& K, int& W, bool arcs = true) {
   static const int ka[] = {3, 2, 4};
   for (int k : ka)
-    if (nsec % (64 * k) == 0 && nsec / (64 * k) >= 2 && nsec / (64 * k) <= 16) { K = k; W = nsec / (64 * k); return; }
+    if (arcs && nsec % (64 * k) == 0 && nsec / (64 * k) >= 2 && nsec / (64 * k) <= 16) {
+      K = k; W = nsec / (64 * k); return;
+    }
   static const int kf[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16};
   static const int kd[] = {1, 2, 3, 4, 5, 6, 8};
   const int* ks = sizeof(real) == 4 ? kf : kd;
