@@ -222,3 +222,19 @@ def test_fine_tiling_equals_coarse(gpu, monkeypatch, case):
         out[frac] = {n: disk.field(n) for n in names}
     for n in out["0"]:
         assert np.array_equal(out["0"][n], out["1e12"][n]), (case, n)
+
+
+@pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("nsec", [384, 512, 768])
+def test_one_wave_per_ring_layout(gpu, monkeypatch, nsec, precision):
+    """FGX_THETA_ARCS=0 runs the azimuthal pass with one wave per ring (6, 8
+    or 12 sectors per lane) where the default splits the ring into arcs; both
+    layouts stay within the oracle's bar."""
+    monkeypatch.setenv("FGX_THETA_ARCS", "0")
+    par, cfg = configs.plain_disk(nrad=16, nsec=nsec)
+    disk, orc, names = pair(par, cfg, precision, DT=DT_SMALL, SUBSTEPS=1)
+    disk.step(4)
+    orc.step(4)
+    err = compare(disk, orc, names)
+    for n, e in err.items():
+        assert e < TOL[precision], (nsec, n, e)
