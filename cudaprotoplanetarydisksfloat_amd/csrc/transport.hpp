@@ -193,15 +193,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     slM[q] = slope_at(i0 - 1, rm2, rm1, ra, q);
     slA[q] = slope_at(i0, rm1, ra, rb1, q);
   }
-  // flux through interface i0 (zero at the inner edge)
+  // flux through interface i0 (zero at the inner edge).  The ring spacings
+  // are formed before the quantity loop: read inside the two upwind arms,
+  // the table loads had turned each selection into a divergent branch.
   {
     const real v = ra.vr0;
+    const real hm = rmed[i0] - rmed[i0 - 1 < 0 ? 0 : i0 - 1], hp = rmed[i0 + 1] - rmed[i0];
     real st[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       if (!qon(q)) { st[q] = 0; continue; }
-      st[q] = (v > 0) ? rm1.q[q] + (rmed[i0] - rmed[i0 - 1 < 0 ? 0 : i0 - 1] - v * dt) * (real)0.5 * slM[q]
-                      : ra.q[q] - (rmed[i0 + 1] - rmed[i0] + v * dt) * (real)0.5 * slA[q];
+      st[q] = (v > 0) ? rm1.q[q] + (hm - v * dt) * (real)0.5 * slM[q] : ra.q[q] - (hp + v * dt) * (real)0.5 * slA[q];
     }
     const real a = dt * dth * RT(rinf)[rb + i0];
     flux[0] = 0;
@@ -222,6 +224,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     const real v = rb1.vr0;
     const bool edge = (i + 1 == nr);
     const real a = edge ? (real)0 : dt * dth * RT(rinf)[rb + i + 1];
+    const real hm = rmed[i + 1] - rmed[i], hp = rmed[i + 2 <= nr ? i + 2 : nr] - rmed[i + 1];
     real fnew[NQ], slB[NQ];
     real st0 = 0;
     fnew[0] = 0;
@@ -229,8 +232,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     for (int q = 0; q < NQ; ++q) {
       if (!qon(q)) { slB[q] = 0; fnew[q] = 0; continue; }
       slB[q] = slope_at(i + 1, ra, rb1, rc, q);
-      const real st = (v > 0) ? ra.q[q] + (rmed[i + 1] - rmed[i] - v * dt) * (real)0.5 * slA[q]
-                              : rb1.q[q] - (rmed[i + 2 <= nr ? i + 2 : nr] - rmed[i + 1] + v * dt) * (real)0.5 * slB[q];
+      const real st = (v > 0) ? ra.q[q] + (hm - v * dt) * (real)0.5 * slA[q] : rb1.q[q] - (hp + v * dt) * (real)0.5 * slB[q];
       if (q == 0) st0 = st;
       else fnew[q] = edge ? (real)0 : a * st * st0 * v;
     }
