@@ -334,16 +334,21 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   };
   // left neighbours of x1 (this lane's X[K-1]) and x2 (X[K-2]), right
   // neighbour of xr (X[0])
+  // Across arcs the wave's edge values are broadcast from lanes 63 and 0 and
+  // written by every lane (one address, one value), and every lane reads its
+  // neighbour's and selects: no divergent branch around the LDS accesses.
   auto halo3 = [&](real x1, real x2, real xr, real& l1, real& l2, real& r1) {
     l1 = lane_from_left(x1);
     l2 = lane_from_left(x2);
     r1 = lane_from_right(xr);
     if constexpr (MULTI) {
-      if (lane == 63) { ex[xb][0][w] = x1; ex[xb][1][w] = x2; }
-      if (lane == 0) ex[xb][2][w] = xr;
+      const real e1 = lane_read(x1, 63), e2 = lane_read(x2, 63), er = lane_read(xr, 0);
+      ex[xb][0][w] = e1; ex[xb][1][w] = e2; ex[xb][2][w] = er;
       __syncthreads();
-      if (lane == 0) { l1 = ex[xb][0][wl]; l2 = ex[xb][1][wl]; }
-      if (lane == 63) r1 = ex[xb][2][wr];
+      const real n1 = ex[xb][0][wl], n2 = ex[xb][1][wl], nr = ex[xb][2][wr];
+      l1 = lane == 0 ? n1 : l1;
+      l2 = lane == 0 ? n2 : l2;
+      r1 = lane == 63 ? nr : r1;
       xb ^= 1;
     } else {
       fix_single(l1, l2, r1, x1, x2, xr);
@@ -362,9 +367,10 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   auto right1 = [&](real xr) -> real {
     real r = lane_from_right(xr);
     if constexpr (MULTI) {
-      if (lane == 0) ex[xb][3][w] = xr;
+      ex[xb][3][w] = lane_read(xr, 0);
       __syncthreads();
-      if (lane == 63) r = ex[xb][3][wr];
+      const real nr = ex[xb][3][wr];
+      r = lane == 63 ? nr : r;
       xb ^= 1;
     } else if constexpr (WM == 0) {
       const real e = lane_read(xr, 0);
@@ -376,9 +382,12 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
     l1 = lane_from_left(x1);
     l2 = lane_from_left(x2);
     if constexpr (MULTI) {
-      if (lane == 63) { ex[xb][0][w] = x1; ex[xb][1][w] = x2; }
+      const real e1 = lane_read(x1, 63), e2 = lane_read(x2, 63);
+      ex[xb][0][w] = e1; ex[xb][1][w] = e2;
       __syncthreads();
-      if (lane == 0) { l1 = ex[xb][0][wl]; l2 = ex[xb][1][wl]; }
+      const real n1 = ex[xb][0][wl], n2 = ex[xb][1][wl];
+      l1 = lane == 0 ? n1 : l1;
+      l2 = lane == 0 ? n2 : l2;
       xb ^= 1;
     } else if constexpr (WM == 0) {
       const real e1 = lane_read(x1, T - 1), e2 = lane_read(x2, T - 1);
