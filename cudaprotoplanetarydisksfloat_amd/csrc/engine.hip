@@ -9,6 +9,7 @@
 // two-int summary every few substeps instead of synchronising per kernel as
 // the reference does.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <chrono>
@@ -190,28 +191,34 @@ struct Batch : BatchBase {
     Pending pd{};
     const bool timed = prof_on && !in_group;
     if (timed) {
+      // the events ride on the kernel's own dispatch, so they time its
+      // execution alone, as a kernel trace does, not the launch gap too
       pd.k = prof_index(name); pd.a = get_event(); pd.b = get_event();
       pd.bytes = bytes_per_cell * sizeof(real) * cells();
-      hipEventRecord(pd.a, stream);
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, stream, pd.a, pd.b, 0, args...);
+    } else if (prof_on && in_group) {
+      // a group spans from its first kernel's start to its last kernel's end
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, stream, grp_started ? nullptr : grp.a, grp.b, 0,
+                            args...);
+      grp_started = true;
+    } else {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, stream, args...);
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, stream, args...);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(FGX_EDEVICE, "launch of %s failed: %s", name, hipGetErrorString(e));
-    if (timed) { hipEventRecord(pd.b, stream); pending.push_back(pd); }
+    if (timed) pending.push_back(pd);
     return FGX_OK;
   }
   // several launches timed as one kernel of the profile
-  bool in_group = false;
+  bool in_group = false, grp_started = false;
   Pending grp{};
   void group_begin(const char* name, double bytes_per_cell) {
-    if (prof_on) {
-      grp = Pending{prof_index(name), get_event(), get_event(), bytes_per_cell * sizeof(real) * cells()};
-      hipEventRecord(grp.a, stream);
-    }
+    if (prof_on) grp = Pending{prof_index(name), get_event(), get_event(), bytes_per_cell * sizeof(real) * cells()};
     in_group = true;
+    grp_started = false;
   }
   void group_end() {
-    if (prof_on) { hipEventRecord(grp.b, stream); pending.push_back(grp); }
+    if (prof_on && grp_started) pending.push_back(grp);
     in_group = false;
   }
 #define LCHK(x) do { int r_ = (x); if (r_) return r_; } while (0)
