@@ -52,6 +52,22 @@ def _case(name):
         par, cfg = configs.plain_disk(nrad=48, nsec=144)
         cfg = "P1\t1.0\t1e-3\t0.0\tYES\tYES\nP2\t1.6\t3e-4\t0.0\tYES\tYES\n"
         return par, cfg, {"FORCEDCIRCULAR": "NO", "EXCLUDEHILL": "YES"}
+    if name == "no_planets":
+        # the star's potential and the indirect term still act (quirk Q5).
+        # Without a planet the disk's vr is a small residual of O(1) terms
+        # that cancel (rms 1e-4 at 1% noise against 1.5e-3 with the planet),
+        # so its relative norm would measure the rounding of that
+        # cancellation; a 10% density perturbation gives it real flow
+        par, _ = configs.plain_disk(nrad=40, nsec=128, noise=1e-1)
+        return par, "", {}
+    if name == "ragged":
+        # 37 rings: partial last strips of 8 and 12 rings; 100 sectors: one
+        # partial wave of 50 lanes x 2 sectors, two source chunks of 57
+        par, cfg = configs.plain_disk(nrad=37, nsec=100)
+        return par, cfg, {}
+    if name == "tiny":
+        par, cfg = configs.plain_disk(nrad=8, nsec=32)
+        return par, cfg, {}
     if name == "self_gravity":
         par, cfg = configs.sg_disk(nrad=24, nsec=96)
         return par, cfg, {}
@@ -63,7 +79,7 @@ def _case(name):
 
 CASES = ["plain", "plain_log_alpha", "adiabatic_evanescent", "adiabatic_cooling_open", "nonreflecting",
          "nonreflecting_adiabatic", "open", "standard_transport", "guiding_center", "two_planets",
-         "self_gravity", "self_gravity_radix5"]
+         "self_gravity", "self_gravity_radix5", "no_planets", "ragged", "tiny"]
 
 
 @pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
