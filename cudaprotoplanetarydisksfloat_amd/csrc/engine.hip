@@ -116,6 +116,114 @@ struct Batch : BatchBase {
   struct ThetaGroup { int k, w, n; const RingTile* tiles; };
   std::vector<ThetaGroup> theta_groups;
   ThetaMulti theta_multi{};   // the multi-arc groups, launched together
+  // Tile lists narrowed to the disks still active in a DT.  A disk only
+  // drops out within a DT, so the disks active at a summary cover every later
+  // substep of the DT; the blocks of finished disks are then not launched at
+  // all, instead of each loading its tile and its disk's state to exit.
+  // Lists: 0 coarse strips, 1 coarse source, 2 fine strips, 3 fine source,
+  // 4 rings, then the multi-arc ring groups and the other ring groups.
+  struct TileList {
+    const int* full = nullptr;   // the whole list (device, padded)
+    int* part = nullptr;         // its narrowed copy (device)
+    int elem = 0;                // ints per tile
+    int n_full = 0;
+    std::vector<int> off, cnt;   // per disk: its range in the whole list
+    int n_part = 0;
+  };
+  std::vector<TileList> tlists;
+  bool compact_on = true;        // FGX_COMPACT=0 launches every disk's tiles
+  int covered = 0;               // disks the current lists cover
+  long long covered_cells = 0;
+  TileSet coarse_full, fine_full;
+  ThetaMulti theta_multi_full{};
+  std::vector<ThetaGroup> theta_groups_full;
+  int theta_multi_n_full = 0, n_rtiles_full = 0;
+  const RingTile* rtiles_full = nullptr;
+  CompactRange* h_ranges = nullptr;   // pinned
+  CompactRange* d_ranges = nullptr;
+  int max_ranges = 0;
+  template <class T>
+  static void tile_ranges(const std::vector<T>& v, int nd, TileList& L) {
+    L.off.assign(nd, 0); L.cnt.assign(nd, 0);
+    for (size_t k = 0; k < v.size(); ++k) {
+      const int d = v[k].d;
+      if (d < 0) continue;
+      if (L.cnt[d] == 0) L.off[d] = (int)k;
+      L.cnt[d]++;
+    }
+  }
+  template <class T>
+  int add_list(const std::vector<T>& v, const T* dev, int& idx) {
+    TileList L;
+    L.full = reinterpret_cast<const int*>(dev);
+    L.elem = (int)(sizeof(T) / sizeof(int));
+    L.n_full = (int)v.size();
+    tile_ranges(v, nd, L);
+    T* part;
+    if (int r = dalloc(&part, v.size())) return r;
+    L.part = reinterpret_cast<int*>(part);
+    idx = (int)tlists.size();
+    tlists.push_back(std::move(L));
+    return FGX_OK;
+  }
+  int list_coarse_st = -1, list_coarse_src = -1, list_fine_st = -1, list_fine_src = -1, list_rings = -1;
+  std::vector<int> list_multi, list_groups;
+  // the whole lists again (every DT starts and ends with them)
+  void lists_full() {
+    coarse = coarse_full; fine = fine_full;
+    theta_multi = theta_multi_full; theta_multi_n = theta_multi_n_full; theta_groups = theta_groups_full;
+    D.rtiles = rtiles_full; n_rtiles = n_rtiles_full;
+    covered = nd; covered_cells = total_cells;
+  }
+  // narrow every list to the disks with act[d] != 0
+  int lists_narrow(const int* act) {
+    int nr = 0;
+    auto range = [&](const int* src, int* dst, int len, int e) {
+      h_ranges[nr++] = CompactRange{src, dst, len, e};
+    };
+    for (TileList& L : tlists) {
+      int n = 0;
+      for (int d = 0; d < nd; ++d) {
+        if (!act[d] || L.cnt[d] == 0) continue;
+        range(L.full + (size_t)L.off[d] * L.elem, L.part + (size_t)n * L.elem, L.cnt[d], L.elem);
+        n += L.cnt[d];
+      }
+      const int np = (n + FGX_TILE_PAD - 1) / FGX_TILE_PAD * FGX_TILE_PAD;
+      if (np > n) range(nullptr, L.part + (size_t)n * L.elem, np - n, L.elem);
+      L.n_part = np;
+    }
+    HIPCHK(hipMemcpyAsync(d_ranges, h_ranges, nr * sizeof(CompactRange), hipMemcpyHostToDevice, stream));
+    if (int r = launch("compact", 0, k_compact, nr, FGX_TPB, 0, (const CompactRange*)d_ranges)) return r;
+    // hipMemcpyAsync reads the pinned table when the copy runs: wait for it
+    // before the table can be refilled
+    HIPCHK(hipStreamSynchronize(stream));
+    auto part = [&](int i) { return std::make_pair((const CellTile*)tlists[i].part, tlists[i].n_part); };
+    coarse.st = part(list_coarse_st).first; coarse.n_st = part(list_coarse_st).second;
+    coarse.src = part(list_coarse_src).first; coarse.n_src = part(list_coarse_src).second;
+    fine.st = part(list_fine_st).first; fine.n_st = part(list_fine_st).second;
+    fine.src = part(list_fine_src).first; fine.n_src = part(list_fine_src).second;
+    D.rtiles = (const RingTile*)tlists[list_rings].part; n_rtiles = tlists[list_rings].n_part;
+    ThetaMulti M{};
+    int first = 0;
+    for (int g = 0; g < theta_multi_full.ng; ++g) {
+      const TileList& L = tlists[list_multi[g]];
+      if (L.n_part == 0) continue;
+      M.k[M.ng] = theta_multi_full.k[g]; M.w[M.ng] = theta_multi_full.w[g];
+      M.tiles[M.ng] = (const RingTile*)L.part;
+      M.first[M.ng] = first;
+      first += L.n_part;
+      M.first[++M.ng] = first;
+    }
+    theta_multi = M; theta_multi_n = first;
+    for (size_t g = 0; g < theta_groups_full.size(); ++g) {
+      const TileList& L = tlists[list_groups[g]];
+      theta_groups[g] = ThetaGroup{theta_groups_full[g].k, theta_groups_full[g].w, L.n_part, (const RingTile*)L.part};
+    }
+    covered = 0; covered_cells = 0;
+    for (int d = 0; d < nd; ++d)
+      if (act[d]) { covered++; covered_cells += (long long)hd[d].nrad * hd[d].nsec; }
+    return FGX_OK;
+  }
   bool theta_arcs = true;     // FGX_THETA_ARCS=0: one wave per ring wherever it fits
   int theta_multi_n = 0, theta_multi_block = 0;
   size_t F = 0, R = 0, S = 0;
@@ -139,6 +247,7 @@ struct Batch : BatchBase {
     for (auto e : free_events) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (h_summary) hipHostFree(h_summary);
+    if (h_ranges) hipHostFree(h_ranges);
     if (stream) hipStreamDestroy(stream);
   }
   int size() const override { return nd; }
@@ -245,7 +354,7 @@ struct Batch : BatchBase {
       if (!p.is_disk) return set_error(FGX_EPARAM, "DISK NO (planets only) is not supported");
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipHostMalloc((void**)&h_summary, (2 + FGX_SUMMARY_HIST) * sizeof(int)));
+    HIPCHK(hipHostMalloc((void**)&h_summary, (2 + FGX_SUMMARY_HIST + nd) * sizeof(int)));
     hd.resize(nd);
     for (int d = 0; d < nd; ++d) LCHK(setup_grid_and_density(ps[d], pls[d], hd[d]));
     // self-gravity set-up (FP64, host): kernel transforms, the initial radial
@@ -436,7 +545,7 @@ struct Batch : BatchBase {
     D.dc = dcd; D.st = std_;
     LCHK(dalloc(&D.fpart, (size_t)n_rtiles * 2 * FGX_NFORCE_T));
     LCHK(dalloc(&D.rpart, (size_t)std::max(max_nrad, 1) * 44));
-    LCHK(dalloc(&D.summary, 2 + FGX_SUMMARY_HIST));
+    LCHK(dalloc(&D.summary, 2 + FGX_SUMMARY_HIST + nd));
     CellTile* ctd; RingTile* rtd;
     LCHK(dalloc(&ctd, ct.size())); LCHK(dalloc(&rtd, rt.size()));
     D.ctiles = ctd; D.rtiles = rtd;
@@ -458,6 +567,10 @@ struct Batch : BatchBase {
       HIPCHK(hipMemcpyAsync((void*)stf_d, stf.data(), stf.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync((void*)srtf_d, srtf.data(), srtf.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
       fine = TileSet{stf_d, srtf_d, (int)stf.size(), fine_strip, (int)srtf.size(), fine_src};
+      LCHK(add_list(st, (const CellTile*)std_t, list_coarse_st));
+      LCHK(add_list(srt, (const CellTile*)srt_d, list_coarse_src));
+      LCHK(add_list(stf, (const CellTile*)stf_d, list_fine_st));
+      LCHK(add_list(srtf, (const CellTile*)srtf_d, list_fine_src));
       for (int kw = 0; kw < 17 * 17; ++kw) {
         std::vector<RingTile>& v = tg[kw];
         if (v.empty()) continue;
@@ -466,6 +579,8 @@ struct Batch : BatchBase {
         LCHK(dalloc(&g, v.size()));
         HIPCHK(hipMemcpyAsync((void*)g, v.data(), v.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
         const int K = kw / 17, W = kw % 17;
+        int li = -1;
+        LCHK(add_list(v, (const RingTile*)g, li));
         if ((K == 2 || K == 3) && W >= 2 && theta_multi.ng < FGX_THETA_MULTI) {
           ThetaMulti& M = theta_multi;
           M.k[M.ng] = K; M.w[M.ng] = W; M.tiles[M.ng] = g;
@@ -473,8 +588,10 @@ struct Batch : BatchBase {
           theta_multi_n += (int)v.size();
           M.first[++M.ng] = theta_multi_n;
           theta_multi_block = std::max(theta_multi_block, 64 * W);
+          list_multi.push_back(li);
         } else {
           theta_groups.push_back(ThetaGroup{K, W, (int)v.size(), g});
+          list_groups.push_back(li);
         }
       }
       HIPCHK(hipStreamSynchronize(stream));
@@ -497,6 +614,16 @@ struct Batch : BatchBase {
     HIPCHK(hipMemcpyAsync((void*)ctd, ct.data(), ct.size() * sizeof(CellTile), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync((void*)rtd, rt.data(), rt.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync((void*)dcd, dch.data(), nd * sizeof(DiskConst), hipMemcpyHostToDevice, stream));
+    // the whole lists, and room for their narrowed copies
+    LCHK(add_list(rt, (const RingTile*)rtd, list_rings));
+    coarse_full = coarse; fine_full = fine;
+    theta_multi_full = theta_multi; theta_multi_n_full = theta_multi_n; theta_groups_full = theta_groups;
+    rtiles_full = D.rtiles; n_rtiles_full = n_rtiles;
+    covered = nd; covered_cells = total_cells;
+    max_ranges = (int)tlists.size() * (nd + 1);
+    HIPCHK(hipHostMalloc((void**)&h_ranges, max_ranges * sizeof(CompactRange)));
+    LCHK(dalloc(&d_ranges, max_ranges));
+    if (const char* e = std::getenv("FGX_COMPACT")) compact_on = std::atoi(e) != 0;
     std::vector<DiskState> sth(nd);
     for (int d = 0; d < nd; ++d) {
       DiskState& s = sth[d];
@@ -764,6 +891,7 @@ struct Batch : BatchBase {
     return FGX_OK;
   }
   int step(int n_dt) override {
+    if (covered < nd) lists_full();   // after a step that stopped on an error
     for (int k = 0; k < n_dt; ++k) {
       LCHK(launch("begin_dt", 0, k_begin_dt<real>, (nd + 63) / 64, 64, 0, D));
       if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
@@ -776,12 +904,19 @@ struct Batch : BatchBase {
         LCHK(substep());
         if (plan > 1) { --plan; continue; }
         LCHK(launch("summary", 0, k_summary<real>, 1, FGX_TPB, 0, D));
-        HIPCHK(hipMemcpyAsync(h_summary, D.summary, (2 + FGX_SUMMARY_HIST) * sizeof(int), hipMemcpyDeviceToHost,
+        HIPCHK(hipMemcpyAsync(h_summary, D.summary, (2 + FGX_SUMMARY_HIST + nd) * sizeof(int), hipMemcpyDeviceToHost,
                               stream));
         HIPCHK(hipStreamSynchronize(stream));
         if (prof_on) prof_collect();
         if (h_summary[0] == 0) break;
+        // the disks still active cover the rest of the DT
+        if (compact_on && h_summary[0] < covered) LCHK(lists_narrow(h_summary + 2 + FGX_SUMMARY_HIST));
         plan = std::max(1, h_summary[1]);
+        // look again where the forecast halves the cells the lists cover (a
+        // forecast that is off only moves that look)
+        if (compact_on)
+          for (int j = 2; j < plan && j < FGX_SUMMARY_HIST; ++j)
+            if (2 * 64LL * h_summary[2 + j] < covered_cells) { plan = j; break; }
         // forecast: cells of the disks still active j substeps ahead (units
         // of 64 cells); short strips where that is a small share of the batch
         fine_plan.assign(plan, 0);
@@ -791,7 +926,10 @@ struct Batch : BatchBase {
           fine_plan[j] = (double)act < fine_cells;
         }
       }
-      // disks that finished the DT before the batch did
+      // every later call sees every disk again; then the disks that finished
+      // the DT before the batch did
+      if (covered < nd) lists_full();
+      use_tiles(false);
       if (fused_source && !D.self_gravity) LCHK(launch("vel_fixup", 0, k_vel_fixup<real>, n_stiles, FGX_TPB, 0, D));
     }
     HIPCHK(hipStreamSynchronize(stream));
@@ -858,6 +996,7 @@ struct Batch : BatchBase {
   }
   int write_field(int d, int field, int dtype, const void* host, size_t count) override {
     if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    if (covered < nd) { lists_full(); use_tiles(false); }
     const HostDisk& h = hd[d];
     const size_t nfull = (size_t)(h.nrad + 1) * h.nsec, ncell = (size_t)h.nrad * h.nsec;
     if (count < ncell) return set_error(FGX_ERANGE, "buffer of %zu values < nrad*nsec = %zu", count, ncell);

@@ -1466,6 +1466,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
   a = block_sum(a, red);
   m = -block_min(-m, red);
   if (threadIdx.x == 0) { D.summary[0] = a; D.summary[1] = m; }
+  // which disks are active: the host narrows the tile lists to them
+  for (int d = threadIdx.x; d < D.ndisk; d += FGX_TPB) D.summary[2 + FGX_SUMMARY_HIST + d] = D.st[d].active;
   __syncthreads();
   if (threadIdx.x < 64) {
     // inclusive scan of the differences across the wave
@@ -1478,6 +1480,21 @@ __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
     }
     D.summary[2 + lane] = v;
   }
+}
+
+// ============================================================ tile lists
+// A tile list narrowed to the disks still active in a DT: range r copies len
+// tiles of e ints from src to dst, or, with src null, writes len empty tiles
+// (first int -1, the rest 0).  One block per range.
+struct CompactRange {
+  const int* src;
+  int* dst;
+  int len, e;
+};
+__global__ void __launch_bounds__(FGX_TPB) k_compact(const CompactRange* __restrict__ rg) {
+  const CompactRange r = rg[blockIdx.x];
+  const int n = r.len * r.e;
+  for (int k = threadIdx.x; k < n; k += FGX_TPB) r.dst[k] = r.src ? r.src[k] : ((k % r.e) == 0 ? -1 : 0);
 }
 
 // ============================================================== diagnostics

@@ -254,3 +254,23 @@ def test_one_wave_per_ring_layout(gpu, monkeypatch, nsec, precision):
     err = compare(disk, orc, names)
     for n, e in err.items():
         assert e < TOL[precision], (nsec, n, e)
+
+
+def test_narrowed_tile_lists_equal_whole_lists(gpu, monkeypatch):
+    """Within a DT the tile lists are narrowed to the disks still active;
+    an uneven batch (disks taking different CFL substep counts) ends bit
+    for bit as with every disk's tiles launched throughout."""
+    disks = configs.batch_disks(n=6, substeps=0)
+    loaded = [configs.load(par, cfg, NRAD=32 + 8 * (k % 3), NSEC=96 + 32 * (k % 2)) for k, (par, cfg) in enumerate(disks)]
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("FGX_COMPACT", flag)
+        b = Batch([p for p, _ in loaded], [pl for _, pl in loaded], F32)
+        b.step(3)
+        out[flag] = [[b.field(n, disk=k) for n in ("dens", "vrad", "vtheta")] for k in range(len(loaded))]
+        counts = [b.info(k).substeps for k in range(len(loaded))]
+    assert len(set(counts)) > 1, counts          # the disks do take different counts
+    for k in range(len(loaded)):
+        for a, c in zip(out["0"][k], out["1"][k]):
+            assert np.array_equal(a, c), k
+
