@@ -461,7 +461,7 @@ struct Batch : BatchBase {
     // separate kernels; FGX_SOURCE_RINGS sets the strip height
     if (const char* e = std::getenv("FGX_FUSED_SOURCE")) fused_source = std::atoi(e) != 0;
     if (const char* e = std::getenv("FGX_SOURCE_RINGS")) src_rings = std::atoi(e);
-    src_rings = std::min(std::max(src_rings, 1), 56);   // k_source stages rs + 8 rings in 64 lanes
+    src_rings = std::min(std::max(src_rings, 1), FGX_SRC_RINGS_MAX);   // k_source stages rs + 8 rings
     std::vector<CellTile> srt;
     for (int d = 0; d < nd; ++d)
       for (int i0 = 0; i0 < hd[d].nrad; i0 += src_rings)
@@ -469,7 +469,9 @@ struct Batch : BatchBase {
     while (srt.size() % FGX_TILE_PAD) srt.push_back(CellTile{-1, 0, 0, 0});
     n_srctiles = (int)srt.size();
     // the fine tiling: 2-ring strips, 4-ring source strips
-    const int fine_strip = std::min(2, strip_rings), fine_src = std::min(4, src_rings);
+    int fine_src = std::min(4, src_rings);
+    if (const char* e = std::getenv("FGX_FINE_SOURCE_RINGS")) fine_src = std::min(std::max(std::atoi(e), 1), src_rings);
+    const int fine_strip = std::min(2, strip_rings);
     std::vector<CellTile> stf, srtf;
     for (int d = 0; d < nd; ++d) {
       total_cells += (long long)hd[d].nrad * hd[d].nsec;

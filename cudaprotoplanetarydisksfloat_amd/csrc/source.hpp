@@ -21,13 +21,18 @@
 // separate kernels, operation for operation.
 #pragma once
 #include "transport.hpp"
+#ifndef FGX_SRC_ATTR
+#define FGX_SRC_ATTR
+#endif
 
 namespace fgx {
 
 #define FGX_SRC_OUT 57   // sectors stored per wave: lanes 4..60
+#define FGX_SRC_STAGE 32  // rings of coefficients staged per strip: rs + 8
+#define FGX_SRC_RINGS_MAX (FGX_SRC_STAGE - 8)
 
 template <class real>
-__global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
+__global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
   const CellTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
@@ -56,12 +61,12 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
   // per ring: inside the loop they are read from LDS into vector registers,
   // instead of some fifty per-ring values and their table addresses being
   // held in scalar registers across the loop
-  __shared__ double cd_[8][64];   // cs2_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d invrmed_d
-  __shared__ real cf_[10][64];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed
+  __shared__ double cd_[8][FGX_SRC_STAGE];   // cs2_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d invrmed_d
+  __shared__ real cf_[10][FGX_SRC_STAGE];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed
   const int rlo = i0 - 4;
   {
     const int r = rlo + lane;
-    if (r >= 0 && r < nr && lane < rs + 8) {
+    if (r >= 0 && r < nr && lane < rs + 8 && lane < FGX_SRC_STAGE) {
       cd_[0][lane] = RT(cs2_d)[rb + r];
       cd_[1][lane] = RT(rmed_d)[rb + r];
       cd_[2][lane] = RT(invdiffrmed_d)[rb + r];
@@ -103,6 +108,31 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
   // consumed here, so that the loop does not wait for them (and, loads
   // retiring in order, for everything behind them) on every ring
   asm volatile("" ::"v"(cj), "v"(sj));
+  // planet 0's potential constants and the indirect term, read once: in the
+  // ring loop they would be fetched again, and waited for, on every ring.
+  // The arithmetic is potential_planets_xy's, in the same order.
+  const int nb = c.nb;
+  const bool indirect = D.indirect;
+  const double px0 = s.pot_x[0], py0 = s.pot_y[0], pm0 = s.pot_m[0], ps0 = s.pot_sm2[0], pi0 = s.pot_inv3[0];
+  const double indx = s.ind_x, indy = s.ind_y;
+  auto potential = [&](double x, double y) -> double {
+    double pot = 0;
+    if (nb > 0) {
+      const double dist = (x - px0) * (x - px0) + (y - py0) * (y - py0);
+      double p = -pm0 * rsqrt_d(dist + ps0);
+      if (indirect) p += pm0 * pi0 * (x * px0 + y * py0);
+      pot += p;
+    }
+    for (int q = 1; q < nb; ++q) {
+      const double xp = s.pot_x[q], yp = s.pot_y[q], mp = s.pot_m[q];
+      const double dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
+      double p = -mp * rsqrt_d(dist + s.pot_sm2[q]);
+      if (indirect) p += mp * s.pot_inv3[q] * (x * xp + y * yp);
+      pot += p;
+    }
+    pot -= indx * x + indy * y;
+    return pot;
+  };
   auto pressure = [&](real sg, real e, int k) -> double {
     return ad ? g1 * (double)e : (double)sg * FGX_CD(0, k);
   };
@@ -113,7 +143,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
     vtA1 = AT(D.vtheta, k, j);
     vtpA1 = AT(D.vtheta, k, jp);
     pA1 = pressure(sgA1, ad ? (real)AT(D.energy, k, j) : (real)0, k);
-    phiA1 = potential_planets_xy(D, c, s, FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
+    phiA1 = potential(FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
   }
   // past the outer edge A and B are skipped and C yields VradInt(nrad) = 0
   const int ke = i1 + 2;
@@ -149,7 +179,7 @@ __global__ void __launch_bounds__(64) k_source(Dev<real> D, const CellTile* __re
       const real vt = cvt, vtp = cvtp;
       if (ad) e0 = ce;
       const double p = pressure(sg0, e0, k);
-      const double phi = potential_planets_xy(D, c, s, FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
+      const double phi = potential(FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
       if (D.write_pressure && out && k >= i0 && k < i1) AT(D.pressure, k, j) = (real)p;
       if (k > 0) {
         const double idr = FGX_CD(2, k);
