@@ -16,9 +16,12 @@
 // radial neighbours from the registers of the previous iterations, so the
 // intermediate fields (potential, VradInt, VthetaInt, the stresses) never
 // reach memory.  Each stage needs one more lane on each side than the next,
-// hence the four extra lanes, and one more ring below and above, hence the
-// three extra rings on each end of a strip.  The arithmetic is that of the
-// separate kernels, operation for operation.
+// hence the four extra lanes.  Radially, the stored rings [i0, i1) need
+// stage C's VradInt on rings i0-1 .. i1, which needs the stresses of rings
+// i0-2 .. i1 (only their TAU_rr, TAU_pp below i0-1) and so stage A on rings
+// i0-2 .. i1+1, with ring i0-3's pressure and potential for A's radial
+// differences.  The arithmetic is that of the separate kernels, operation
+// for operation.
 #pragma once
 #include "transport.hpp"
 #ifndef FGX_SRC_ATTR
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   auto pressure = [&](real sg, real e, int k) -> double {
     return ad ? g1 * (double)e : (double)sg * FGX_CD(0, k);
   };
-  const int ks = (i0 - 3 > 0) ? i0 - 3 : 0;
+  const int ks = (i0 - 2 > 0) ? i0 - 2 : 0;
   if (ks > 0) {   // prime stage A with ring ks-1
     const int k = ks - 1;
     sgA1 = AT(D.dens, k, j);
@@ -173,7 +176,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     real sg0 = 0, e0 = 0, vri0 = 0, vti0 = 0;
     const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne, cc0 = nc0;
     fetch(k + 1);
-    if (k < nr) {
+    if (k < nr && k <= i1 + 1) {
       sg0 = csg;
       const real vr = cvr;
       const real vt = cvt, vtp = cvtp;
@@ -206,7 +209,9 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     // ---------------------------------------------------------- stage B
     const int b = k - 1;
     real trr0 = 0, tpp0 = 0, trp0 = 0;
-    if (visc && b >= 0 && b < nr && b >= i0 - 2) {
+    // ring i0-2's TAU_rp reads stage A's ring i0-3, which is not formed:
+    // it is never used (stage C's VthetaInt of ring i0-1 is not stored)
+    if (visc && b >= 0 && b < nr && b >= i0 - 2 && b <= i1) {
       const real irm = FGX_CF(2, b);
       const real vr0 = vri1, vr1 = vri0;
       const real vt0 = vti1, vtp = lane_from_right(vti1);
