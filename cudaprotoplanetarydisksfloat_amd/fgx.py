@@ -238,6 +238,10 @@ class Batch:
     def step(self, n_dt=1):
         _check(lib.fgx_batch_step(self.h, n_dt))
 
+    def freeze(self, disk, frozen=True):
+        """Skip (or resume) one disk in every later DT."""
+        _check(lib.fgx_batch_freeze(self.h, disk, int(bool(frozen))))
+
     def info(self, disk=0):
         out = DiskInfo()
         _check(lib.fgx_batch_info(self.h, disk, ctypes.byref(out)))

@@ -274,3 +274,20 @@ def test_narrowed_tile_lists_equal_whole_lists(gpu, monkeypatch):
         for a, c in zip(out["0"][k], out["1"][k]):
             assert np.array_equal(a, c), k
 
+
+def test_frozen_disk_is_skipped(gpu):
+    """A frozen disk keeps its fields and DT count while the batch steps;
+    the others end as without it."""
+    disks = configs.batch_disks(n=3, substeps=0)
+    loaded = [configs.load(par, cfg, NRAD=24, NSEC=64) for par, cfg in disks]
+    b = Batch([p for p, _ in loaded], [pl for _, pl in loaded], F32)
+    b.step(1)
+    b.freeze(1, True)
+    d1 = b.field("dens", disk=1)
+    b.step(2)
+    assert np.array_equal(b.field("dens", disk=1), d1)
+    assert [b.info(k).dt_steps for k in range(3)] == [3, 1, 3]
+    solo = Batch(*loaded[2], F32)
+    solo.step(3)
+    assert np.array_equal(b.field("dens", disk=2), solo.field("dens"))
+
