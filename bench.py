@@ -204,6 +204,7 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": traffic_src,
                 "alg_bytes_per_launch": round(d["alg_bytes_per_launch"]),
+                "kernel_ms_per_step": round(sub_ms / args.steps, 3),
                 "avg_us": round(d["avg_us"], 2),
                 "kernels": {k: {"avg_us": round(v["avg_us"], 2), "share": round(v["ms"] / sub_ms, 3),
                                 "GBs": round(v["alg_bytes_per_launch"] / (v["avg_us"] * 1e-6) / 1e9, 1)}

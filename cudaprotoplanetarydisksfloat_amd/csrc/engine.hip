@@ -194,9 +194,9 @@ struct Batch : BatchBase {
     }
     HIPCHK(hipMemcpyAsync(d_ranges, h_ranges, nr * sizeof(CompactRange), hipMemcpyHostToDevice, stream));
     if (int r = launch("compact", 0, k_compact, nr, FGX_TPB, 0, (const CompactRange*)d_ranges)) return r;
-    // hipMemcpyAsync reads the pinned table when the copy runs: wait for it
-    // before the table can be refilled
-    HIPCHK(hipStreamSynchronize(stream));
+    // hipMemcpyAsync reads the pinned table when the copy runs; the table is
+    // only refilled after a later summary has been waited for, which the
+    // stream orders after this copy
     auto part = [&](int i) { return std::make_pair((const CellTile*)tlists[i].part, tlists[i].n_part); };
     coarse.st = part(list_coarse_st).first; coarse.n_st = part(list_coarse_st).second;
     coarse.src = part(list_coarse_src).first; coarse.n_src = part(list_coarse_src).second;

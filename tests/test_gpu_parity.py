@@ -290,4 +290,3 @@ def test_frozen_disk_is_skipped(gpu):
     solo = Batch(*loaded[2], F32)
     solo.step(3)
     assert np.array_equal(b.field("dens", disk=2), solo.field("dens"))
-
