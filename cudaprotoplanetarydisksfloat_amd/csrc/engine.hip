@@ -888,7 +888,6 @@ struct Batch : BatchBase {
     LCHK(trans_theta(ad ? 11 : 9));
     LCHK(launch("trans_vr", 4, k_trans_vr<real>, n_stiles, FGX_TPB, 0, D));
     LCHK(boundary());
-    LCHK(launch("end_substep", 0, k_end_substep<real>, nd, 64, 0, D));
     use_tiles(false);
     return FGX_OK;
   }
@@ -928,8 +927,10 @@ struct Batch : BatchBase {
           fine_plan[j] = (double)act < fine_cells;
         }
       }
-      // every later call sees every disk again; then the disks that finished
-      // the DT before the batch did
+      // the end of every disk's last substep, then every later call sees
+      // every disk again; then the disks that finished the DT before the
+      // batch did
+      LCHK(launch("finish_dt", 0, k_finish_dt<real>, nd, 64, 0, D));
       if (covered < nd) lists_full();
       use_tiles(false);
       if (fused_source && !D.self_gravity) LCHK(launch("vel_fixup", 0, k_vel_fixup<real>, n_stiles, FGX_TPB, 0, D));

@@ -128,7 +128,8 @@ struct DiskState {
   double lost_mass, mdcp1, exces_mdcp;
   double ev_tin, ev_tout, ev_drmin, ev_drmax;  // EvanescentBoundary constants
   long long substeps, dt_steps;
-  int ncfl, active, sub_in_dt, health, rem_est;
+  int ncfl, active, sub_in_dt, health;
+  int pending;                          // a substep ran and its end-of-substep update is still due
   int nr_iangle, nr_iangle2;            // non-reflecting shifts of this call
   int frozen;                           // skipped by every later DT (served request done)
   int vpar;                             // velocity buffer pair holding this disk's Vrad/Vtheta

@@ -187,6 +187,15 @@ __device__ __forceinline__ double fviscosity_dev(const DiskConst& c, int alpha, 
   return nu;
 }
 
+// between two substeps (before k_substep_setup): a disk still taking
+// substeps in this DT, i.e. active, and its pending substep (if any) has not
+// reached the DT (that substep's end-of-substep update is applied where the
+// next substep starts).  Within a substep, after k_substep_setup, the active
+// flag alone says it.
+__device__ __forceinline__ bool disk_running(const DiskConst& c, const DiskState& s) {
+  return s.active && !(s.pending && !(s.dtemp < 0.999999999 * c.DT));
+}
+
 #define FGX_CELL_PROLOGUE                                                   \
   const CellTile t = D.ctiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
   if (t.d < 0) return;                                                      \
@@ -316,7 +325,9 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
   const FGX_AS4 DiskState& s = cref(D.st + t.d);
-  if (!D.ignore_active && !s.active) return;
+  // mode 0 runs before k_substep_setup, where the disk's previous substep may
+  // still be pending; mode 1 runs after it, within the substep
+  if (!D.ignore_active && !(mode == 0 ? disk_running(c, s) : s.active)) return;
   if (mode == 1 && !c.accretes) return;
   const int ns = c.nsec, i = t.i, rb = c.ro;
   const long long fb = c.fo;
@@ -418,6 +429,36 @@ __device__ void set_force_targets(const Dev<real>& D, const DiskConst& c, DiskSt
     nt++;
   }
   s.ntarget = nt;
+}
+
+// The end of a substep (SourceEuler.cu:297-298) and the DT loop condition
+// (SourceEuler.cu:230), LostMass accumulation (TransportEuler.cu:71).  It is
+// applied where the disk's next substep starts (k_substep_setup) or at the
+// end of the DT (k_finish_dt), not in a launch of its own: until then the
+// substep is pending, and a pending disk whose dtemp has reached DT counts as
+// done (disk_running).
+template <class real>
+__device__ void end_substep_state(const Dev<real>& D, const DiskConst& c, DiskState& s, double lost) {
+  s.lost_mass += lost;
+  s.time += s.dt;
+  s.substeps++;
+  s.sub_in_dt++;
+  if (!(s.dtemp < 0.999999999 * c.DT)) {
+    s.active = 0;
+    s.dt_steps++;
+    s.vpar = D.vpar;   // later swaps of the DT leave this disk's velocities behind
+  }
+  s.pending = 0;
+}
+// the lost mass of the pending substep: ring 0's outflow, summed by one wave
+template <class real>
+__device__ double lost_sum_wave(const Dev<real>& D, const DiskConst& c) {
+  double lost = 0;
+  if (D.open_inner) {
+    for (int j = threadIdx.x; j < c.nsec; j += 64) lost += (double)D.lost[c.so + j];
+    lost = wave_sum(lost);
+  }
+  return lost;
 }
 
 // ============================================================ substep begin
@@ -692,10 +733,34 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_setup(Dev<real> D, int part
   constexpr int NW = (int)(sizeof(DiskState) / sizeof(int));
   for (int q = threadIdx.x; q < NW; q += FGX_TPB) reinterpret_cast<int*>(&ls)[q] = reinterpret_cast<const int*>(D.st + d)[q];
   __syncthreads();
-  if (part != 2) substep_begin_block(D, d, ls, redr);
-  if (part != 1) {
+  const FGX_AS4 DiskConst& c = cref(D.dc + d);
+  bool run = true;
+  if (part != 2 && ls.pending) {
+    // the end of the disk's previous substep
+    if (threadIdx.x < 64) {
+      const double lost = lost_sum_wave(D, c);
+      if (threadIdx.x == 0) end_substep_state(D, c, ls, lost);
+    }
     __syncthreads();
-    planets_block(D, d, ls, redd, tot, rk);
+    run = ls.active;
+  }
+  if (run) {
+    if (part != 2) {
+      substep_begin_block(D, d, ls, redr);
+      if (threadIdx.x == 0) ls.pending = 1;
+    }
+    if (part != 1) {
+      __syncthreads();
+      planets_block(D, d, ls, redd, tot, rk);
+      // the force targets of the disk's next substep, at the time that
+      // substep starts
+      if (threadIdx.x == 0) {
+        const double t0 = ls.time;
+        ls.time = t0 + ls.dt;
+        set_force_targets(D, c, ls);
+        ls.time = t0;
+      }
+    }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < NW; q += FGX_TPB) reinterpret_cast<int*>(D.st + d)[q] = reinterpret_cast<const int*>(&ls)[q];
@@ -1391,35 +1456,20 @@ __global__ void k_begin_dt(Dev<real> D) {
   if (s.frozen) { s.active = 0; return; }
   s.dtemp = 0;
   s.sub_in_dt = 0;
+  s.pending = 0;
   s.active = 1;
   set_force_targets(D, D.dc[d], s);
 }
-// end of a substep (SourceEuler.cu:297-298) and the DT loop condition
-// (SourceEuler.cu:230); LostMass accumulation (TransportEuler.cu:71)
+// the end of each disk's last substep of the DT (end_substep_state): one
+// wave per disk
 template <class real>
-__global__ void __launch_bounds__(64) k_end_substep(Dev<real> D) {
+__global__ void __launch_bounds__(64) k_finish_dt(Dev<real> D) {
   const int d = blockIdx.x;
   const DiskConst& c = D.dc[d];
   DiskState& s = D.st[d];
-  if (!s.active) return;
-  double lost = 0;
-  if (D.open_inner) {
-    for (int j = threadIdx.x; j < c.nsec; j += 64) lost += (double)D.lost[c.so + j];
-    lost = wave_sum(lost);
-  }
-  if (threadIdx.x != 0) return;
-  s.lost_mass += lost;
-  s.time += s.dt;
-  s.substeps++;
-  s.sub_in_dt++;
-  s.rem_est = D.sloppy ? s.ncfl - s.sub_in_dt : s.ncfl - 1;
-  if (!(s.dtemp < 0.999999999 * c.DT)) {
-    s.active = 0;
-    s.dt_steps++;
-    s.vpar = D.vpar;   // later swaps of the DT leave this disk's velocities behind
-  } else {
-    set_force_targets(D, c, s);
-  }
+  if (!s.pending) return;
+  const double lost = lost_sum_wave(D, c);
+  if (threadIdx.x == 0) end_substep_state(D, c, s, lost);
 }
 // end of a DT with the fused source pass: a disk that stopped taking
 // substeps before the batch did holds its velocities in the buffer pair that
@@ -1451,8 +1501,12 @@ __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
   __syncthreads();
   int a = 0, m = 0;
   for (int d = threadIdx.x; d < D.ndisk; d += FGX_TPB) {
-    a += D.st[d].active;
-    const int rem = D.st[d].active ? D.st[d].rem_est : 0;
+    const DiskState& st = D.st[d];
+    const int act = disk_running(D.dc[d], st) ? 1 : 0;
+    a += act;
+    // substeps left after the pending one
+    const int sub = st.sub_in_dt + st.pending;
+    const int rem = act ? (D.sloppy ? st.ncfl - sub : st.ncfl - 1) : 0;
     m = rem > m ? rem : m;
     // forecast of the cells still active j substeps ahead, in units of 64:
     // the disk adds its cells to entries [0, rem)
@@ -1467,7 +1521,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_summary(Dev<real> D) {
   m = -block_min(-m, red);
   if (threadIdx.x == 0) { D.summary[0] = a; D.summary[1] = m; }
   // which disks are active: the host narrows the tile lists to them
-  for (int d = threadIdx.x; d < D.ndisk; d += FGX_TPB) D.summary[2 + FGX_SUMMARY_HIST + d] = D.st[d].active;
+  for (int d = threadIdx.x; d < D.ndisk; d += FGX_TPB) D.summary[2 + FGX_SUMMARY_HIST + d] = disk_running(D.dc[d], D.st[d]);
   __syncthreads();
   if (threadIdx.x < 64) {
     // inclusive scan of the differences across the wave
