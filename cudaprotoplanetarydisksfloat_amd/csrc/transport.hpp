@@ -176,9 +176,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   real* const o_lab = LAB ? D.t_lab + fb : nullptr;
 
   using Ring = RadRing<real, LAB>;
-  auto slope_at = [&](int k, const Ring& m, const Ring& z, const Ring& p, int q) -> real {
-    if (k <= 0 || k >= nr - 1) return (real)0;
-    return rad_vl(m.q[q], z.q[q], p.q[q], idr[k], idr[k + 1]);
+  // a ring's slope coefficients, read once for all quantities (read inside
+  // each quantity's edge test, they were fetched and waited for again per
+  // quantity); the slope is zero on the first and last ring
+  struct SlopeRing { real im, ip; bool inner; };
+  auto slope_ring = [&](int k) -> SlopeRing {
+    const int kc = k < 0 ? 0 : (k < nr - 1 ? k : (nr >= 2 ? nr - 2 : 0));
+    return SlopeRing{idr[kc], idr[kc + 1], k > 0 && k < nr - 1};
+  };
+  auto slope_at = [&](const SlopeRing& g, const Ring& m, const Ring& z, const Ring& p, int q) -> real {
+    const real sl = rad_vl(m.q[q], z.q[q], p.q[q], g.im, g.ip);
+    return g.inner ? sl : (real)0;
   };
   // prologue: rings i0-2 .. i0+1
   Ring rm2, rm1, ra, rb1;
@@ -187,11 +195,12 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   rad_load<real, LAB>(F, ns, nr, i0, j, jp, om, ad, ra);
   rad_load<real, LAB>(F, ns, nr, i0 + 1, j, jp, om, ad, rb1);
   real slA[NQ], slM[NQ], flux[NQ];
+  const SlopeRing gm = slope_ring(i0 - 1), ga = slope_ring(i0);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     if (!qon(q)) { slA[q] = slM[q] = (real)0; continue; }
-    slM[q] = slope_at(i0 - 1, rm2, rm1, ra, q);
-    slA[q] = slope_at(i0, rm1, ra, rb1, q);
+    slM[q] = slope_at(gm, rm2, rm1, ra, q);
+    slA[q] = slope_at(ga, rm1, ra, rb1, q);
   }
   // flux through interface i0 (zero at the inner edge).  The ring spacings
   // are formed before the quantity loop: read inside the two upwind arms,
@@ -220,10 +229,15 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     if (i + 1 < i1) rad_fetch<real, LAB>(F, ns, nr, i + 3, j, jp, ad, nxt);
     Ring rc;
     rad_derive<real, LAB>(F, nr, i + 2, om, cur, rc);
-    // interface i+1 (zero flux at the outer edge)
+    // interface i+1 (zero flux at the outer edge); the ring values this
+    // iteration needs are read together, ahead of their use
     const real v = rb1.vr0;
     const bool edge = (i + 1 == nr);
-    const real a = edge ? (real)0 : dt * dth * RT(rinf)[rb + i + 1];
+    const SlopeRing gb = slope_ring(i + 1);
+    const real rinf1 = RT(rinf)[rb + (edge ? i : i + 1)];
+    const real is = isurf[i];
+    const double rd = F.rmed_d[i], vref_i = F.vref[i];
+    const real a = edge ? (real)0 : dt * dth * rinf1;
     const real hm = rmed[i + 1] - rmed[i], hp = rmed[i + 2 <= nr ? i + 2 : nr] - rmed[i + 1];
     real fnew[NQ], slB[NQ];
     real st0 = 0;
@@ -231,13 +245,12 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       if (!qon(q)) { slB[q] = 0; fnew[q] = 0; continue; }
-      slB[q] = slope_at(i + 1, ra, rb1, rc, q);
+      slB[q] = slope_at(gb, ra, rb1, rc, q);
       const real st = (v > 0) ? ra.q[q] + (hm - v * dt) * (real)0.5 * slA[q] : rb1.q[q] - (hp + v * dt) * (real)0.5 * slB[q];
       if (q == 0) st0 = st;
       else fnew[q] = edge ? (real)0 : a * st * st0 * v;
     }
     // cell i: quantity + (flux in - flux out) / Surf
-    const real is = isurf[i];
     const real inc = (flux[1] - fnew[1]) * is;
     const real sg = ra.sg;
     const int oc = i * ns + j;
@@ -249,8 +262,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     // density taken before its FP32 rounding, so the azimuthal velocity
     // rebuilt from them carries no trace of that rounding
     const double sgx = (double)sg + (double)inc;
-    const double rd = F.rmed_d[i];
-    const double l0i = (F.vref[i] + rd * om) * rd;
+    const double l0i = (vref_i + rd * om) * rd;
     o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - sgx * l0i);
     o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - sgx * l0i);
     if (ad) o_e[oc] = ra.e + (flux[6] - fnew[6]) * is;
