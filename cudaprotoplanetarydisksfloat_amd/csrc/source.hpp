@@ -116,8 +116,11 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   // The arithmetic is potential_planets_xy's, in the same order.
   const int nb = c.nb;
   const bool indirect = D.indirect;
-  const double px0 = s.pot_x[0], py0 = s.pot_y[0], pm0 = s.pot_m[0], ps0 = s.pot_sm2[0], pi0 = s.pot_inv3[0];
-  const double indx = s.ind_x, indy = s.ind_y;
+  double px0 = s.pot_x[0], py0 = s.pot_y[0], pm0 = s.pot_m[0], ps0 = s.pot_sm2[0], pi0 = s.pot_inv3[0];
+  double indx = s.ind_x, indy = s.ind_y;
+  // held in scalar registers: left as plain loads, the compiler re-reads
+  // some of them from memory inside the loop and waits for them on every ring
+  asm("" : "+s"(px0), "+s"(py0), "+s"(pm0), "+s"(ps0), "+s"(pi0), "+s"(indx), "+s"(indy));
   auto potential = [&](double x, double y) -> double {
     double pot = 0;
     if (nb > 0) {
