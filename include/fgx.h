@@ -233,7 +233,7 @@ int fgx_batch_restore(fgx_batch* b, int disk, const char* path);
 /* bytes of device memory a disk of these parameters needs */
 int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes);
 
-/* ---- serving loop (north-star C5; no counterpart in the reference) ------
+/* ---- serving loop (a stream of run requests; no counterpart in the reference)
  * Requests (one disk each: parameters, planets, number of DTs, arrival time
  * in seconds from the start of fgx_server_run) are admitted in arrival order
  * while the summed device footprint stays within hbm_budget; requests that
