@@ -63,7 +63,7 @@ struct BatchBase {
 // checkpoint file: header, per-disk state record, ring profiles carried from
 // substep to substep, fields in the engine's representation, FNV-1a 64 sum
 struct CkptHeader {
-  char magic[8];             // "FGXCKPT1"
+  char magic[8];             // "FGXCKPT2"
   int32_t real_bytes, nrad, nsec, adiabatic, nfields, nrings;
   uint32_t state_bytes, pad;
 };
@@ -648,7 +648,13 @@ struct Batch : BatchBase {
       for (int i = 0; i < hd[d].nrad; ++i)
         for (int j = 0; j < hd[d].nsec; ++j) res[(size_t)i * hd[d].nsec + j] -= vref_h[d][i];
       LCHK(upload_field(d, D.vtheta, res));
-      if (ad) LCHK(upload_field(d, D.energy, hd[d].energy));
+      if (ad) {
+        // the energy is kept as a residual of its ring profile EnergyMed
+        std::vector<double> eres(hd[d].energy);
+        for (int i = 0; i < hd[d].nrad; ++i)
+          for (int j = 0; j < hd[d].nsec; ++j) eres[(size_t)i * hd[d].nsec + j] -= hd[d].energymed[i];
+        LCHK(upload_field(d, D.energy, eres));
+      }
       if (lab) LCHK(upload_field(d, D.label, hd[d].label));
     }
     // initial pressure (InitEuler's ComputePressureField), used by the
@@ -724,7 +730,10 @@ struct Batch : BatchBase {
       LCHK(upload_d(D.cs2_d + o, cs2));
       LCHK(upload_d(D.sigmamed_d + o, h.sigmamed)); LCHK(upload_d(D.energymed_d + o, h.energymed));
       for (int i = 0; i < h.nrad; ++i) vref[i] = h.vtheta[(size_t)i * h.nsec];
-      for (int i = 1; i < h.nrad; ++i) gstar[i] = (1.0 / h.rmed[i - 1] - 1.0 / h.rmed[i]) / (h.rmed[i] - h.rmed[i - 1]);
+      // the star's radial gravity between ring centres, -(phi_i - phi_i-1) /
+      // (r_i - r_i-1) with phi = -G/r, is exactly G / (r_i r_i-1): formed as that
+      // product it carries one rounding instead of the difference's cancellation
+      for (int i = 1; i < h.nrad; ++i) gstar[i] = 1.0 / (h.rmed[i] * h.rmed[i - 1]);
       for (int i = 0; i < h.nrad; ++i) invrinf[i] = 1.0 / h.rinf[i];
       for (int i = 0; i < h.nrad; ++i) invrmed[i] = 1.0 / h.rmed[i];
       vref_h[d] = vref;
@@ -767,30 +776,33 @@ struct Batch : BatchBase {
   // g.w: 0 = one full wave per ring, 1 = one partial wave, >= 2 = arcs
   template <int K>
   int theta_k(const ThetaGroup& g) {
-    return D.advect_label ? theta_kl<K, true>(g) : theta_kl<K, false>(g);
+    if (D.advect_label) return D.adiabatic ? theta_kl<K, true, true>(g) : theta_kl<K, true, false>(g);
+    return D.adiabatic ? theta_kl<K, false, true>(g) : theta_kl<K, false, false>(g);
   }
-  template <int K, bool LAB>
+  template <int K, bool LAB, bool AD>
   int theta_kl(const ThetaGroup& g) {
     if constexpr (sizeof(real) == 8 && K > 8) {
       return set_error(FGX_EDEVICE, "no FP64 azimuthal kernel for K=%d", g.k);
     } else {
-      if (g.w == 0) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 1, LAB>, g.n, 64, 0, D, g.tiles, 1);
-      if (g.w == 1) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 0, LAB>, g.n, 64, 0, D, g.tiles, 1);
+      if (g.w == 0) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 1, LAB, AD>, g.n, 64, 0, D, g.tiles, 1);
+      if (g.w == 1) return launch("trans_theta", 0, k_trans_theta_arc<real, K, 0, LAB, AD>, g.n, 64, 0, D, g.tiles, 1);
       if constexpr (K >= 2 && K <= 4) {
-        return launch("trans_theta", 0, k_trans_theta_arc<real, K, 2, LAB>, g.n, 64 * g.w, 0, D, g.tiles, g.w);
+        return launch("trans_theta", 0, k_trans_theta_arc<real, K, 2, LAB, AD>, g.n, 64 * g.w, 0, D, g.tiles, g.w);
       } else {
         return set_error(FGX_EDEVICE, "no multi-wave azimuthal kernel for K=%d", g.k);
       }
     }
   }
+  template <bool LAB, bool AD>
+  int theta_multi_launch() {
+    return launch("trans_theta", 0, k_trans_theta_multi<real, LAB, AD>, theta_multi_n, theta_multi_block, 0, D, theta_multi);
+  }
   int trans_theta(double bytes_per_cell) {
     group_begin("trans_theta", bytes_per_cell);
     int r = FGX_OK;
     if (theta_multi.ng > 0) {
-      r = D.advect_label ? launch("trans_theta", 0, k_trans_theta_multi<real, true>, theta_multi_n, theta_multi_block, 0,
-                                  D, theta_multi)
-                         : launch("trans_theta", 0, k_trans_theta_multi<real, false>, theta_multi_n, theta_multi_block,
-                                  0, D, theta_multi);
+      if (D.advect_label) r = D.adiabatic ? theta_multi_launch<true, true>() : theta_multi_launch<true, false>();
+      else r = D.adiabatic ? theta_multi_launch<false, true>() : theta_multi_launch<false, false>();
     }
     for (const ThetaGroup& g : theta_groups) {
       if (r) break;
@@ -985,10 +997,12 @@ struct Batch : BatchBase {
       HIPCHK(hipMemcpyAsync(tmp.data(), src + dch[d].fo, nfull * sizeof(real), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
-    if (field == FGX_VTHETA) {
-      // vtheta is stored as a residual of the reference profile
+    if (field == FGX_VTHETA || (field == FGX_ENERGY && D.adiabatic)) {
+      // vtheta and the energy are stored as residuals of ring profiles
+      const std::vector<double>& prof = field == FGX_VTHETA ? vref_h[d] : h.energymed;
       for (size_t k = 0; k < n; ++k) {
-        const double v = (double)tmp[k] + vref_h[d][k / h.nsec];
+        const size_t i = k / h.nsec;
+        const double v = (double)tmp[k] + (i < (size_t)h.nrad ? prof[i] : 0.0);
         if (dtype == FGX_F32) ((float*)host)[k] = (float)v; else ((double*)host)[k] = v;
       }
       return FGX_OK;
@@ -1010,7 +1024,9 @@ struct Batch : BatchBase {
     std::vector<real> tmp(n);
     for (size_t k = 0; k < n; ++k) {
       double v = dtype == FGX_F32 ? (double)((const float*)host)[k] : ((const double*)host)[k];
-      if (field == FGX_VTHETA) v -= vref_h[d][k / h.nsec];
+      const size_t i = k / h.nsec;
+      if (field == FGX_VTHETA) v -= vref_h[d][i];
+      if (field == FGX_ENERGY && i < (size_t)h.nrad) v -= h.energymed[i];
       tmp[k] = (real)v;
     }
     HIPCHK(hipMemcpyAsync(dst + dch[d].fo, tmp.data(), n * sizeof(real), hipMemcpyHostToDevice, stream));
@@ -1165,7 +1181,7 @@ struct Batch : BatchBase {
     const std::vector<real*> fields = ckpt_fields();
     real* rings[2] = {D.csmean, D.visc};
     CkptHeader hdr{};
-    std::memcpy(hdr.magic, "FGXCKPT1", 8);
+    std::memcpy(hdr.magic, "FGXCKPT2", 8);
     hdr.real_bytes = (int32_t)sizeof(real); hdr.nrad = h.nrad; hdr.nsec = h.nsec;
     hdr.adiabatic = D.adiabatic; hdr.nfields = (int32_t)fields.size(); hdr.nrings = 2;
     hdr.state_bytes = (uint32_t)sizeof(DiskState);
@@ -1207,7 +1223,7 @@ struct Batch : BatchBase {
     const size_t want = sizeof(CkptHeader) + sizeof(DiskState) + (2 * nrr + fields.size() * nf) * sizeof(real) + 8;
     CkptHeader hdr{};
     if (buf.size() >= sizeof hdr) std::memcpy(&hdr, buf.data(), sizeof hdr);
-    if (buf.size() < sizeof hdr || std::memcmp(hdr.magic, "FGXCKPT1", 8) != 0)
+    if (buf.size() < sizeof hdr || std::memcmp(hdr.magic, "FGXCKPT2", 8) != 0)
       return set_error(FGX_EIO, "'%s' is not a checkpoint", path);
     if (hdr.real_bytes != (int32_t)sizeof(real) || hdr.nrad != h.nrad || hdr.nsec != h.nsec ||
         hdr.adiabatic != D.adiabatic || hdr.nfields != (int32_t)fields.size() || hdr.nrings != 2 ||

@@ -103,6 +103,23 @@ __device__ __forceinline__ real qdiv(real a, real b) {
   if constexpr (sizeof(real) == 4) return a * __builtin_amdgcn_rcpf(b);
   else return a / b;
 }
+// The energy field holds e - EnergyMed(ring): a residual of the FP64 ring
+// profile (Theo.cu:33-51), so that its rounding, and that of the
+// intermediate energies, is relative to the departure from the profile and
+// not to e itself.  The whole value of a residual x on ring i:
+template <class real>
+__device__ __forceinline__ real e_whole(const Dev<real>& D, int rb, int i, real x) {
+  return (real)(RT(energymed_d)[rb + i] + (double)x);
+}
+// DivisePolarGrid's work of the density itself, density / (density + 1e-20)
+// (Kernels.cu:1127-1134): exactly 1 wherever the 1e-20 is below half an ulp of
+// the density (in FP32 above 1e-12, in FP64 above 3e-4), else the quotient
+// (FP32 forms that rare quotient with the hardware reciprocal)
+template <class real>
+__device__ __forceinline__ real dens_work(real sg) {
+  if constexpr (sizeof(real) == 4) return sg >= 1e-12f ? 1.0f : sg * __builtin_amdgcn_rcpf(sg + 1e-20f);
+  else return sg >= 3e-4 ? 1.0 : sg / (sg + 1e-20);
+}
 // Substep1's azimuthal pressure gradient (times dx_theta): the FP64 pressure
 // difference over the mean density; the quotient is rounded to the field
 // precision anyway, so FP32 forms it with a reciprocal
@@ -883,7 +900,7 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_sound_speed(Dev<real> D) {
   FGX_STRIP_BEGIN {
     const real g = (real)c.adiabaticindex;
-    AT(D.cs, i, j) = sqrt(g * (g - (real)1) * AT(D.energy, i, j) / AT(D.dens, i, j));
+    AT(D.cs, i, j) = sqrt(g * (g - (real)1) * e_whole(D, rb, i, (real)AT(D.energy, i, j)) / AT(D.dens, i, j));
   }
 }
 
@@ -904,7 +921,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
     // isothermal cs^2 of the ring, or (gamma-1)*e
     const double g1 = c.adiabaticindex - 1.0;
     auto press = [&](int ii, int jj) -> double {
-      if (D.adiabatic) return g1 * (double)AT(D.energy, ii, jj);
+      if (D.adiabatic) return g1 * (RT(energymed_d)[rb + ii] + (double)AT(D.energy, ii, jj));
       return (double)AT(D.dens, ii, jj) * RT(cs2_d)[rb + ii];
     };
     const real sig = AT(D.dens, i, j);
@@ -1080,15 +1097,19 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
       qp = q1 * (real)exp((double)log(q1 / q2) * log(RT(rmed_d)[rb] / RT(rmed_d)[rb + 1]) / log(RT(rmed_d)[rb + 1] / RT(rmed_d)[rb + 2]));
     } else qp = 0;
     AT(D.qplus, i, j) = qp;
+    // on residuals of the energy profile e0: e_new - e0 = (num - e0 * den) / den
     const real g1 = (real)c.adiabaticindex - (real)1;
     const real eint = AT(D.energyint, i, j), dvv = AT(D.divv, i, j);
+    const real e0 = RT(energymed)[rb + i];
     real en;
     if (!D.cooling) {
-      en = (dt * qp + eint) / ((real)1 + g1 * dt * dvv);
+      const real gd = g1 * dt * dvv;
+      en = (dt * qp + eint - e0 * gd) / ((real)1 + gd);
     } else {
       const real sig = AT(D.dens, i, j);
       const real tc = RT(coolingtime)[rb + i], sm = RT(sigmamed)[rb + i];
-      const real num = RT(energymed)[rb + i] * dt * sig / sm + tc * eint + dt * tc * (qp - RT(qplusmed)[rb + i] * sig / sm);
+      const real num = e0 * dt * (sig - sm) / sm + tc * eint + dt * tc * (qp - RT(qplusmed)[rb + i] * sig / sm) -
+                       e0 * g1 * dt * tc * dvv;
       const real den = dt + tc + g1 * dt * tc * dvv;
       en = num / den;
     }
@@ -1106,9 +1127,11 @@ __global__ void __launch_bounds__(FGX_TPB) k_open_boundary(Dev<real> D) {
   const int ns = c.nsec, rb = c.ro;
   const long long fb = c.fo;
   const real sm0 = RT(sigmamed)[rb];
+  // energy residuals move between rings with the profile difference
+  const double de01 = D.adiabatic ? RT(energymed_d)[rb + 1] - RT(energymed_d)[rb] : 0.0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     AT(D.dens, 0, j) = AT(D.dens, 1, j);
-    if (D.adiabatic) AT(D.energy, 0, j) = AT(D.energy, 1, j);
+    if (D.adiabatic) AT(D.energy, 0, j) = (real)(de01 + (double)AT(D.energy, 1, j));
     if (AT(D.vrad, 2, j) > 0 || AT(D.dens, 1, j) < sm0) AT(D.vrad, 1, j) = 0;
     else AT(D.vrad, 1, j) = AT(D.vrad, 2, j);
   }
@@ -1146,12 +1169,15 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
   __syncthreads();
   const int ia = ia_sh[0], ia2 = ia_sh[1];
   const real sm1 = RT(sigmamed)[rb + 1], sm2 = RT(sigmamed)[rb + nr - 2];
+  // energy residuals move between rings with the profile difference
+  const double de01 = D.adiabatic ? RT(energymed_d)[rb + 1] - RT(energymed_d)[rb] : 0.0;
+  const double deo = D.adiabatic ? RT(energymed_d)[rb + nr - 2] - RT(energymed_d)[rb + nr - 1] : 0.0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     int jp = j + ia;
     if (jp >= ns) jp -= ns;
     if (jp < 0) jp += ns;
     AT(D.dens, 0, jp) = AT(D.dens, 1, j);
-    if (D.adiabatic) AT(D.energy, 0, jp) = AT(D.energy, 1, j);
+    if (D.adiabatic) AT(D.energy, 0, jp) = (real)(de01 + (double)AT(D.energy, 1, j));
     const real vm = -csv(1, j) * (AT(D.dens, 1, j) - sm1) / sm1;
     AT(D.vrad, 1, j) = (real)2 * vm - AT(D.vrad, 2, j);
     const int io = nr - 1;
@@ -1159,7 +1185,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
     if (jo >= ns) jo -= ns;
     if (jo < 0) jo += ns;
     AT(D.dens, io, j) = AT(D.dens, io - 1, jo);
-    if (D.adiabatic) AT(D.energy, io, j) = AT(D.energy, io - 1, jo);
+    if (D.adiabatic) AT(D.energy, io, j) = (real)(deo + (double)AT(D.energy, io - 1, jo));
     const real vm2 = csv(io, j) * (AT(D.dens, io - 1, j) - sm2) / sm2;
     AT(D.vrad, io, j) = (real)2 * vm2 - AT(D.vrad, io - 1, j);
   }
@@ -1173,8 +1199,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
   const double me = block_sum(e0, redd) / ns, me2 = block_sum(e1, redd) / ns;
   // ring means restored to the initial profile, in FP64
   const double cd = RT(sigmamed_d)[rb] - md, cd2 = RT(sigmamed_d)[rb + nr - 1] - md2;
-  const double ce = D.adiabatic ? RT(energymed_d)[rb] - me : 0.0;
-  const double ce2 = D.adiabatic ? RT(energymed_d)[rb + nr - 1] - me2 : 0.0;
+  // (the energy field is a residual of the profile: its ring mean goes to 0)
+  const double ce = D.adiabatic ? -me : 0.0;
+  const double ce2 = D.adiabatic ? -me2 : 0.0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     AT(D.dens, 0, j) = (real)((double)AT(D.dens, 0, j) + cd);
     AT(D.dens, nr - 1, j) = (real)((double)AT(D.dens, nr - 1, j) + cd2);
@@ -1210,7 +1237,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
     AT(D.vrad, i, j) = (real)(((double)AT(D.vrad, i, j) + lambda * vr0) * inv);
     AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - RT(vref)[rb + i])) * inv);
     AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * RT(sigmamed_d)[rb + i]) * inv);
-    if (D.adiabatic) AT(D.energy, i, j) = (real)(((double)AT(D.energy, i, j) + lambda * RT(energymed_d)[rb + i]) * inv);
+    // the energy relaxes to its profile EnergyMed, of which it is stored as a
+    // residual: (e + lambda e0) / (1 + lambda) - e0 = (e - e0) / (1 + lambda)
+    if (D.adiabatic) AT(D.energy, i, j) = (real)((double)AT(D.energy, i, j) * inv);
   }
 }
 
@@ -1348,8 +1377,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) sS2[j] = th_star(sD2, j, ns, v2, dt, dxt, invdxt);
   __syncthreads();
   // one pass of VanLeerTheta on sQ with divisor Dv, DensStar S, velocity v
-  auto pass = [&](const real* Dv, const real* S, const real* varr, real vu) {
-    for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = sQ[j] / (Dv[j] + (real)1e-20);
+  // xb: what the stored quantity lacks of the whole one (the energy profile
+  // for the energy residual)
+  auto pass = [&](const real* Dv, const real* S, const real* varr, real vu, real xb) {
+    for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = (sQ[j] + xb) / (Dv[j] + (real)1e-20);
     __syncthreads();
     for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
       const int jp = (j + 1 == ns) ? 0 : j + 1;
@@ -1373,8 +1404,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
     const real* src = q == 0 ? D.t_rmp : q == 1 ? D.t_rmm : q == 2 ? D.t_tmp : q == 3 ? D.t_tmm : q == 4 ? D.t_e : D.t_lab;
     for (int j = threadIdx.x; j < ns; j += FGX_TPB) sQ[j] = src[row + j];
     __syncthreads();
-    pass(sD1, sS1, sV, (real)0);
-    pass(sD2, sS2, nullptr, v2);
+    const real xb = q == 4 ? RT(energymed)[rb + i] : (real)0;
+    pass(sD1, sS1, sV, (real)0, xb);
+    pass(sD2, sS2, nullptr, v2, xb);
     if (q == 2 || q == 5) {
       real* hold = q == 2 ? sH : sL;
       for (int j = threadIdx.x; j < ns; j += FGX_TPB) hold[j] = sQ[j];
@@ -1574,7 +1606,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_temperature(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
   if (!D.adiabatic) AT(D.work, i, j) = AT(D.pressure, i, j) / AT(D.dens, i, j);
-  else AT(D.work, i, j) = (real)(c.adiabaticindex - 1.0) * AT(D.energy, i, j) / AT(D.dens, i, j);
+  else AT(D.work, i, j) = (real)(c.adiabaticindex - 1.0) * e_whole(D, rb, i, (real)AT(D.energy, i, j)) / AT(D.dens, i, j);
 }
 // isothermal pressure at set-up (ComputePressureFieldKernel at InitEuler)
 template <class real>
@@ -1585,7 +1617,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_pressure_init(Dev<real> D) {
     const real cs = D.cs_ring[rb + i];
     AT(D.pressure, i, j) = AT(D.dens, i, j) * cs * cs;
   } else {
-    AT(D.pressure, i, j) = (real)(c.adiabaticindex - 1.0) * AT(D.energy, i, j);
+    AT(D.pressure, i, j) = (real)(c.adiabaticindex - 1.0) * e_whole(D, rb, i, (real)AT(D.energy, i, j));
   }
 }
 // ComputeForce with all eleven Hill-cut variants (Force.cu:31-67,

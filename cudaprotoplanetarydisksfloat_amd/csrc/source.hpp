@@ -64,13 +64,15 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   // per ring: inside the loop they are read from LDS into vector registers,
   // instead of some fifty per-ring values and their table addresses being
   // held in scalar registers across the loop
-  __shared__ double cd_[8][FGX_SRC_STAGE];   // cs2_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d invrmed_d
+  // cd_[0]: the isothermal cs^2, or with an energy equation the energy profile
+  // EnergyMed of which the energy field is a residual
+  __shared__ double cd_[8][FGX_SRC_STAGE];   // cs2_d|energymed_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d invrmed_d
   __shared__ real cf_[10][FGX_SRC_STAGE];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed
   const int rlo = i0 - 4;
   {
     const int r = rlo + lane;
     if (r >= 0 && r < nr && lane < rs + 8 && lane < FGX_SRC_STAGE) {
-      cd_[0][lane] = RT(cs2_d)[rb + r];
+      cd_[0][lane] = ad ? RT(energymed_d)[rb + r] : RT(cs2_d)[rb + r];
       cd_[1][lane] = RT(rmed_d)[rb + r];
       cd_[2][lane] = RT(invdiffrmed_d)[rb + r];
       cd_[3][lane] = RT(gstar)[rb + r];
@@ -140,7 +142,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     return pot;
   };
   auto pressure = [&](real sg, real e, int k) -> double {
-    return ad ? g1 * (double)e : (double)sg * FGX_CD(0, k);
+    return ad ? g1 * (FGX_CD(0, k) + (double)e) : (double)sg * FGX_CD(0, k);
   };
   const int ks = (i0 - 2 > 0) ? i0 - 2 : 0;
   if (ks > 0) {   // prime stage A with ring ks-1

@@ -26,6 +26,13 @@ namespace fgx {
 
 __device__ __forceinline__ float vl_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double vl_rcp(double x) { return 1.0 / x; }
+// DivisePolarGrid's work x / (d + 1e-20): FP32 as a product with the
+// reciprocal `inv` of d + 1e-20, FP64 as the quotient itself
+__device__ __forceinline__ float work_of(float x, float d, float inv) { (void)d; return x * inv; }
+__device__ __forceinline__ double work_of(double x, double d, double inv) { (void)inv; return x / (d + 1e-20); }
+// a * b + c with one rounding
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // whole-wave rotations by one lane (DPP wave_ror:1 / wave_rol:1): lane l
 // receives lane l-1 (lane 0 receives lane 63), resp. lane l+1 (lane 63
@@ -66,6 +73,7 @@ template <class real>
 struct RadFields {
   const real *dens, *vrad, *vtheta, *energy, *label;
   const FGX_AS4 double *rmed_d, *vref;
+  const FGX_AS4 real* emed;   // ring profile the stored energy is a residual of
 };
 
 // the raw values of ring k a thread needs (clamped rings only feed zero
@@ -89,24 +97,24 @@ __device__ __forceinline__ void rad_fetch(const RadFields<real>& f, int ns, int 
 }
 
 template <class real, bool LAB = false>
-__device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int k, double om, const RadRaw<real>& w,
-                                           RadRing<real, LAB>& r) {
+__device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int k, double om, bool ad,
+                                           const RadRaw<real>& w, RadRing<real, LAB>& r) {
   k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
   const real sg = w.sg;
   const double rk = f.rmed_d[k];
   const double l0 = (f.vref[k] + rk * om) * rk;
   const double lp = l0 + (double)w.vtp * rk, lm = l0 + (double)w.vtj * rk;
-  // DivisePolarGrid: quantity / (density + 1e-20), as a product with one
-  // reciprocal per cell; the density's own work is exactly 1 wherever the
-  // 1e-20 is below its rounding (density >= 1e-12), as in the quotient
+  // DivisePolarGrid: quantity / (density + 1e-20); FP32 forms it as a
+  // product with one reciprocal per cell.  The density's own work is exactly
+  // 1 wherever the 1e-20 is below half an ulp of the density, as in the
+  // quotient.
   const real inv = vl_rcp(sg + (real)1e-20);
-  real w1 = (real)1;
-  if (sg < (real)1e-12) w1 = sg / (sg + (real)1e-20);
+  const real w1 = dens_work(sg);
   r.sg = sg; r.vr0 = w.vr0; r.vr1 = w.vr1;
   r.q[0] = sg;
   r.q[1] = w1;
-  r.q[2] = (sg * w.vr1) * inv;
-  r.q[3] = (sg * w.vr0) * inv;
+  r.q[2] = work_of(sg * w.vr1, sg, inv);
+  r.q[3] = work_of(sg * w.vr0, sg, inv);
   const double dd = (double)sg;
   r.tp = dd * lp; r.tm = dd * lm;
   if constexpr (sizeof(real) == 4) {
@@ -115,8 +123,11 @@ __device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int
   } else {
     r.q[4] = (real)(r.tp / (dd + 1e-20)); r.q[5] = (real)(r.tm / (dd + 1e-20));
   }
+  // the energy is stored as a residual of its ring profile: its work is
+  // formed from the whole value (its flux is small, so one rounding of the
+  // sum costs nothing), its update stays a residual
   r.e = w.e;
-  r.q[6] = w.e * inv;
+  r.q[6] = ad ? work_of(f.emed[k] + w.e, sg, inv) : (real)0;
   if constexpr (LAB) {
     // ComputeExtQty (Kernels.cu:899-906): the label travels as density * label
     r.x = sg * w.lab;
@@ -129,7 +140,7 @@ __device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int n
                                          bool ad, RadRing<real, LAB>& r) {
   RadRaw<real> w;
   rad_fetch<real, LAB>(f, ns, nr, k, j, jp, ad, w);
-  rad_derive<real, LAB>(f, nr, k, om, w, r);
+  rad_derive<real, LAB>(f, nr, k, om, ad, w, r);
 }
 
 // van Leer slope of ring k from rings k-1, k, k+1 (StarRadKernel); zero on
@@ -166,7 +177,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   // quantities in use: 0..5 always, 6 (energy) if adiabatic, 7 (label) if LAB
   auto qon = [&](int q) { return q < 6 || q == 7 || ad; };
   const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0),
-                          LAB ? D.label + fb : nullptr, RT(rmed_d) + rb, RT(vref) + rb};
+                          LAB ? D.label + fb : nullptr, RT(rmed_d) + rb, RT(vref) + rb, RT(energymed) + rb};
   real* const o_dens = D.t_dens + fb;
   real* const o_rmp = D.t_rmp + fb;
   real* const o_rmm = D.t_rmm + fb;
@@ -228,7 +239,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     const RadRaw<real> cur = nxt;
     if (i + 1 < i1) rad_fetch<real, LAB>(F, ns, nr, i + 3, j, jp, ad, nxt);
     Ring rc;
-    rad_derive<real, LAB>(F, nr, i + 2, om, cur, rc);
+    rad_derive<real, LAB>(F, nr, i + 2, om, ad, cur, rc);
     // interface i+1 (zero flux at the outer edge); the ring values this
     // iteration needs are read together, ahead of their use
     const real v = rb1.vr0;
@@ -311,7 +322,7 @@ __device__ __forceinline__ void ld_chunk(const real* __restrict__ p, real (&v)[K
 // multiple of 64 elements, so each lane's chunk is aligned for vector loads.
 // LAB: the extensive label is carried as one more quantity and the label
 // (ComputeSpeQty, Kernels.cu:1309-1319) formed from the final density
-template <class real, int K, int WM, bool LAB>
+template <class real, int K, int WM, bool LAB, bool AD>
 __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTile t, int W) {
   // WM: 2 = W arcs of one full wave each, 1 = one full wave (nsec = 64*K),
   // 0 = one wave of T < 64 lanes
@@ -331,7 +342,7 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   const long long row = c.fo + (long long)i * ns;
   const int j0 = on ? (w * 64 + lane) * K : 0;   // first sector of this lane
   const long long my = row + j0;
-  const bool ad = D.adiabatic;
+  constexpr bool ad = AD;   // the energy equation, fixed per instantiation
   int xb = 0;
 
   // neighbour lanes along the ring; in(): values of this lane's left and
@@ -520,13 +531,34 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
 #pragma unroll
     for (int k = 0; k < K; ++k) X[k] += F[k] - ((k == K - 1) ? fr : F[k + 1]);
   };
-  // pass 1 (residual velocity), pass 2 (uniform fractional velocity) of one
-  // quantity; work = X / (DensInt + 1e-20)
+  // the same with one coefficient for the whole ring
+  auto apply_u = [&](real(&X)[K], real(&F)[K], real cu) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) F[k] *= cu;
+    const real fr = right1(F[0]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) X[k] += F[k] - ((k == K - 1) ? fr : F[k + 1]);
+  };
+  // Pass 2 moves the ring by one uniform fraction of a cell, so its fluxes
+  // are nearly equal from sector to sector and their differences, the
+  // updates, are small.  Formed as products of whole values, each flux would
+  // carry a rounding error of the size of the update itself.  They are
+  // therefore formed against ring constants: the interface density as sbar
+  // plus a residual S2r, a quantity's work as a ring value ab plus a residual
+  // (ab = 1 for the density, the profile's specific energy for the energy,
+  // 0 for the momenta), and the uniform part ab * sbar * v, which cancels
+  // between neighbours, is left out:
+  //   F = c2 * (ab * S2r + star(work - ab) * S2),  c2 = dxrad * v2 / Surf.
+  // The arithmetic is the reference's to within the rounding of the terms.
   unsigned pos1 = 0;
-  real cf1[K], SV1[K], invD1[K], SV2[K], invD2[K];
+  real cf1[K], SV1[K], invD1[K], S2r[K], SV2[K], invD2[K];
   const real ksi2 = v2 * dt;
   const bool pos2 = ksi2 > 0;
   const real cf2 = (pos2 ? dxt - ksi2 : dxt + ksi2) * invdxt;
+  const real sbar = RT(sigmamed)[rb + i];
+  const real c2 = dxrad * v2 * invsurf;
+  // one quantity X through both passes: the momenta (ring value 0 of the
+  // work: F = star(work) * S2 * c2 in pass 2)
   auto both = [&](real(&X)[K]) {
     real A[K], F[K];
 #pragma unroll
@@ -537,6 +569,24 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
     for (int k = 0; k < K; ++k) A[k] = X[k] * invD2[k];
     star_u(A, pos2, cf2, F);
     apply(X, F, SV2);
+  };
+  // the energy, stored as a residual X of its ring profile e0: pass 1 takes
+  // the work from the whole value (its fluxes are small), pass 2 splits it
+  // as ab + (X + e0 - ab * density) / density
+  auto both_energy = [&](real(&X)[K]) {
+    const real e0 = RT(energymed)[rb + i];
+    const real ab = e0 * vl_rcp(sbar);   // any value near the ring's e / density serves
+    real A[K], F[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) A[k] = (e0 + X[k]) * invD1[k];
+    star(A, pos1, cf1, F);
+    apply(X, F, SV1);
+#pragma unroll
+    for (int k = 0; k < K; ++k) A[k] = (fmadd(-ab, Dn[k], e0) + X[k]) * invD2[k];
+    star_u(A, pos2, cf2, F);
+#pragma unroll
+    for (int k = 0; k < K; ++k) F[k] = ab * S2r[k] + F[k] * (sbar + S2r[k]);
+    apply_u(X, F, c2);
   };
 
   // pass-1 velocity (ComputeResiduals); STANDARD transport folds the
@@ -550,7 +600,8 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
     pos1 |= (ksi > 0 ? 1u : 0u) << k;
     cf1[k] = ((ksi > 0) ? dxt - ksi : dxt + ksi) * invdxt;
   }
-  // density: DensStar and pass 1
+  // density: DensStar and pass 1 (the density's work is exactly 1 but for
+  // densities below the 1e-20's resolution)
   star(Dn, pos1, cf1, SV1);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -560,37 +611,46 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   {
     real A[K], F[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) A[k] = Dn[k] * invD1[k];
+    for (int k = 0; k < K; ++k) A[k] = dens_work(Dn[k]);
     star(A, pos1, cf1, F);
     apply(Dn, F, SV1);
   }
-  // pass-2 coefficients from the density after pass 1
-  star_u(Dn, pos2, cf2, SV2);
+  // pass-2 interface densities from the density after pass 1, as residuals
+  // of sbar (the slopes are the same)
+  {
+    real R[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) R[k] = Dn[k] - sbar;
+    star_u(R, pos2, cf2, S2r);
+  }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    SV2[k] = dxrad * SV2[k] * v2 * invsurf;
+    SV2[k] = (sbar + S2r[k]) * c2;
     invD2[k] = vl_rcp(Dn[k] + (real)1e-20);
   }
   // quantities in the order RadMomP, RadMomM, [energy], ThetaMomP (held),
-  // ThetaMomM (held)
+  // ThetaMomM (held); the energy is stored as a residual of its ring profile
   real* dsts[3] = {D.t_rmp, D.t_rmm, D.energy};
   real H[K], Qn[K];
 #pragma unroll
   for (int o = 0; o < 5; ++o) {
     if (o >= nq) break;
-    both(Qa[o]);
+    if (ad && o == 2) both_energy(Qa[o]);
+    else both(Qa[o]);
     if (o < oh) store_shift(dsts[o], Qa[o]);
   }
   if constexpr (LAB) both(Lq);
 #pragma unroll
   for (int k = 0; k < K; ++k) { H[k] = ad ? Qa[3][k] : Qa[2][k]; Qn[k] = ad ? Qa[4][k] : Qa[3][k]; }
-  // density, pass 2
+  // density, pass 2 (work 1 + residual)
   {
     real A[K], F[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) A[k] = Dn[k] * invD2[k];
+    for (int k = 0; k < K; ++k) A[k] = dens_work(Dn[k]) - (real)1;
     star_u(A, pos2, cf2, F);
-    apply(Dn, F, SV2);
+#pragma unroll
+    for (int k = 0; k < K; ++k) F[k] = S2r[k] + F[k] * (sbar + S2r[k]);
+    apply_u(Dn, F, c2);
   }
   store_shift(D.dens, Dn);
   if constexpr (LAB) {
@@ -628,10 +688,10 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   store_shift(D.vtheta, V);
 }
 
-template <class real, int K, int WM, bool LAB>
+template <class real, int K, int WM, bool LAB, bool AD>
 __global__ void __launch_bounds__(WM == 2 ? 1024 : 64) k_trans_theta_arc(Dev<real> D, const RingTile* __restrict__ tiles,
                                                                         int W) {
-  theta_arc_ring<real, K, WM, LAB>(D, tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)], W);
+  theta_arc_ring<real, K, WM, LAB, AD>(D, tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)], W);
 }
 
 // The multi-arc ring groups of a batch (K = 2 or 3 sectors per lane, W = 2..16
@@ -646,7 +706,7 @@ struct ThetaMulti {
   int k[FGX_THETA_MULTI], w[FGX_THETA_MULTI];
   const RingTile* tiles[FGX_THETA_MULTI];
 };
-template <class real, bool LAB>
+template <class real, bool LAB, bool AD>
 __global__ void __launch_bounds__(1024) k_trans_theta_multi(Dev<real> D, ThetaMulti M) {
   const int b = (int)blockIdx.x;
   int g = 0;
@@ -654,8 +714,8 @@ __global__ void __launch_bounds__(1024) k_trans_theta_multi(Dev<real> D, ThetaMu
   const int W = M.w[g];
   if ((int)threadIdx.x >= 64 * W) return;
   const RingTile t = M.tiles[g][xcd_tile(b - M.first[g], M.first[g + 1] - M.first[g])];
-  if (M.k[g] == 3) theta_arc_ring<real, 3, 2, LAB>(D, t, W);
-  else theta_arc_ring<real, 2, 2, LAB>(D, t, W);
+  if (M.k[g] == 3) theta_arc_ring<real, 3, 2, LAB, AD>(D, t, W);
+  else theta_arc_ring<real, 2, 2, LAB, AD>(D, t, W);
 }
 
 // split of a ring of nsec sectors: W arcs of 64*K sectors (K in {3, 2, 4},

@@ -61,9 +61,14 @@ RINGS = {"radii": 0, "rmed": 1, "surf": 2, "sigmamed": 3, "energymed": 4,
          "invrmed": 10}
 
 
+FP32, FP64, EXT80 = 0, 1, 2   # arithmetic of an oracle instance (EXT80: x87 long double)
+
+
 class Oracle:
     """One disk on the CPU; params/planets are the product's fgx structs
-    (only their bytes are used)."""
+    (only their bytes are used).  precision: FP32, FP64 (the restatement
+    proper and the CPU baseline) or EXT80 (the same code in 80-bit long
+    double, the checker of the FP64 engine)."""
 
     def __init__(self, params, planets, precision=1):
         L = lib()

@@ -33,6 +33,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace orc {
@@ -101,6 +102,9 @@ inline void fft2d(std::vector<std::complex<double>>& a, int rows, int cols, int 
 
 template <class real>
 struct Disk {
+  // FP64 for the FP32 and FP64 instances, the extended type for the 80-bit
+  // instance (the potential is formed in it)
+  using wide = typename std::conditional<(sizeof(real) > sizeof(double)), real, double>::type;
   fgx_params P;
   fgx_planets sys;
   int NRAD = 0, NSEC = 0;
@@ -183,9 +187,9 @@ struct Disk {
   }
   // ring mean, FP64 accumulation (Make1DprofileKernel, Kernels.cu:726-738)
   real ring_mean(const std::vector<real>& f, int i) const {
-    double s = 0;
-    for (int j = 0; j < NSEC; ++j) s += (double)f[id(i, j)];
-    return (real)(s / (double)NSEC);
+    wide s = 0;
+    for (int j = 0; j < NSEC; ++j) s += (wide)f[id(i, j)];
+    return (real)(s / (wide)NSEC);
   }
 
   // ------------------------------------------------------------------ setup
@@ -665,17 +669,17 @@ struct Disk {
       }
 #pragma omp parallel for schedule(static)
       for (int i = 0; i < NRAD; ++i) {
-        double r = Rmed[i];
+        wide r = Rmed[i];
         for (int j = 0; j < NSEC; ++j) {
-          double x = r * (double)cosj[j], y = r * (double)sinj[j];
-          double pot = 0;
+          wide x = r * (wide)cosj[j], y = r * (wide)sinj[j];
+          wide pot = 0;
           if (have_planet) {
-            double dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
+            wide dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
             pot = -G * mp / std::sqrt(dist + smooth);
             if (P.indirect_term) pot += G * mp * inv3 * (x * xp + y * yp);
           }
-          if (k == 0) pot += -G * 1.0 / r - (Indx * x + Indy * y);
-          Potential[id(i, j)] = (real)((double)Potential[id(i, j)] + pot);
+          if (k == 0) pot += -G * (wide)1.0 / r - (Indx * x + Indy * y);
+          Potential[id(i, j)] = (real)((wide)Potential[id(i, j)] + pot);
         }
       }
     }
@@ -877,19 +881,19 @@ struct Disk {
       double lambda = 0, damping;
       if (r < DRMIN) { damping = (r - DRMIN) / ((double)Rmed[0] - DRMIN); lambda = damping * damping * 10.0 * step / Tin; }
       if (r > DRMAX) { damping = (r - DRMAX) / ((double)Rmed[NRAD - 1] - DRMAX); lambda = damping * damping * 10.0 * step / Tout; }
-      double h = aspect_ratio(r);
+      const wide h = aspect_ratio(r), rw = Rmed[i];
       // Q9: the self-gravity profile of this kernel is never filled (zero)
-      double vt0 = std::sqrt(G * 1.0 / r * (1.0 - (1.0 + P.sigmaslope - 2.0 * P.flaringindex) * std::pow(h, 2.0) *
-                                                       std::pow(r, 2.0 * P.flaringindex)));
-      vt0 -= r * OmegaFrame;
-      double vr0 = -3.0 * (double)viscosity_array[i] / r * (-P.sigmaslope + .5);  // Q12
-      double d0 = SigmaMed[i], e0 = EnergyMed[i];
+      wide vt0 = std::sqrt(G * (wide)1.0 / rw * ((wide)1.0 - ((wide)1.0 + P.sigmaslope - (wide)2.0 * P.flaringindex) * h * h *
+                                                     std::pow(rw, (wide)2.0 * P.flaringindex)));
+      vt0 -= rw * OmegaFrame;
+      const wide vr0 = (wide)-3.0 * (wide)viscosity_array[i] / rw * (-P.sigmaslope + (wide).5);  // Q12
+      const wide d0 = SigmaMed[i], e0 = EnergyMed[i], lw = lambda;
       for (int j = 0; j < NSEC; ++j) {
         size_t k = id(i, j);
-        Vrad[k] = (real)(((double)Vrad[k] + lambda * vr0) / (1.0 + lambda));
-        Vtheta[k] = (real)(((double)Vtheta[k] + lambda * vt0) / (1.0 + lambda));
-        Dens[k] = (real)(((double)Dens[k] + lambda * d0) / (1.0 + lambda));
-        if (P.adiabatic) Energy[k] = (real)(((double)Energy[k] + lambda * e0) / (1.0 + lambda));
+        Vrad[k] = (real)(((wide)Vrad[k] + lw * vr0) / ((wide)1.0 + lw));
+        Vtheta[k] = (real)(((wide)Vtheta[k] + lw * vt0) / ((wide)1.0 + lw));
+        Dens[k] = (real)(((wide)Dens[k] + lw * d0) / ((wide)1.0 + lw));
+        if (P.adiabatic) Energy[k] = (real)(((wide)Energy[k] + lw * e0) / ((wide)1.0 + lw));
       }
     }
   }

@@ -15,9 +15,10 @@
 namespace {
 
 struct Handle {
-  int precision;  // 0 float, 1 double
+  int precision;  // 0 float, 1 double, 2 long double (x87 80-bit, 64-bit mantissa)
   std::unique_ptr<orc::Disk<float>> f;
   std::unique_ptr<orc::Disk<double>> d;
+  std::unique_ptr<orc::Disk<long double>> q;
 };
 
 template <class D>
@@ -93,15 +94,17 @@ void* orc_create(const fgx_params* p, const fgx_planets* pl, int precision) {
   h->precision = precision;
   try {
     if (precision == 0) { h->f.reset(new orc::Disk<float>()); h->f->setup(*p, *pl); }
+    else if (precision == 2) { h->q.reset(new orc::Disk<long double>()); h->q->setup(*p, *pl); }
     else { h->d.reset(new orc::Disk<double>()); h->d->setup(*p, *pl); }
   } catch (...) { delete h; return nullptr; }
   return h;
 }
 void orc_destroy(void* hp) { delete (Handle*)hp; }
 
-#define DISPATCH(hp, expr)                           \
-  Handle* h = (Handle*)hp;                           \
-  if (h->precision == 0) { auto& s = *h->f; expr; }  \
+#define DISPATCH(hp, expr)                                \
+  Handle* h = (Handle*)hp;                                \
+  if (h->precision == 0) { auto& s = *h->f; expr; }       \
+  else if (h->precision == 2) { auto& s = *h->q; expr; }  \
   else { auto& s = *h->d; expr; }
 
 // n calls of AlgoGas; returns the total number of substeps
