@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -29,7 +30,35 @@
 #include "sg.hpp"
 #include "source.hpp"
 
+// Device scratch of fgx_disk_force_noalloc: the per-ring partials of the
+// force kernel (44 doubles per ring, up to FGX_NOALLOC_FORCE_RINGS rings) and
+// the 44 totals after them.  Each GPU the library runs on holds its own copy
+// until the process ends.  In host code this name is only a stand-in: the
+// device address comes from hipGetSymbolAddress (fgx::noalloc_scratch).
+#define FGX_NOALLOC_FORCE_RINGS 4096
+#define FGX_NOALLOC_SCRATCH (FGX_NOALLOC_FORCE_RINGS * 44 + 64)
+static __device__ __align__(256) double scratch[FGX_NOALLOC_SCRATCH];
+
 namespace fgx {
+
+// the device address of `scratch` on device `dev` (which must be current),
+// looked up once per device and kept
+#define FGX_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[FGX_MAX_DEVICES] = {};
+static int noalloc_scratch(int dev, double** out) {
+  if (dev < 0 || dev >= FGX_MAX_DEVICES) return set_error(FGX_EDEVICE, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(FGX_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FGX_OK;
+}
 
 #define HIPCHK(expr)                                                                        \
   do {                                                                                      \
@@ -49,6 +78,8 @@ struct BatchBase {
   virtual int planets(int disk, fgx_planets* out) = 0;
   virtual int set_planets(int disk, const fgx_planets* in) = 0;
   virtual int force(int disk, double x, double y, double sm, double m, double* out44) = 0;
+  virtual int force_noalloc(int disk, double x, double y, double sm, double m, double* out44) = 0;
+  int device = 0;   // the HIP device the batch lives on
   virtual int mdcp(int disk, double* out) = 0;
   virtual int profile_enable(int on) = 0;
   virtual int profile_get(int k, const char** name, double* ms, long long* n, double* bytes) = 0;
@@ -353,6 +384,7 @@ struct Batch : BatchBase {
       if (p.corotating && pls[d].nb == 0) return set_error(FGX_EPARAM, "a corotating frame needs a planet");
       if (!p.is_disk) return set_error(FGX_EPARAM, "DISK NO (planets only) is not supported");
     }
+    HIPCHK(hipGetDevice(&device));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void**)&h_summary, (2 + FGX_SUMMARY_HIST + nd) * sizeof(int)));
     hd.resize(nd);
@@ -1121,7 +1153,7 @@ struct Batch : BatchBase {
   int force(int d, double x, double y, double sm, double m, double* out44) override {
     if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
     const int nr = hd[d].nrad;
-    LCHK(launch("force_full", 1, k_force_full<real>, nr, FGX_TPB, 0, D, d, x, y, sm, m));
+    LCHK(launch("force_full", 1, k_force_full<real>, nr, FGX_TPB, 0, D, d, x, y, sm, m, D.rpart));
     std::vector<double> part((size_t)nr * 44);
     HIPCHK(hipMemcpyAsync(part.data(), D.rpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -1131,6 +1163,35 @@ struct Batch : BatchBase {
       out44[q] = s;
     }
     return FGX_OK;
+  }
+  // the same 44 values with no allocation at all: the partials and their
+  // ring-order sums live in the static device scratch, the totals are copied
+  // straight into the caller's array.  Calls that overlap on one GPU share
+  // that scratch and race.
+  int force_noalloc(int d, double x, double y, double sm, double m, double* out44) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    const int nr = hd[d].nrad;
+    if (nr > FGX_NOALLOC_FORCE_RINGS)
+      return set_error(FGX_ERANGE, "fgx_disk_force_noalloc: NRAD=%d exceeds its scratch of %d rings", nr,
+                       FGX_NOALLOC_FORCE_RINGS);
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    double* sp = nullptr;
+    int r = noalloc_scratch(device, &sp);
+    if (!r) {
+      // every partial the sum reads (rings 0..nr-1) is written by this call,
+      // so nothing is cleared: the scratch keeps the last call's contents
+      double* tot = sp + (size_t)FGX_NOALLOC_FORCE_RINGS * 44;
+      hipLaunchKernelGGL(k_force_full<real>, dim3(nr), dim3(FGX_TPB), 0, stream, D, d, x, y, sm, m, sp);
+      hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(64), 0, stream, (const double*)sp, nr, tot);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(out44, tot, 44 * sizeof(double), hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) r = set_error(FGX_EDEVICE, "fgx_disk_force_noalloc: %s", hipGetErrorString(e));
+    }
+    if (cur != device) hipSetDevice(cur);
+    return r;
   }
   int mdcp(int d, double* out) override {
     if (hd[d].sys.nb == 0) { *out = 0; return FGX_OK; }
@@ -1380,6 +1441,10 @@ int fgx_disk_set_planets(fgx_disk* d, const fgx_planets* pl) {
 int fgx_disk_force(fgx_disk* d, double x, double y, double sm, double m, double out[44]) {
   if (!d || !out) return set_error(FGX_EPARAM, "null argument");
   return d->impl->force(0, x, y, sm, m, out);
+}
+int fgx_disk_force_noalloc(fgx_disk* d, double x, double y, double sm, double m, double out[44]) {
+  if (!d || !out) return set_error(FGX_EPARAM, "null argument");
+  return d->impl->force_noalloc(0, x, y, sm, m, out);
 }
 int fgx_profile_enable(fgx_batch* b, int enable) {
   if (!b) return set_error(FGX_EPARAM, "null handle");

@@ -1621,10 +1621,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_pressure_init(Dev<real> D) {
   }
 }
 // ComputeForce with all eleven Hill-cut variants (Force.cu:31-67,
-// Kernels.cu:804-839) for one point, per-ring FP64 partials into rpart[44]
+// Kernels.cu:804-839) for one point, per-ring FP64 partials into part[44 i ..]
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_force_full(Dev<real> D, int disk, double x, double y, double sm,
-                                                        double mass) {
+                                                        double mass, double* __restrict__ part) {
   const int i = blockIdx.x;
   const DiskConst& c = D.dc[disk];
   if (i >= c.nrad) return;
@@ -1653,7 +1653,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_force_full(Dev<real> D, int disk, d
       f[11 + k] += (double)(cm * dy * inv3 * cut);
     }
   }
-  double* out = D.rpart + (size_t)i * 44;
+  double* out = part + (size_t)i * 44;
   for (int k = 0; k < 11; ++k) {
     const double fx = block_sum(f[k], redd), fy = block_sum(f[11 + k], redd);
     if (threadIdx.x == 0) {
@@ -1661,6 +1661,16 @@ __global__ void __launch_bounds__(FGX_TPB) k_force_full(Dev<real> D, int disk, d
       out[22 + k] = inner ? fy : 0; out[33 + k] = inner ? 0 : fy;
     }
   }
+}
+// the 44 force totals from k_force_full's per-ring partials, each summed in
+// ring order (the order of the host sum of fgx_disk_force, so both agree bit
+// for bit): one thread per component
+__global__ void __launch_bounds__(64) k_force_sum(const double* __restrict__ part, int nr, double* __restrict__ out) {
+  const int q = (int)threadIdx.x;
+  if (q >= 44) return;
+  double s = 0;
+  for (int i = 0; i < nr; ++i) s += part[(size_t)i * 44 + q];
+  out[q] = s;
 }
 // CircumPlanetaryMassKernel (Kernels.cu:245-261), per-ring FP64 partials
 template <class real>

@@ -111,6 +111,7 @@ _sigs = {
     "fgx_disk_planets": ([_vp, ctypes.POINTER(Planets)], _i),
     "fgx_disk_set_planets": ([_vp, ctypes.POINTER(Planets)], _i),
     "fgx_disk_force": ([_vp, _d, _d, _d, _d, ctypes.POINTER(_d)], _i),
+    "fgx_disk_force_noalloc": ([_vp, _d, _d, _d, _d, ctypes.POINTER(_d)], _i),
     "fgx_disk_write_outputs": ([_vp, _i], _i), "fgx_disk_write_logs": ([_vp, _i], _i),
     "fgx_disk_write_setup_files": ([_vp], _i),
     "fgx_footprint_bytes": ([_vp, _i, ctypes.POINTER(_sz)], _i),
@@ -281,9 +282,12 @@ class Batch:
     def set_planets(self, pl):
         _check(lib.fgx_disk_set_planets(self.h, ctypes.byref(pl)))
 
-    def force(self, x, y, smoothing, mass):
+    def force(self, x, y, smoothing, mass, noalloc=False):
+        """ComputeForce's 44 values (Force.cu:31-67) of disk 0 at (x, y);
+        noalloc=True uses the allocation-free entry point."""
         out = (ctypes.c_double * 44)()
-        _check(lib.fgx_disk_force(self.h, x, y, smoothing, mass, out))
+        fn = lib.fgx_disk_force_noalloc if noalloc else lib.fgx_disk_force
+        _check(fn(self.h, x, y, smoothing, mass, out))
         return np.array(out[:])
 
     def write_setup_files(self):

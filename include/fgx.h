@@ -199,6 +199,16 @@ int fgx_disk_write_setup_files(fgx_disk* d);
  * (fxi[0..10], fxo[0..10], fyi[0..10], fyo[0..10]) */
 int fgx_disk_force(fgx_disk* d, double x, double y, double smoothing, double mass,
                    double out[44]);
+/* fgx_disk_force with no allocation of any kind: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The kernels'
+ * per-ring partials and the totals live in a static device array of the
+ * library (4096 rings x 44 doubles + 64, about 1.4 MB per GPU, held until the
+ * process ends); a disk of more than 4096 rings is refused with FGX_ERANGE.
+ * The result equals fgx_disk_force's bit for bit.  Calls on one GPU share
+ * that array: calls that overlap in time on one GPU race. */
+int fgx_disk_force_noalloc(fgx_disk* d, double x, double y, double smoothing, double mass,
+                           double out[44]);
 
 /* ---- batches of many disks (no reference counterpart: the reference runs
  * one disk per process, src/Main.cu:87-98).  All disks of one batch share
