@@ -94,7 +94,7 @@ struct BatchBase {
 // checkpoint file: header, per-disk state record, ring profiles carried from
 // substep to substep, fields in the engine's representation, FNV-1a 64 sum
 struct CkptHeader {
-  char magic[8];             // "FGXCKPT2"
+  char magic[8];             // "FGXCKPT3"
   int32_t real_bytes, nrad, nsec, adiabatic, nfields, nrings;
   uint32_t state_bytes, pad;
 };
@@ -448,6 +448,7 @@ struct Batch : BatchBase {
       c.cavityradius = p.cavityradius; c.cavityratio = p.cavityratio; c.cavitywidth = p.cavitywidth;
       c.viscosity = p.viscosity; c.alphaviscosity = p.alphaviscosity;
       c.hill_radius = h.hill_radius; c.sigma0 = p.sigma0; c.eccentricity = p.eccentricity;
+      c.omega0 = h.omega_frame;
       c.substeps_forced = p.ext_substeps;
       // the viscosity profile changes only through the ring-mean sound speed
       // (adiabatic alpha viscosity) or the time-dependent transition scale
@@ -562,15 +563,16 @@ struct Batch : BatchBase {
     for (real** r : rings) LCHK(dalloc(r, R));
     LCHK(dalloc(&D.nshift, R));
     {
-      double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d;
-      LCHK(dalloc(&vref, R)); LCHK(dalloc(&gstar, R)); LCHK(dalloc(&rinf_d, R)); LCHK(dalloc(&invrinf_d, R));
+      double *vref, *acc0, *u0, *gq0, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d;
+      LCHK(dalloc(&vref, R)); LCHK(dalloc(&acc0, R)); LCHK(dalloc(&u0, R)); LCHK(dalloc(&gq0, R));
+      LCHK(dalloc(&rinf_d, R)); LCHK(dalloc(&invrinf_d, R));
       LCHK(dalloc(&cs2_d, R)); D.cs2_d = cs2_d;
       double *sigmamed_d, *energymed_d;
       LCHK(dalloc(&sigmamed_d, R)); LCHK(dalloc(&energymed_d, R));
       D.sigmamed_d = sigmamed_d; D.energymed_d = energymed_d;
       LCHK(dalloc(&rmed_d, R)); LCHK(dalloc(&invrmed_d, R)); LCHK(dalloc(&invdiffrmed_d, R)); LCHK(dalloc(&D.vmoy, R));
       LCHK(dalloc(&cosj_d, S)); LCHK(dalloc(&sinj_d, S));
-      D.vref = vref; D.gstar = gstar; D.rinf_d = rinf_d; D.invrinf_d = invrinf_d; D.rmed_d = rmed_d;
+      D.vref = vref; D.acc0 = acc0; D.u0 = u0; D.gq0 = gq0; D.rinf_d = rinf_d; D.invrinf_d = invrinf_d; D.rmed_d = rmed_d;
       D.invdiffrmed_d = invdiffrmed_d; D.cosj_d = cosj_d; D.sinj_d = sinj_d; D.invrmed_d = invrmed_d;
     }
     LCHK(dalloc(&D.cosj, S)); LCHK(dalloc(&D.sinj, S)); LCHK(dalloc(&D.lost, S));
@@ -674,7 +676,13 @@ struct Batch : BatchBase {
     HIPCHK(hipStreamSynchronize(stream));
     LCHK(upload_rings());
     for (int d = 0; d < nd; ++d) {
-      LCHK(upload_field(d, D.dens, hd[d].dens));
+      // the density is kept as a residual of S0, the field-precision SigmaMed
+      {
+        std::vector<double> sres(hd[d].dens);
+        for (int i = 0; i < hd[d].nrad; ++i)
+          for (int j = 0; j < hd[d].nsec; ++j) sres[(size_t)i * hd[d].nsec + j] -= field_base(d, FGX_DENS, i);
+        LCHK(upload_field(d, D.dens, sres));
+      }
       LCHK(upload_field(d, D.vrad, hd[d].vrad));
       std::vector<double> res(hd[d].vtheta);
       for (int i = 0; i < hd[d].nrad; ++i)
@@ -735,6 +743,19 @@ struct Batch : BatchBase {
     return FGX_OK;
   }
 
+  // The ring profile a field is stored as a residual of (vtheta: vref; the
+  // energy: EnergyMed; the density: SigmaMed in the field precision), 0 for
+  // the other fields and for the ghost ring
+  double field_base(int d, int field, int i) const {
+    const HostDisk& h = hd[d];
+    if (i < 0 || i >= h.nrad) return 0.0;
+    switch (field) {
+      case FGX_VTHETA: return vref_h.empty() ? 0.0 : vref_h[d][i];
+      case FGX_ENERGY: return h.P.adiabatic ? h.energymed[i] : 0.0;
+      case FGX_DENS: return (double)(real)h.sigmamed[i];
+      default: return 0.0;
+    }
+  }
   template <class V>
   int upload_vec(real* dst, const V& src, size_t n) {
     std::vector<real> tmp(n);
@@ -756,20 +777,46 @@ struct Batch : BatchBase {
       const int o = dch[d].ro;
       const size_t n = (size_t)h.nrad + 1;
       // FP64 tables: reference vtheta (the initial, ring-uniform profile),
-      // star gravity between ring centres, geometry
-      std::vector<double> vref(n, 0.0), gstar(n, 0.0), invrinf(n, 0.0), cs2(n, 0.0), invrmed(n, 0.0);
+      // geometry, and Substep1's ring-profile terms
+      std::vector<double> vref(n, 0.0), acc0(n, 0.0), u0(n, 0.0), gq0(n, 0.0), invrinf(n, 0.0), cs2(n, 0.0),
+          invrmed(n, 0.0);
       for (int i = 0; i < h.nrad; ++i) cs2[i] = h.cs_ring[i] * h.cs_ring[i];
       LCHK(upload_d(D.cs2_d + o, cs2));
       LCHK(upload_d(D.sigmamed_d + o, h.sigmamed)); LCHK(upload_d(D.energymed_d + o, h.energymed));
       for (int i = 0; i < h.nrad; ++i) vref[i] = h.vtheta[(size_t)i * h.nsec];
-      // the star's radial gravity between ring centres, -(phi_i - phi_i-1) /
-      // (r_i - r_i-1) with phi = -G/r, is exactly G / (r_i r_i-1): formed as that
-      // product it carries one rounding instead of the difference's cancellation
-      for (int i = 1; i < h.nrad; ++i) gstar[i] = 1.0 / (h.rmed[i] * h.rmed[i - 1]);
       for (int i = 0; i < h.nrad; ++i) invrinf[i] = 1.0 / h.rinf[i];
       for (int i = 0; i < h.nrad; ++i) invrmed[i] = 1.0 / h.rmed[i];
+      // Substep1's radial acceleration (Kernels.cu:52-68) of the ring profiles
+      // themselves (density S0, the field-precision SigmaMed; energy EnergyMed;
+      // vtheta vref; frame OmegaFrame at set-up; star potential -G/r), at each
+      // interface i, in long double:
+      //   acc0 = -(P0_i - P0_i-1) 2 / (S0_i + S0_i-1) / dRmed - (phi_i - phi_i-1) / dRmed
+      //          + u0^2 / Rinf,   u0 = (vref_i + vref_i-1) / 2 + Rinf OmegaFrame
+      // with the same FP64 geometry tables the kernels use.  The kernels add
+      // the change that the fields' departures from the profiles make, so the
+      // O(1) gravity and centrifugal terms, which cancel to O(h^2), never meet
+      // in the field arithmetic.  gq0 = (P0_i - P0_i-1) / (S0_i + S0_i-1).
+      {
+        typedef long double W;
+        const W g1 = (W)(h.P.adiabaticindex - 1.0);
+        auto s0 = [&](int i) -> W { return (W)(real)h.sigmamed[i]; };
+        auto p0 = [&](int i) -> W {
+          return h.P.adiabatic ? g1 * (W)h.energymed[i] : s0(i) * (W)h.cs_ring[i] * (W)h.cs_ring[i];
+        };
+        for (int i = 1; i < h.nrad; ++i) {
+          const W idr = (W)h.invdiffrmed[i];
+          const W ss = s0(i) + s0(i - 1), dp = p0(i) - p0(i - 1);
+          const W gradp = dp * (W)2 / ss * idr;
+          const W gradphi = ((W)-1 / (W)h.rmed[i] + (W)1 / (W)h.rmed[i - 1]) * idr;
+          const W u = ((W)2 * (W)vref[i] + (W)2 * (W)vref[i - 1]) / (W)4 + (W)h.rinf[i] * (W)h.omega_frame;
+          acc0[i] = (double)(-gradp - gradphi + u * u * (W)invrinf[i]);
+          u0[i] = (double)u;
+          gq0[i] = (double)(dp / ss);
+        }
+      }
       vref_h[d] = vref;
-      LCHK(upload_d(D.vref + o, vref)); LCHK(upload_d(D.gstar + o, gstar));
+      LCHK(upload_d(D.vref + o, vref)); LCHK(upload_d(D.acc0 + o, acc0)); LCHK(upload_d(D.u0 + o, u0));
+      LCHK(upload_d(D.gq0 + o, gq0));
       LCHK(upload_d(D.rinf_d + o, h.rinf)); LCHK(upload_d(D.invrinf_d + o, invrinf));
       LCHK(upload_d(D.rmed_d + o, h.rmed)); LCHK(upload_d(D.invdiffrmed_d + o, h.invdiffrmed));
       LCHK(upload_d(D.invrmed_d + o, invrmed));
@@ -1029,12 +1076,10 @@ struct Batch : BatchBase {
       HIPCHK(hipMemcpyAsync(tmp.data(), src + dch[d].fo, nfull * sizeof(real), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
-    if (field == FGX_VTHETA || (field == FGX_ENERGY && D.adiabatic)) {
-      // vtheta and the energy are stored as residuals of ring profiles
-      const std::vector<double>& prof = field == FGX_VTHETA ? vref_h[d] : h.energymed;
+    if (field == FGX_VTHETA || field == FGX_DENS || (field == FGX_ENERGY && D.adiabatic)) {
+      // vtheta, the density and the energy are stored as residuals of ring profiles
       for (size_t k = 0; k < n; ++k) {
-        const size_t i = k / h.nsec;
-        const double v = (double)tmp[k] + (i < (size_t)h.nrad ? prof[i] : 0.0);
+        const double v = (double)tmp[k] + field_base(d, field, (int)(k / h.nsec));
         if (dtype == FGX_F32) ((float*)host)[k] = (float)v; else ((double*)host)[k] = v;
       }
       return FGX_OK;
@@ -1056,9 +1101,7 @@ struct Batch : BatchBase {
     std::vector<real> tmp(n);
     for (size_t k = 0; k < n; ++k) {
       double v = dtype == FGX_F32 ? (double)((const float*)host)[k] : ((const double*)host)[k];
-      const size_t i = k / h.nsec;
-      if (field == FGX_VTHETA) v -= vref_h[d][i];
-      if (field == FGX_ENERGY && i < (size_t)h.nrad) v -= h.energymed[i];
+      v -= field_base(d, field, (int)(k / h.nsec));
       tmp[k] = (real)v;
     }
     HIPCHK(hipMemcpyAsync(dst + dch[d].fo, tmp.data(), n * sizeof(real), hipMemcpyHostToDevice, stream));
@@ -1242,7 +1285,7 @@ struct Batch : BatchBase {
     const std::vector<real*> fields = ckpt_fields();
     real* rings[2] = {D.csmean, D.visc};
     CkptHeader hdr{};
-    std::memcpy(hdr.magic, "FGXCKPT2", 8);
+    std::memcpy(hdr.magic, "FGXCKPT3", 8);
     hdr.real_bytes = (int32_t)sizeof(real); hdr.nrad = h.nrad; hdr.nsec = h.nsec;
     hdr.adiabatic = D.adiabatic; hdr.nfields = (int32_t)fields.size(); hdr.nrings = 2;
     hdr.state_bytes = (uint32_t)sizeof(DiskState);
@@ -1284,7 +1327,7 @@ struct Batch : BatchBase {
     const size_t want = sizeof(CkptHeader) + sizeof(DiskState) + (2 * nrr + fields.size() * nf) * sizeof(real) + 8;
     CkptHeader hdr{};
     if (buf.size() >= sizeof hdr) std::memcpy(&hdr, buf.data(), sizeof hdr);
-    if (buf.size() < sizeof hdr || std::memcmp(hdr.magic, "FGXCKPT2", 8) != 0)
+    if (buf.size() < sizeof hdr || std::memcmp(hdr.magic, "FGXCKPT3", 8) != 0)
       return set_error(FGX_EIO, "'%s' is not a checkpoint", path);
     if (hdr.real_bytes != (int32_t)sizeof(real) || hdr.nrad != h.nrad || hdr.nsec != h.nsec ||
         hdr.adiabatic != D.adiabatic || hdr.nfields != (int32_t)fields.size() || hdr.nrings != 2 ||

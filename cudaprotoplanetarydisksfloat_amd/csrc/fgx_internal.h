@@ -95,6 +95,7 @@ struct DiskConst {
   double transitionradius, transitionratio, transitionwidth, lambdadoubling;
   double cavityradius, cavityratio, cavitywidth, viscosity, alphaviscosity;
   double hill_radius, sigma0, eccentricity;
+  double omega0;           // OmegaFrame at set-up: the frame of Substep1's profile tables
   int substeps_forced;     // SUBSTEPS key, 0 = CFL
   int visc_dynamic;        // recompute the viscosity profile each substep
   double sg_eps, sg_rstep, sg_tstep;

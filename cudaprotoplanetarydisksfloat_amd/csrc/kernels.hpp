@@ -37,9 +37,11 @@ struct Dev {
   real *cosj, *sinj, *lost;
   // FP64 ring/sector tables for the terms whose FP32 rounding would dominate:
   // the reference azimuthal velocity profile (vtheta and the angular momenta
-  // are stored as FP32 residuals from it), the star's radial gravity between
-  // ring centres, and the cell geometry of the potential
-  const double *vref, *gstar, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d,
+  // are stored as FP32 residuals from it), the cell geometry of the
+  // potential, and Substep1's ring-profile terms: acc0 the radial acceleration
+  // of the ring profiles themselves at interface i, u0 their interface
+  // azimuthal velocity, gq0 their pressure difference over the density sum
+  const double *vref, *acc0, *u0, *gq0, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d,
       *energymed_d;
   double* vmoy;  // ring-mean vtheta of the CFL condition
   double* phi;   // potential of the planets and the indirect term, FP64 field
@@ -102,6 +104,19 @@ template <class real>
 __device__ __forceinline__ real qdiv(real a, real b) {
   if constexpr (sizeof(real) == 4) return a * __builtin_amdgcn_rcpf(b);
   else return a / b;
+}
+// The density field holds Sigma - S0(ring), S0 the field-precision value of
+// SigmaMed (the ring profile of InitGasDensity, Theo.cu:13-30): the density's
+// rounding is then relative to its departure from the profile.  S0 is exactly
+// representable in the field precision, so the whole value of a residual x is
+// S0 + x with one rounding (or exactly, in FP64 arithmetic).
+template <class real>
+__device__ __forceinline__ real s_whole(const Dev<real>& D, int rb, int i, real x) {
+  return RT(sigmamed)[rb + i] + x;
+}
+template <class real>
+__device__ __forceinline__ double s_whole_d(const Dev<real>& D, int rb, int i, real x) {
+  return (double)RT(sigmamed)[rb + i] + (double)x;
 }
 // The energy field holds e - EnergyMed(ring): a residual of the FP64 ring
 // profile (Theo.cu:33-51), so that its rounding, and that of the
@@ -298,7 +313,8 @@ __device__ __forceinline__ FieldRef<T> field_at(T* base, int soff, unsigned voff
 // potential of the planets and the indirect term at cell (i,j) in FP64:
 // FillForcesArraysKernel (Kernels.cu:1322-1349) for all planets in one
 // evaluation; the star's -1/r is handled per ring (its radial difference is
-// the FP64 table gstar, its azimuthal difference vanishes)
+// part of the ring-profile acceleration acc0, its azimuthal difference
+// vanishes)
 // 1/sqrt(x) in FP64: the hardware estimate refined by two Newton steps
 // (relative error below 1e-16; the estimate alone is good to ~2^-29)
 __device__ __forceinline__ double rsqrt_d(double x) {
@@ -351,7 +367,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   const int lane = (int)threadIdx.x;
   // disk forces on the targets (k = 0 or k = 10 Hill cut), per ring in FP64
   if (mode == 1 || !c.accretes) {
-    const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
+    const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i], s0 = RT(sigmamed)[rb + i];
     double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
     const int nt = s.ntarget;
     for (int q = 0; q < nt; ++q) {
@@ -360,7 +376,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
       double fx = 0, fy = 0;
 #pragma unroll 4
       for (int j = lane; j < ns; j += 64) {
-        const real cm = surf * (real)AT(D.dens, i, j);
+        const real cm = surf * (s0 + (real)AT(D.dens, i, j));
         const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
         const real d2 = dx * dx + dy * dy;
         const real dist2 = d2 + sm2;
@@ -841,16 +857,18 @@ __global__ void __launch_bounds__(FGX_TPB) k_accrete(Dev<real> D) {
       const double vr = 0.5 * ((double)(real)AT(D.vrad, i, jf) + (double)(real)AT(D.vrad, i + 1, jf));
       const double vx = (vr * xc - vt * yc) / rmd, vy = (vr * yc + vt * xc) / rmd;
       const double surf = (double)RT(surf)[rb + i];
-      double den = (double)(real)AT(D.dens, i, jf);
+      // the density field is a residual of S0 (rounded as a residual)
+      const double s0 = (double)RT(sigmamed)[rb + i];
+      double den = s0 + (double)(real)AT(D.dens, i, jf);
       double m = facc1 * den * surf;
-      den = (double)(real)(den * (1.0 - facc1));
+      den = s0 + (double)(real)(den * (1.0 - facc1) - s0);
       dpx += m * vx; dpy += m * vy; dm += m;
       if (dist < 0.45 * rroche) {
         m = facc2 * den * surf;
-        den = (double)(real)(den * (1.0 - facc2));
+        den = s0 + (double)(real)(den * (1.0 - facc2) - s0);
         dpx += m * vx; dpy += m * vy; dm += m;
       }
-      AT(D.dens, i, jf) = (real)den;
+      AT(D.dens, i, jf) = (real)(den - s0);
     }
     // the sums also order this planet's density updates before the next one's
     dm = block_sum(dm, redd);
@@ -900,7 +918,8 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_sound_speed(Dev<real> D) {
   FGX_STRIP_BEGIN {
     const real g = (real)c.adiabaticindex;
-    AT(D.cs, i, j) = sqrt(g * (g - (real)1) * e_whole(D, rb, i, (real)AT(D.energy, i, j)) / AT(D.dens, i, j));
+    AT(D.cs, i, j) = sqrt(g * (g - (real)1) * e_whole(D, rb, i, (real)AT(D.energy, i, j)) /
+                          s_whole(D, rb, i, (real)AT(D.dens, i, j)));
   }
 }
 
@@ -914,40 +933,41 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
   FGX_STRIP_BEGIN {
     const double dt = s.dt;
-    const double om = s.omega_frame;
     const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-    const double rmed = RT(rmed_d)[rb + i];
-    // pressure in FP64 from the FP32 state: Sigma*cs^2 with the exact
-    // isothermal cs^2 of the ring, or (gamma-1)*e
+    // the pressure's departure from the ring profile's, in FP64 from the
+    // residual fields: (Sigma - S0) cs^2 with the exact isothermal cs^2 of the
+    // ring, or (gamma - 1)(e - EnergyMed)
     const double g1 = c.adiabaticindex - 1.0;
     auto press = [&](int ii, int jj) -> double {
-      if (D.adiabatic) return g1 * (RT(energymed_d)[rb + ii] + (double)AT(D.energy, ii, jj));
+      if (D.adiabatic) return g1 * (double)AT(D.energy, ii, jj);
       return (double)AT(D.dens, ii, jj) * RT(cs2_d)[rb + ii];
     };
-    const real sig = AT(D.dens, i, j);
+    const real sr = AT(D.dens, i, j);
+    const real sig = s_whole(D, rb, i, sr);
     const double p = press(i, j);
     const double phi = AT(D.phi, i, j);
-    if (D.write_pressure) AT(D.pressure, i, j) = (real)p;
-    // radial
+    if (D.write_pressure) AT(D.pressure, i, j) = (real)((double)RT(sigmamed)[rb + i] * RT(cs2_d)[rb + i] + p);
+    // radial: the ring profiles' own acceleration acc0 (formed at set-up in
+    // extended precision, where the O(1) gravity and centrifugal terms cancel)
+    // plus the change the departures from the profiles make
     if (i > 0) {
-      const real sigm = AT(D.dens, i - 1, j);
+      const real srm = AT(D.dens, i - 1, j);
+      const real sigm = s_whole(D, rb, i - 1, srm);
       const double pm = press(i - 1, j);
       const double phim = AT(D.phi, i - 1, j);
       const double idr = RT(invdiffrmed_d)[rb + i];
-      const double gradp = (p - pm) * 2.0 / ((double)sig + (double)sigm) * idr;
-      const double gradphi = (phi - phim) * idr + RT(gstar)[rb + i];
-      double vt2 = 2.0 * RT(vref)[rb + i] + 2.0 * RT(vref)[rb + i - 1] +
-                   (double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) +
-                   (double)AT(D.vtheta, i - 1, j) + (double)AT(D.vtheta, i - 1, jp);
-      vt2 = vt2 / 4.0 + RT(rinf_d)[rb + i] * om;
-      vt2 = vt2 * vt2;
-      AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * (-gradp - gradphi + vt2 * RT(invrinf_d)[rb + i]));
+      const double dgp = (p - pm - RT(gq0)[rb + i] * ((double)sr + (double)srm)) * 2.0 / ((double)sig + (double)sigm) * idr;
+      const double dphi = (phi - phim) * idr;
+      const double sv = ((double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) + (double)AT(D.vtheta, i - 1, j) +
+                         (double)AT(D.vtheta, i - 1, jp)) / 4.0 + RT(rinf_d)[rb + i] * (s.omega_frame - c.omega0);
+      const double acc = RT(acc0)[rb + i] - dgp - dphi + sv * (2.0 * RT(u0)[rb + i] + sv) * RT(invrinf_d)[rb + i];
+      AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * acc);
     } else {
       AT(D.vradint, i, j) = 0;
     }
     // azimuthal (the star's potential is the same at j and j-1)
     const real invdxt = (real)1 / RT(dxtheta)[rb + i];
-    const real sigl = AT(D.dens, i, jm);
+    const real sigl = s_whole(D, rb, i, (real)AT(D.dens, i, jm));
     const double pl = press(i, jm);
     const double phil = AT(D.phi, i, jm);
     const real gradp = azim_gradp(p, pl, sig, sigl) * invdxt;
@@ -976,7 +996,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
     real divv = (vr1 * RT(rsup)[rb + i] - vr0 * RT(rinf)[rb + i]) * RT(invdiffrsup)[rb + i] * irm;
     divv += (vtp - vt0) * invdphi * irm;
     const real nu = RT(visc)[rb + i];
-    const real sig = AT(D.dens, i, j);
+    const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
     const real third = (real)(1.0 / 3.0);
     AT(D.taurr, i, j) = (real)2 * sig * nu * (drr - third * divv);
     AT(D.taupp, i, j) = (real)2 * sig * nu * (dpp - third * divv);
@@ -990,8 +1010,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
                                     (vr0 - AT(vr, i, jm)) * invdphi * RT(invrinf)[rb + i]);
       // corner density: column 0 of ring i as the reference indexes it
       // (Kernels.cu:876), or column j-1 with the quirk fix
-      const real sc = D.fix_taurp ? AT(D.dens, i, jm) : AT(D.dens, i, 0);
-      AT(D.taurp, i, j) = (real)(2.0 * 0.25) * (sig + AT(D.dens, i - 1, j) + sc + AT(D.dens, i - 1, jm)) * nu * drp;
+      const real sc = s_whole(D, rb, i, D.fix_taurp ? (real)AT(D.dens, i, jm) : (real)AT(D.dens, i, 0));
+      const real sm_ = s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, j)), sml = s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, jm));
+      AT(D.taurp, i, j) = (real)(2.0 * 0.25) * (sig + sm_ + sc + sml) * nu * drp;
     } else {
       AT(D.taurp, i, j) = 0;
     }
@@ -1012,7 +1033,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
           qdiv(dt * RT(invrmed)[rb + i] *
                    ((RT(rsup)[rb + i] * trp1 - RT(rinf)[rb + i] * trp0) * RT(invdiffrsup)[rb + i] +
                     (AT(D.taupp, i, j) - AT(D.taupp, i, jm)) * invdphi + (real)0.5 * (trp0 + trp1)),
-               (real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i, jm)));
+               (real)0.5 * (s_whole(D, rb, i, (real)AT(D.dens, i, j)) + s_whole(D, rb, i, (real)AT(D.dens, i, jm))));
     }
     if (D.viscous && i > 0) {
       AT(D.vradint, i, j) +=
@@ -1021,7 +1042,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
                         RT(invdiffrmed)[rb + i] +
                     (AT(D.taurp, i, jp) - AT(D.taurp, i, j)) * invdphi -
                     (real)0.5 * (AT(D.taupp, i, j) + AT(D.taupp, i - 1, j))),
-               (real)0.5 * (AT(D.dens, i, j) + AT(D.dens, i - 1, j)));
+               (real)0.5 * (s_whole(D, rb, i, (real)AT(D.dens, i, j)) + s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, j))));
     }
     if (!D.evanescent) {
       // with self-gravity the Keplerian values include -r * <SG_Accr>_ring
@@ -1048,19 +1069,19 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep2(Dev<real> D) {
     const real dt = (real)s.dt;
     const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
     const real cv2 = (real)(kCVNR * kCVNR);
-    const real sig = AT(D.dens, i, j);
+    const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
     const real vri = AT(D.vradint, i, j), vrp = AT(D.vradint, i + 1, j);
     real dv = vrp - vri;
     const real qr = dv < 0 ? cv2 * sig * dv * dv : (real)0;
     const real vti = AT(D.vthetaint, i, j), vtp = AT(D.vthetaint, i, jp), vtm = AT(D.vthetaint, i, jm);
     dv = vtp - vti;
     const real qt = dv < 0 ? cv2 * sig * dv * dv : (real)0;
-    const real sigl = AT(D.dens, i, jm);
+    const real sigl = s_whole(D, rb, i, (real)AT(D.dens, i, jm));
     dv = vti - vtm;
     const real qtl = dv < 0 ? cv2 * sigl * dv * dv : (real)0;
     const real invdxt = (real)1 / RT(dxtheta)[rb + i];
     if (i > 0) {
-      const real sigm = AT(D.dens, i - 1, j);
+      const real sigm = s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, j));
       dv = vri - AT(D.vradint, i - 1, j);
       const real qrm = dv < 0 ? cv2 * sigm * dv * dv : (real)0;
       AT(D.vrad, i, j) = vri - qdiv(dt * (real)2, sig + sigm) * (qr - qrm) * RT(invdiffrmed)[rb + i];
@@ -1080,7 +1101,7 @@ template <class real>
 __device__ __forceinline__ real qplus_at(const Dev<real>& D, const DiskConst& c, int rb, long long fb, int ns, int i, int j) {
   const real nu = RT(visc)[rb + i];
   if (nu == (real)0) return 0;
-  const real sig = AT(D.dens, i, j);
+  const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
   const real trr = AT(D.taurr, i, j), trp = AT(D.taurp, i, j), tpp = AT(D.taupp, i, j), dv = AT(D.divv, i, j);
   real q = (real)0.5 / nu / sig * (trr * trr + trp * trp + tpp * tpp);
   q += (real)(2.0 / 9.0) * nu * sig * dv * dv;
@@ -1106,9 +1127,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
       const real gd = g1 * dt * dvv;
       en = (dt * qp + eint - e0 * gd) / ((real)1 + gd);
     } else {
-      const real sig = AT(D.dens, i, j);
-      const real tc = RT(coolingtime)[rb + i], sm = RT(sigmamed)[rb + i];
-      const real num = e0 * dt * (sig - sm) / sm + tc * eint + dt * tc * (qp - RT(qplusmed)[rb + i] * sig / sm) -
+      // the density residual is Sigma - SigmaMed
+      const real sres = AT(D.dens, i, j), sm = RT(sigmamed)[rb + i], sig = sm + sres;
+      const real tc = RT(coolingtime)[rb + i];
+      const real num = e0 * dt * sres / sm + tc * eint + dt * tc * (qp - RT(qplusmed)[rb + i] * sig / sm) -
                        e0 * g1 * dt * tc * dvv;
       const real den = dt + tc + g1 * dt * tc * dvv;
       en = num / den;
@@ -1129,10 +1151,12 @@ __global__ void __launch_bounds__(FGX_TPB) k_open_boundary(Dev<real> D) {
   const real sm0 = RT(sigmamed)[rb];
   // energy residuals move between rings with the profile difference
   const double de01 = D.adiabatic ? RT(energymed_d)[rb + 1] - RT(energymed_d)[rb] : 0.0;
+  const double ds01 = (double)RT(sigmamed)[rb + 1] - (double)RT(sigmamed)[rb];
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    AT(D.dens, 0, j) = AT(D.dens, 1, j);
+    const real s1 = AT(D.dens, 1, j);
+    AT(D.dens, 0, j) = (real)(ds01 + (double)s1);
     if (D.adiabatic) AT(D.energy, 0, j) = (real)(de01 + (double)AT(D.energy, 1, j));
-    if (AT(D.vrad, 2, j) > 0 || AT(D.dens, 1, j) < sm0) AT(D.vrad, 1, j) = 0;
+    if (AT(D.vrad, 2, j) > 0 || s_whole(D, rb, 1, s1) < sm0) AT(D.vrad, 1, j) = 0;
     else AT(D.vrad, 1, j) = AT(D.vrad, 2, j);
   }
 }
@@ -1172,21 +1196,23 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
   // energy residuals move between rings with the profile difference
   const double de01 = D.adiabatic ? RT(energymed_d)[rb + 1] - RT(energymed_d)[rb] : 0.0;
   const double deo = D.adiabatic ? RT(energymed_d)[rb + nr - 2] - RT(energymed_d)[rb + nr - 1] : 0.0;
+  const double ds01 = (double)RT(sigmamed)[rb + 1] - (double)RT(sigmamed)[rb];
+  const double dso = (double)RT(sigmamed)[rb + nr - 2] - (double)RT(sigmamed)[rb + nr - 1];
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     int jp = j + ia;
     if (jp >= ns) jp -= ns;
     if (jp < 0) jp += ns;
-    AT(D.dens, 0, jp) = AT(D.dens, 1, j);
+    AT(D.dens, 0, jp) = (real)(ds01 + (double)AT(D.dens, 1, j));
     if (D.adiabatic) AT(D.energy, 0, jp) = (real)(de01 + (double)AT(D.energy, 1, j));
-    const real vm = -csv(1, j) * (AT(D.dens, 1, j) - sm1) / sm1;
+    const real vm = -csv(1, j) * AT(D.dens, 1, j) / sm1;   // (Sigma - SigmaMed) / SigmaMed
     AT(D.vrad, 1, j) = (real)2 * vm - AT(D.vrad, 2, j);
     const int io = nr - 1;
     int jo = j - ia2;
     if (jo >= ns) jo -= ns;
     if (jo < 0) jo += ns;
-    AT(D.dens, io, j) = AT(D.dens, io - 1, jo);
+    AT(D.dens, io, j) = (real)(dso + (double)AT(D.dens, io - 1, jo));
     if (D.adiabatic) AT(D.energy, io, j) = (real)(deo + (double)AT(D.energy, io - 1, jo));
-    const real vm2 = csv(io, j) * (AT(D.dens, io - 1, j) - sm2) / sm2;
+    const real vm2 = csv(io, j) * AT(D.dens, io - 1, j) / sm2;
     AT(D.vrad, io, j) = (real)2 * vm2 - AT(D.vrad, io - 1, j);
   }
   __syncthreads();
@@ -1197,8 +1223,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
   }
   const double md = block_sum(m0, redd) / ns, md2 = block_sum(m1, redd) / ns;
   const double me = block_sum(e0, redd) / ns, me2 = block_sum(e1, redd) / ns;
-  // ring means restored to the initial profile, in FP64
-  const double cd = RT(sigmamed_d)[rb] - md, cd2 = RT(sigmamed_d)[rb + nr - 1] - md2;
+  // ring means restored to the initial profile, in FP64 (md, md2: means of
+  // the residuals of S0)
+  const double cd = (RT(sigmamed_d)[rb] - (double)RT(sigmamed)[rb]) - md;
+  const double cd2 = (RT(sigmamed_d)[rb + nr - 1] - (double)RT(sigmamed)[rb + nr - 1]) - md2;
   // (the energy field is a residual of the profile: its ring mean goes to 0)
   const double ce = D.adiabatic ? -me : 0.0;
   const double ce2 = D.adiabatic ? -me2 : 0.0;
@@ -1236,7 +1264,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
     const double inv = 1.0 / (1.0 + lambda);
     AT(D.vrad, i, j) = (real)(((double)AT(D.vrad, i, j) + lambda * vr0) * inv);
     AT(D.vtheta, i, j) = (real)(((double)AT(D.vtheta, i, j) + lambda * (vt0 - RT(vref)[rb + i])) * inv);
-    AT(D.dens, i, j) = (real)(((double)AT(D.dens, i, j) + lambda * RT(sigmamed_d)[rb + i]) * inv);
+    AT(D.dens, i, j) =
+        (real)(((double)AT(D.dens, i, j) + lambda * (RT(sigmamed_d)[rb + i] - (double)RT(sigmamed)[rb + i])) * inv);
     // the energy relaxes to its profile EnergyMed, of which it is stored as a
     // residual: (e + lambda e0) / (1 + lambda) - e0 = (e - e0) / (1 + lambda)
     if (D.adiabatic) AT(D.energy, i, j) = (real)((double)AT(D.energy, i, j) * inv);
@@ -1260,7 +1289,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_outer_source(Dev<real> D) {
   double a = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) a += (double)(real)AT(D.dens, i, j);
   a = block_sum(a, redd);
-  const double add = RT(sigmamed_d)[c.ro + i] - a / (double)ns;
+  // a: sum of the residuals of S0
+  const double add = (RT(sigmamed_d)[c.ro + i] - (double)RT(sigmamed)[c.ro + i]) - a / (double)ns;
   const real vr = (real)c.outer_vr;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     AT(D.dens, i, j) = (real)((double)(real)AT(D.dens, i, j) + add);
@@ -1325,8 +1355,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   const int rb = c.ro;
   if (i >= nr) return;
   const int MS = D.max_nsec;
+  // sD1, sD2 hold the density as the field does, a residual of S0 = s0
   real* sD1 = reinterpret_cast<real*>(fgx_smem);  // density after the radial pass
-  real* sS1 = sD1 + MS;                            // DensStar, pass 1
+  real* sS1 = sD1 + MS;                            // DensStar, pass 1 (whole)
   real* sD2 = sD1 + 2 * MS;                        // density after pass 1
   real* sS2 = sD1 + 3 * MS;                        // DensStar, pass 2
   real* sV = sD1 + 4 * MS;                         // residual velocity
@@ -1363,9 +1394,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
     sV[j] = v;
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sS1[j] = th_star(sD1, j, ns, sV[j], dt, dxt, invdxt);
+  const real s0 = RT(sigmamed)[rb + i];
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sS1[j] = s0 + th_star(sD1, j, ns, sV[j], dt, dxt, invdxt);
   // density, pass 1 -> sD2
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = sD1[j] / (sD1[j] + (real)1e-20);
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = dens_work(s0 + sD1[j]);
   __syncthreads();
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     const int jp = (j + 1 == ns) ? 0 : j + 1;
@@ -1374,13 +1406,13 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
     sD2[j] = sD1[j] + varq * invsurf;
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sS2[j] = th_star(sD2, j, ns, v2, dt, dxt, invdxt);
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sS2[j] = s0 + th_star(sD2, j, ns, v2, dt, dxt, invdxt);
   __syncthreads();
   // one pass of VanLeerTheta on sQ with divisor Dv, DensStar S, velocity v
   // xb: what the stored quantity lacks of the whole one (the energy profile
   // for the energy residual)
   auto pass = [&](const real* Dv, const real* S, const real* varr, real vu, real xb) {
-    for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = (sQ[j] + xb) / (Dv[j] + (real)1e-20);
+    for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = (sQ[j] + xb) / (s0 + Dv[j] + (real)1e-20);
     __syncthreads();
     for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
       const int jp = (j + 1 == ns) ? 0 : j + 1;
@@ -1423,7 +1455,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
     }
   }
   // density, pass 2 -> sD1 (the radial-pass density is no longer needed)
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = sD2[j] / (sD2[j] + (real)1e-20);
+  for (int j = threadIdx.x; j < ns; j += FGX_TPB) sA[j] = dens_work(s0 + sD2[j]);
   __syncthreads();
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     const int jp = (j + 1 == ns) ? 0 : j + 1;
@@ -1443,8 +1475,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
     js += (js < 0) ? ns : 0;
     const int jsm = (js == 0) ? ns - 1 : js - 1;
     D.dens[row + j] = sD1[js];
-    if (D.advect_label) D.label[row + j] = sL[js] / sD1[js];   // ComputeSpeQty (Kernels.cu:1309-1319)
-    const double S = (double)sD1[js] + (double)sD1[jsm];
+    if (D.advect_label) D.label[row + j] = sL[js] / (s0 + sD1[js]);   // ComputeSpeQty (Kernels.cu:1309-1319)
+    const double S = 2.0 * (double)s0 + (double)sD1[js] + (double)sD1[jsm];
     D.vtheta[row + j] = (real)(((double)sH[jsm] + (double)sQ[js] - l0 * 1e-15) / (S + 1e-15) / rd);
   }
 }
@@ -1465,11 +1497,12 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_vr(Dev<real> D) {
   if (j >= ns) return;
   const long long fb = c.fo;
   const int i0 = t.i, i_end = (t.i + D.strip_rings < nr) ? t.i + D.strip_rings : nr;
-  real pm = 0, dm = 0;   // RadMomP and Sigma of ring i-1
-  if (i0 > 0) { pm = AT(D.t_rmp, i0 - 1, j); dm = AT(D.dens, i0 - 1, j); }
+  const int rb = c.ro;
+  real pm = 0, dm = 0;   // RadMomP and Sigma of ring i-1 (whole values)
+  if (i0 > 0) { pm = AT(D.t_rmp, i0 - 1, j); dm = s_whole(D, rb, i0 - 1, (real)AT(D.dens, i0 - 1, j)); }
   real nm = AT(D.t_rmm, i0, j), nd = AT(D.dens, i0, j), np = AT(D.t_rmp, i0, j);
   for (int i = i0; i < i_end; ++i) {
-    const real rmm = nm, den = nd, rmp = np;
+    const real rmm = nm, den = s_whole(D, rb, i, nd), rmp = np;
     if (i + 1 < i_end) { nm = AT(D.t_rmm, i + 1, j); nd = AT(D.dens, i + 1, j); np = AT(D.t_rmp, i + 1, j); }
     AT(D.vrad, i, j) = (i == 0) ? (real)0 : (pm + rmm) / (den + dm + (real)1e-20);
     pm = rmp;
@@ -1605,8 +1638,9 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_temperature(Dev<real> D) {
   FGX_CELL_PROLOGUE
   if (i >= nr) return;
-  if (!D.adiabatic) AT(D.work, i, j) = AT(D.pressure, i, j) / AT(D.dens, i, j);
-  else AT(D.work, i, j) = (real)(c.adiabaticindex - 1.0) * e_whole(D, rb, i, (real)AT(D.energy, i, j)) / AT(D.dens, i, j);
+  const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
+  if (!D.adiabatic) AT(D.work, i, j) = AT(D.pressure, i, j) / sig;
+  else AT(D.work, i, j) = (real)(c.adiabaticindex - 1.0) * e_whole(D, rb, i, (real)AT(D.energy, i, j)) / sig;
 }
 // isothermal pressure at set-up (ComputePressureFieldKernel at InitEuler)
 template <class real>
@@ -1615,7 +1649,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_pressure_init(Dev<real> D) {
   if (i >= nr) return;
   if (!D.adiabatic) {
     const real cs = D.cs_ring[rb + i];
-    AT(D.pressure, i, j) = AT(D.dens, i, j) * cs * cs;
+    AT(D.pressure, i, j) = s_whole(D, rb, i, (real)AT(D.dens, i, j)) * cs * cs;
   } else {
     AT(D.pressure, i, j) = (real)(c.adiabaticindex - 1.0) * e_whole(D, rb, i, (real)AT(D.energy, i, j));
   }
@@ -1638,7 +1672,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_force_full(Dev<real> D, int disk, d
   double f[22];
   for (int k = 0; k < 22; ++k) f[k] = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    const real cm = surf * AT(D.dens, i, j);
+    const real cm = surf * s_whole(D, rb, i, (real)AT(D.dens, i, j));
     const real dx = rmed * RT(cosj)[c.so + j] - (real)x, dy = rmed * RT(sinj)[c.so + j] - (real)y;
     const real d2 = dx * dx + dy * dy;
     const real dist2 = d2 + (real)(sm * sm);
@@ -1685,7 +1719,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_mdcp(Dev<real> D, int disk, double 
   double acc = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
     const real dx = rmed * RT(cosj)[c.so + j] - (real)xpl, dy = rmed * RT(sinj)[c.so + j] - (real)ypl;
-    if ((double)sqrt(dx * dx + dy * dy) < rhill) acc += (double)(surf * AT(D.dens, i, j));
+    if ((double)sqrt(dx * dx + dy * dy) < rhill) acc += (double)(surf * s_whole(D, rb, i, (real)AT(D.dens, i, j)));
   }
   acc = block_sum(acc, redd);
   if (threadIdx.x == 0) D.rpart[i] = acc;

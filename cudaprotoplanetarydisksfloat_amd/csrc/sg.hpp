@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_rows_fwd(Dev<real> D) {
   const double q = RT(rmed_d)[rb + i] / RT(rmed_d)[rb];
   const double sq = sqrt(q);
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    const double var = (double)AT(D.dens, i, j) * sq;
+    const double var = s_whole_d(D, rb, i, (real)AT(D.dens, i, j)) * sq;
     a[j] = cxd{var, var * q};
   }
   __syncthreads();
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_rows_inv(Dev<real> D) {
   const double self = kG * c.sg_rstep * c.sg_tstep / c.sg_eps;
   double acc = 0;
   for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    const double gr = out[j].x * nrr + self * (double)AT(D.dens, i, j);
+    const double gr = out[j].x * nrr + self * s_whole_d(D, rb, i, (real)AT(D.dens, i, j));
     AT(D.sg_accr, i, j) = (real)gr;
     AT(D.sg_acct, i, j) = (real)(out[j].y * ntt);
     acc += gr;

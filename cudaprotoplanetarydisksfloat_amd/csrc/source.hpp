@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   const bool out = lane >= 4 && lane <= 60 && first + lane - 4 < ns;
   const int i0 = t.i, i1 = (t.i + rs < nr) ? t.i + rs : nr;
   const bool visc = D.viscous, ad = D.adiabatic;
-  const double dtd = s.dt, om = s.omega_frame;
+  const double dtd = s.dt, dom = s.omega_frame - c.omega0;
   const real dt = (real)dtd;
   const double g1 = c.adiabaticindex - 1.0;
   const real invdphi = (real)((double)ns / (2.0 * kPI));
@@ -64,18 +64,22 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   // per ring: inside the loop they are read from LDS into vector registers,
   // instead of some fifty per-ring values and their table addresses being
   // held in scalar registers across the loop
-  // cd_[0]: the isothermal cs^2, or with an energy equation the energy profile
-  // EnergyMed of which the energy field is a residual
-  __shared__ double cd_[8][FGX_SRC_STAGE];   // cs2_d|energymed_d rmed_d invdiffrmed_d gstar vref rinf_d invrinf_d invrmed_d
-  __shared__ real cf_[10][FGX_SRC_STAGE];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed
+  // acc0, u0, gq0: the ring profile's own radial acceleration, interface
+  // velocity and pressure-gradient coefficient (Substep1 below); S0 the density
+  // profile of which the density field is a residual
+  __shared__ double cd_[10][FGX_SRC_STAGE];  // cs2_d rmed_d invdiffrmed_d acc0 vref rinf_d invrinf_d invrmed_d u0 gq0
+  __shared__ real cf_[11][FGX_SRC_STAGE];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed S0
   const int rlo = i0 - 4;
   {
     const int r = rlo + lane;
     if (r >= 0 && r < nr && lane < rs + 8 && lane < FGX_SRC_STAGE) {
-      cd_[0][lane] = ad ? RT(energymed_d)[rb + r] : RT(cs2_d)[rb + r];
+      cd_[0][lane] = RT(cs2_d)[rb + r];
       cd_[1][lane] = RT(rmed_d)[rb + r];
       cd_[2][lane] = RT(invdiffrmed_d)[rb + r];
-      cd_[3][lane] = RT(gstar)[rb + r];
+      cd_[3][lane] = RT(acc0)[rb + r];
+      cd_[8][lane] = RT(u0)[rb + r];
+      cd_[9][lane] = RT(gq0)[rb + r];
+      cf_[10][lane] = RT(sigmamed)[rb + r];
       cd_[4][lane] = RT(vref)[rb + r];
       cd_[5][lane] = RT(rinf_d)[rb + r];
       cd_[6][lane] = RT(invrinf_d)[rb + r];
@@ -96,8 +100,8 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
 #define FGX_CD(q, r) cd_[q][(r) - rlo]
 #define FGX_CF(q, r) cf_[q][(r) - rlo]
 
-  // ring k-1 inputs of stage A
-  real sgA1 = 0, vtA1 = 0, vtpA1 = 0;
+  // ring k-1 inputs of stage A (sgA1 the whole density, srA1 its residual)
+  real sgA1 = 0, srA1 = 0, vtA1 = 0, vtpA1 = 0;
   double pA1 = 0, phiA1 = 0;
   // density and energy of rings k-1 .. k-4 (index 1..4), A's outputs of
   // rings k-1, k-2, the stresses of rings k-2, k-3, C's outputs of rings
@@ -141,16 +145,19 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     pot -= indx * x + indy * y;
     return pot;
   };
-  auto pressure = [&](real sg, real e, int k) -> double {
-    return ad ? g1 * (FGX_CD(0, k) + (double)e) : (double)sg * FGX_CD(0, k);
+  // the pressure's departure from the ring profile's: (Sigma - S0) cs^2, or
+  // (gamma - 1)(e - EnergyMed), from the residual fields
+  auto pressure = [&](real sr, real e, int k) -> double {
+    return ad ? g1 * (double)e : (double)sr * FGX_CD(0, k);
   };
   const int ks = (i0 - 2 > 0) ? i0 - 2 : 0;
   if (ks > 0) {   // prime stage A with ring ks-1
     const int k = ks - 1;
-    sgA1 = AT(D.dens, k, j);
+    srA1 = AT(D.dens, k, j);
+    sgA1 = FGX_CF(10, k) + srA1;
     vtA1 = AT(D.vtheta, k, j);
     vtpA1 = AT(D.vtheta, k, jp);
-    pA1 = pressure(sgA1, ad ? (real)AT(D.energy, k, j) : (real)0, k);
+    pA1 = pressure(srA1, ad ? (real)AT(D.energy, k, j) : (real)0, k);
     phiA1 = potential(FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
   }
   // past the outer edge A and B are skipped and C yields VradInt(nrad) = 0
@@ -173,8 +180,8 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     ne = ad ? (real)AT(D.energy, kk, j) : (real)0;
     nc0 = AT(D.dens, kk, 0);
   };
-  real c0A1 = 0;   // column-0 density of ring k-1
-  if (ks > 0) c0A1 = AT(D.dens, ks - 1, 0);
+  real c0A1 = 0;   // column-0 density of ring k-1 (whole)
+  if (ks > 0) c0A1 = FGX_CF(10, ks - 1) + (real)AT(D.dens, ks - 1, 0);
   fetch(ks);
   for (int k = ks; k <= ke; ++k) {
     // ---------------------------------------------------------- stage A
@@ -182,22 +189,25 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne, cc0 = nc0;
     fetch(k + 1);
     if (k < nr && k <= i1 + 1) {
-      sg0 = csg;
+      const real sr = csg;
+      sg0 = FGX_CF(10, k) + sr;
       const real vr = cvr;
       const real vt = cvt, vtp = cvtp;
       if (ad) e0 = ce;
-      const double p = pressure(sg0, e0, k);
+      const double p = pressure(sr, e0, k);
       const double phi = potential(FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
-      if (D.write_pressure && out && k >= i0 && k < i1) AT(D.pressure, k, j) = (real)p;
+      if (D.write_pressure && out && k >= i0 && k < i1) AT(D.pressure, k, j) = (real)((double)FGX_CF(10, k) * FGX_CD(0, k) + p);
       if (k > 0) {
+        // Substep1's radial acceleration -gradP/Sigma - gradPhi + vt^2/Rinf,
+        // formed as the ring profile's own (acc0, exact, from the set-up) plus
+        // the change the departures from the profile make: the O(1) gravity
+        // and centrifugal terms then cancel in the set-up's extended
+        // precision instead of in the field arithmetic
         const double idr = FGX_CD(2, k);
-        const double gradp = (p - pA1) * 2.0 / ((double)sg0 + (double)sgA1) * idr;
-        const double gradphi = (phi - phiA1) * idr + FGX_CD(3, k);
-        double vt2 = 2.0 * FGX_CD(4, k) + 2.0 * FGX_CD(4, k - 1) + (double)vt + (double)vtp +
-                     (double)vtA1 + (double)vtpA1;
-        vt2 = vt2 / 4.0 + FGX_CD(5, k) * om;
-        vt2 = vt2 * vt2;
-        vri0 = (real)((double)vr + dtd * (-gradp - gradphi + vt2 * FGX_CD(6, k)));
+        const double dgp = (p - pA1 - FGX_CD(9, k) * ((double)sr + (double)srA1)) * 2.0 / ((double)sg0 + (double)sgA1) * idr;
+        const double dphi = (phi - phiA1) * idr;
+        const double sv = ((double)vt + (double)vtp + (double)vtA1 + (double)vtpA1) / 4.0 + FGX_CD(5, k) * dom;
+        vri0 = (real)((double)vr + dtd * (FGX_CD(3, k) - dgp - dphi + sv * (2.0 * FGX_CD(8, k) + sv) * FGX_CD(6, k)));
       }
       // azimuthal (the star's potential is the same at j and j-1)
       const real invdxt = FGX_CF(0, k);
@@ -209,7 +219,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
       real v = vt - (real)dtd * (gradp + gradphi);
       v += (real)dtd * FGX_CF(1, k);
       vti0 = v;
-      sgA1 = sg0; vtA1 = vt; vtpA1 = vtp; pA1 = p; phiA1 = phi;
+      sgA1 = sg0; srA1 = sr; vtA1 = vt; vtpA1 = vtp; pA1 = p; phiA1 = phi;
     }
     // ---------------------------------------------------------- stage B
     const int b = k - 1;
@@ -262,11 +272,11 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
         if (!D.evanescent) {
           if (cr == 0) {
             const double v = s.vkep_in - (D.self_gravity ? RT(rmed_d)[rb] * D.axisg[rb] : 0.0);
-            vtc0 = (real)(sqrt(v) - RT(rmed_d)[rb] * om - RT(vref)[rb]);
+            vtc0 = (real)(sqrt(v) - RT(rmed_d)[rb] * s.omega_frame - RT(vref)[rb]);
           }
           if (cr == nr - 1) {
             const double v = s.vkep_out - (D.self_gravity ? RT(rmed_d)[rb + nr - 1] * D.axisg[rb + nr - 1] : 0.0);
-            vtc0 = (real)(sqrt(v) - RT(rmed_d)[rb + nr - 1] * om - RT(vref)[rb + nr - 1]);
+            vtc0 = (real)(sqrt(v) - RT(rmed_d)[rb + nr - 1] * s.omega_frame - RT(vref)[rb + nr - 1]);
           }
         }
       }
@@ -304,7 +314,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
       }
     }
     // ---------------------------------------------------------- shift
-    c0A1 = cc0;
+    c0A1 = FGX_CF(10, k < nr ? k : nr - 1) + cc0;
     sg4 = sg3; sg3 = sg2; sg2 = sg1; sg1 = sg0;
     e3 = e2; e2 = e1; e1 = e0;
     vri2 = vri1; vri1 = vri0; vti2 = vti1; vti1 = vti0;

@@ -63,9 +63,10 @@ template <class real, bool LAB = false>
 struct RadRing {
   real q[LAB ? 8 : 7];   // 0 density (for DensStar), 1..6 work = quantity / (density + 1e-20):
                          // density, RadMomP, RadMomM, ThetaMomP, ThetaMomM, energy, [label]
-  real sg, vr0, vr1, e;
+  real sg, sr, vr0, vr1, e;   // sg the whole density, sr its residual of S0 (the field's value)
   real x;         // extensive label, density * label (label advection)
-  double tp, tm;  // angular momenta in FP64 (density * (vref + residual + R*Omega) * R)
+  double tp, tm;  // the angular momenta's departures from density * l0, in FP64:
+                  // density * (vtheta residual) * R, l0 = (vref + R*Omega) * R
 };
 
 // the disk's fields seen from its first element; offsets in 32 bits
@@ -74,6 +75,7 @@ struct RadFields {
   const real *dens, *vrad, *vtheta, *energy, *label;
   const FGX_AS4 double *rmed_d, *vref;
   const FGX_AS4 real* emed;   // ring profile the stored energy is a residual of
+  const FGX_AS4 real* smed;   // S0, the ring profile the stored density is a residual of
 };
 
 // the raw values of ring k a thread needs (clamped rings only feed zero
@@ -100,7 +102,8 @@ template <class real, bool LAB = false>
 __device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int k, double om, bool ad,
                                            const RadRaw<real>& w, RadRing<real, LAB>& r) {
   k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
-  const real sg = w.sg;
+  const real s0 = f.smed[k];
+  const real sg = s0 + w.sg;   // the density field holds the residual of S0
   const double rk = f.rmed_d[k];
   const double l0 = (f.vref[k] + rk * om) * rk;
   const double lp = l0 + (double)w.vtp * rk, lm = l0 + (double)w.vtj * rk;
@@ -110,18 +113,18 @@ __device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int
   // quotient.
   const real inv = vl_rcp(sg + (real)1e-20);
   const real w1 = dens_work(sg);
-  r.sg = sg; r.vr0 = w.vr0; r.vr1 = w.vr1;
+  r.sg = sg; r.sr = w.sg; r.vr0 = w.vr0; r.vr1 = w.vr1;
   r.q[0] = sg;
   r.q[1] = w1;
   r.q[2] = work_of(sg * w.vr1, sg, inv);
   r.q[3] = work_of(sg * w.vr0, sg, inv);
-  const double dd = (double)sg;
-  r.tp = dd * lp; r.tm = dd * lm;
+  const double dd = (double)s0 + (double)w.sg;   // the whole density, exactly
+  r.tp = dd * ((double)w.vtp * rk); r.tm = dd * ((double)w.vtj * rk);
   if constexpr (sizeof(real) == 4) {
     // (density * l) / (density + 1e-20): l times the density's work
     r.q[4] = (real)lp * w1; r.q[5] = (real)lm * w1;
   } else {
-    r.q[4] = (real)(r.tp / (dd + 1e-20)); r.q[5] = (real)(r.tm / (dd + 1e-20));
+    r.q[4] = (real)(dd * lp / (dd + 1e-20)); r.q[5] = (real)(dd * lm / (dd + 1e-20));
   }
   // the energy is stored as a residual of its ring profile: its work is
   // formed from the whole value (its flux is small, so one rounding of the
@@ -177,7 +180,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   // quantities in use: 0..5 always, 6 (energy) if adiabatic, 7 (label) if LAB
   auto qon = [&](int q) { return q < 6 || q == 7 || ad; };
   const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0),
-                          LAB ? D.label + fb : nullptr, RT(rmed_d) + rb, RT(vref) + rb, RT(energymed) + rb};
+                          LAB ? D.label + fb : nullptr, RT(rmed_d) + rb, RT(vref) + rb, RT(energymed) + rb,
+                          RT(sigmamed) + rb};
   real* const o_dens = D.t_dens + fb;
   real* const o_rmp = D.t_rmp + fb;
   real* const o_rmm = D.t_rmm + fb;
@@ -265,17 +269,20 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     const real inc = (flux[1] - fnew[1]) * is;
     const real sg = ra.sg;
     const int oc = i * ns + j;
-    o_dens[oc] = sg + inc;
+    o_dens[oc] = ra.sr + inc;   // a residual of S0, as the field
     if (i == 0 && D.open_inner) D.lost[c.so + j] = flux[1] - fnew[1];
     o_rmp[oc] = sg * ra.vr1 + (flux[2] - fnew[2]) * is;
     o_rmm[oc] = sg * ra.vr0 + (flux[3] - fnew[3]) * is;
     // angular momenta leave as residuals of density * l0(ring), with the
-    // density taken before its FP32 rounding, so the azimuthal velocity
-    // rebuilt from them carries no trace of that rounding
-    const double sgx = (double)sg + (double)inc;
+    // density taken before its field rounding, so the azimuthal velocity
+    // rebuilt from them carries no trace of that rounding:
+    //   (L + dL) - (Sigma + dSigma) l0 = Sigma (l - l0) + dL - dSigma l0,
+    // formed from its small terms only (the O(1) products Sigma l and
+    // Sigma l0 would each carry a rounding of the size of a vtheta ulp)
     const double l0i = (vref_i + rd * om) * rd;
-    o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - sgx * l0i);
-    o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - sgx * l0i);
+    const double dsl = (double)inc * l0i;
+    o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - dsl);
+    o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - dsl);
     if (ad) o_e[oc] = ra.e + (flux[6] - fnew[6]) * is;
     if constexpr (LAB) o_lab[oc] = ra.x + (flux[7] - fnew[7]) * is;
     // slide the window
@@ -576,13 +583,15 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   auto both_energy = [&](real(&X)[K]) {
     const real e0 = RT(energymed)[rb + i];
     const real ab = e0 * vl_rcp(sbar);   // any value near the ring's e / density serves
+    const real eb = fmadd(-ab, sbar, e0);   // e0 - ab * sbar, near 0
     real A[K], F[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) A[k] = (e0 + X[k]) * invD1[k];
     star(A, pos1, cf1, F);
     apply(X, F, SV1);
+    // (X + e0 - ab * density) / density with density = sbar + Dn
 #pragma unroll
-    for (int k = 0; k < K; ++k) A[k] = (fmadd(-ab, Dn[k], e0) + X[k]) * invD2[k];
+    for (int k = 0; k < K; ++k) A[k] = (fmadd(-ab, Dn[k], eb) + X[k]) * invD2[k];
     star_u(A, pos2, cf2, F);
 #pragma unroll
     for (int k = 0; k < K; ++k) F[k] = ab * S2r[k] + F[k] * (sbar + S2r[k]);
@@ -601,32 +610,29 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
     cf1[k] = ((ksi > 0) ? dxt - ksi : dxt + ksi) * invdxt;
   }
   // density: DensStar and pass 1 (the density's work is exactly 1 but for
-  // densities below the 1e-20's resolution)
+  // densities below the 1e-20's resolution).  Dn holds the density as the
+  // field does, a residual of sbar = S0: the van Leer slopes of the residual
+  // are those of the density, and its star plus sbar is the density's.
   star(Dn, pos1, cf1, SV1);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    SV1[k] = dxrad * SV1[k] * V[k] * invsurf;
-    invD1[k] = vl_rcp(Dn[k] + (real)1e-20);
+    SV1[k] = dxrad * (sbar + SV1[k]) * V[k] * invsurf;
+    invD1[k] = vl_rcp(sbar + Dn[k] + (real)1e-20);
   }
   {
     real A[K], F[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) A[k] = dens_work(Dn[k]);
+    for (int k = 0; k < K; ++k) A[k] = dens_work(sbar + Dn[k]);
     star(A, pos1, cf1, F);
     apply(Dn, F, SV1);
   }
   // pass-2 interface densities from the density after pass 1, as residuals
-  // of sbar (the slopes are the same)
-  {
-    real R[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) R[k] = Dn[k] - sbar;
-    star_u(R, pos2, cf2, S2r);
-  }
+  // of sbar
+  star_u(Dn, pos2, cf2, S2r);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     SV2[k] = (sbar + S2r[k]) * c2;
-    invD2[k] = vl_rcp(Dn[k] + (real)1e-20);
+    invD2[k] = vl_rcp(sbar + Dn[k] + (real)1e-20);
   }
   // quantities in the order RadMomP, RadMomM, [energy], ThetaMomP (held),
   // ThetaMomM (held); the energy is stored as a residual of its ring profile
@@ -646,7 +652,7 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   {
     real A[K], F[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) A[k] = dens_work(Dn[k]) - (real)1;
+    for (int k = 0; k < K; ++k) A[k] = dens_work(sbar + Dn[k]) - (real)1;
     star_u(A, pos2, cf2, F);
 #pragma unroll
     for (int k = 0; k < K; ++k) F[k] = S2r[k] + F[k] * (sbar + S2r[k]);
@@ -655,7 +661,7 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   store_shift(D.dens, Dn);
   if constexpr (LAB) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) Lq[k] = Lq[k] / Dn[k];
+    for (int k = 0; k < K; ++k) Lq[k] = Lq[k] / (sbar + Dn[k]);
     store_shift(D.label, Lq);
   }
   // azimuthal velocity of ComputeVelocitiesKernel at the face j-1/2:
@@ -670,11 +676,12 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
     // the quotient is a small residual of vtheta: FP32 with a reciprocal
     // keeps it far inside the FP32 field's own rounding
     const real l0e = (real)(l0 * 1e-15);
+    const real sbar2 = sbar + sbar;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const real hm = (k == 0) ? hl : H[k - 1];
       const real dm = (k == 0) ? dl : Dn[k - 1];
-      V[k] = (hm + Qn[k] - l0e) * vl_rcp(Dn[k] + dm + (real)1e-15) * irm;
+      V[k] = (hm + Qn[k] - l0e) * vl_rcp(sbar2 + (Dn[k] + dm) + (real)1e-15) * irm;
     }
   } else {
     const double ird = 1.0 / rd;
@@ -682,7 +689,7 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
     for (int k = 0; k < K; ++k) {
       const double hm = (double)((k == 0) ? hl : H[k - 1]);
       const double dm = (double)((k == 0) ? dl : Dn[k - 1]);
-      V[k] = (real)((hm + (double)Qn[k] - l0 * 1e-15) / ((double)Dn[k] + dm + 1e-15) * ird);
+      V[k] = (real)((hm + (double)Qn[k] - l0 * 1e-15) / (2.0 * (double)sbar + (double)Dn[k] + dm + 1e-15) * ird);
     }
   }
   store_shift(D.vtheta, V);
