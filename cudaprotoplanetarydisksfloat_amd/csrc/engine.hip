@@ -1055,6 +1055,11 @@ struct Batch : BatchBase {
     const size_t n = std::min(count, nfull);
     std::vector<real> tmp(nfull, (real)0);
     real* src = field_ptr(field);
+    if (field == FGX_TEMPERATURE && !D.adiabatic && !D.write_pressure)
+      // ComputeTemperatureField (Kernels.cu:211-223) divides the pressure of
+      // the substep's start by the density of its end; that pressure is kept
+      // only when the temperature is an output
+      return set_error(FGX_EPARAM, "the isothermal temperature needs WRITETEMPERATURE YES");
     if (field == FGX_TEMPERATURE || field == FGX_POTENTIAL) {
       D.ignore_active = 1;
       int r = (field == FGX_TEMPERATURE) ? launch("temperature", 0, k_temperature<real>, n_ctiles, FGX_TPB, 0, D)
