@@ -35,6 +35,8 @@ def load():
         "orc_fargo_shift_f32": ([ctypes.c_float, ctypes.c_float, d, i], i),
         "orc_fargo_shift_f64": ([d, d, d, i], i),
         "orc_fft2d": ([vp, vp, i, i, i], None), "orc_sg_accel": ([vp, vp, vp], i),
+        "orc_get_array": ([vp, ctypes.c_char_p, vp, sz], i), "orc_set_array": ([vp, ctypes.c_char_p, vp, sz], i),
+        "orc_call": ([vp, ctypes.c_char_p, d], i),
         "orc_sizeof_params": ([], sz), "orc_sizeof_planets": ([], sz), "orc_num_threads": ([], i),
     }
     for n, (a, r) in sigs.items():
@@ -124,6 +126,25 @@ class Oracle:
 
     def condition_cfl(self, delta_t):
         return lib().orc_condition_cfl(self.h, delta_t)
+
+    def array(self, name):
+        """a named array of the restatement (fields, work arrays, stresses),
+        shaped (nrad+1, nsec)"""
+        n = (self.nrad + 1) * self.nsec
+        out = np.empty(n)
+        if lib().orc_get_array(self.h, name.encode(), out.ctypes.data, n):
+            raise ValueError(name)
+        return out.reshape(-1, self.nsec)
+
+    def set_array(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        if lib().orc_set_array(self.h, name.encode(), v.ctypes.data, v.size):
+            raise ValueError(name)
+
+    def call(self, name, dt=0.0):
+        """one function of the restatement on the current arrays"""
+        if lib().orc_call(self.h, name.encode(), dt):
+            raise ValueError(name)
 
     def stage(self, name, dt):
         r = lib().orc_stage(self.h, name.encode(), dt)

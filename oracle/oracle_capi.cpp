@@ -85,6 +85,39 @@ void info(D& s, double* o) {
   o[9] = s.Indx; o[10] = s.Indy;
 }
 
+// Named arrays of the restatement, for the pins of single functions
+// (tests/test_oracle_pins.py): the work fields, the stresses, the star values
+template <class D>
+std::vector<typename std::remove_reference<decltype(std::declval<D>().Dens)>::type::value_type>* array_of(D& s,
+                                                                                                       const std::string& n) {
+  if (n == "Dens") return &s.Dens;
+  if (n == "Vrad") return &s.Vrad;
+  if (n == "Vtheta") return &s.Vtheta;
+  if (n == "Energy") return &s.Energy;
+  if (n == "SoundSpeed") return &s.SoundSpeed;
+  if (n == "Pressure") return &s.Pressure;
+  if (n == "Potential") return &s.Potential;
+  if (n == "VradInt") return &s.VradInt;
+  if (n == "VthetaInt") return &s.VthetaInt;
+  if (n == "VradNew") return &s.VradNew;
+  if (n == "VthetaNew") return &s.VthetaNew;
+  if (n == "EnergyInt") return &s.EnergyInt;
+  if (n == "EnergyNew") return &s.EnergyNew;
+  if (n == "DRR") return &s.DRR;
+  if (n == "DPP") return &s.DPP;
+  if (n == "DRP") return &s.DRP;
+  if (n == "DivV") return &s.DivV;
+  if (n == "TAURR") return &s.TAURR;
+  if (n == "TAUPP") return &s.TAUPP;
+  if (n == "TAURP") return &s.TAURP;
+  if (n == "Qplus") return &s.Qplus;
+  if (n == "Work") return &s.Work;
+  if (n == "QRStar") return &s.QRStar;
+  if (n == "dq") return &s.dq;
+  if (n == "VthetaRes") return &s.VthetaRes;
+  return nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -155,6 +188,44 @@ int orc_get_int_velocities(void* hp, double* vr, double* vt, size_t n) {
   });
   return 0;
 }
+int orc_get_array(void* hp, const char* name, double* out, size_t n) {
+  DISPATCH(hp, {
+    auto* v = array_of(s, std::string(name));
+    if (!v || n > v->size()) return FGX_ERANGE;
+    for (size_t k = 0; k < n; ++k) out[k] = (double)(*v)[k];
+  });
+  return 0;
+}
+int orc_set_array(void* hp, const char* name, const double* in, size_t n) {
+  DISPATCH(hp, {
+    auto* v = array_of(s, std::string(name));
+    if (!v || n > v->size()) return FGX_ERANGE;
+    using T = typename std::remove_reference<decltype(*v)>::type::value_type;
+    for (size_t k = 0; k < n; ++k) (*v)[k] = (T)in[k];
+  });
+  return 0;
+}
+// single functions of the restatement on the current arrays
+int orc_call(void* hp, const char* name, double dt) {
+  std::string n(name);
+  DISPATCH(hp, {
+    if (n == "viscous_terms") s.compute_viscous_terms(s.Vrad, s.Vtheta);
+    else if (n == "visc_update") s.update_velocities_with_viscosity(dt);
+    else if (n == "substep1_kernels") s.substep1_kernels(dt);
+    else if (n == "substep2") s.substep2(dt);
+    else if (n == "substep3") s.substep3(dt);
+    else if (n == "star_rad") s.star_rad(s.Work, s.QRStar, dt);
+    else if (n == "star_theta") s.star_theta(s.Work, s.VthetaRes, s.QRStar, dt);
+    else if (n == "open_boundary") s.open_boundary();
+    else if (n == "nonreflecting_boundary") s.nonreflecting_boundary();
+    else if (n == "evanescent_boundary") s.evanescent_boundary(dt);
+    else if (n == "mdcp") s.mdcp1 = s.circumplanetary_mass();
+    else if (n == "fill_forces_arrays") s.fill_forces_arrays();
+    else return -1;
+  });
+  return 0;
+}
+
 // FARGO integer shift with the reference's FP32 promotion order
 // (ComputeConstantResidualKernel, Kernels.cu:1216-1221)
 int orc_fargo_shift_f32(float vmed, float invrmed, double dt, int nsec) {
