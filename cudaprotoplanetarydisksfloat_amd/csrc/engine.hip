@@ -264,6 +264,7 @@ struct Batch : BatchBase {
   bool outer_source = false;    // some disk has the outer mass source (StubbedPhysics)
   size_t theta_shmem = 0;
   std::vector<std::vector<double>> vref_h;  // reference vtheta profile per disk (nrad+1)
+  std::vector<int> health_h;                // FGX_HEALTH_* flags of each disk known to the host
   // profiling
   struct Prof { std::string name; double ms = 0; long long n = 0; double bytes = 0; };
   bool prof_on = false;
@@ -450,6 +451,7 @@ struct Batch : BatchBase {
       c.hill_radius = h.hill_radius; c.sigma0 = p.sigma0; c.eccentricity = p.eccentricity;
       c.omega0 = h.omega_frame;
       c.substeps_forced = p.ext_substeps;
+      c.max_substeps = p.ext_max_substeps > 0 ? p.ext_max_substeps : 0x7fffffff;
       // the viscosity profile changes only through the ring-mean sound speed
       // (adiabatic alpha viscosity) or the time-dependent transition scale
       c.visc_dynamic = ((p.viscosity_alpha && p.adiabatic) || p.lambdadoubling != 0.0f) ? 1 : 0;
@@ -582,6 +584,8 @@ struct Batch : BatchBase {
     LCHK(dalloc(&D.fpart, (size_t)n_rtiles * 2 * FGX_NFORCE_T));
     LCHK(dalloc(&D.rpart, (size_t)std::max(max_nrad, 1) * 44));
     LCHK(dalloc(&D.summary, 2 + FGX_SUMMARY_HIST + nd));
+    LCHK(dalloc(&D.health, nd));
+    health_h.assign(nd, 0);
     CellTile* ctd; RingTile* rtd;
     LCHK(dalloc(&ctd, ct.size())); LCHK(dalloc(&rtd, rt.size()));
     D.ctiles = ctd; D.rtiles = rtd;
@@ -1028,6 +1032,21 @@ struct Batch : BatchBase {
     }
     HIPCHK(hipStreamSynchronize(stream));
     if (prof_on) prof_collect();
+    return check_health();
+  }
+  // disks found numerically unhealthy since the last look: the others have
+  // completed the call; these stopped and were frozen where they were found
+  int check_health() {
+    std::vector<int> h(nd, 0);
+    HIPCHK(hipMemcpy(h.data(), D.health, nd * sizeof(int), hipMemcpyDeviceToHost));
+    int first = -1, nbad = 0;
+    for (int d = 0; d < nd; ++d) {
+      if (h[d] && !health_h[d]) { if (first < 0) first = d; ++nbad; }
+      health_h[d] = h[d];
+    }
+    if (first >= 0)
+      return set_error(FGX_ENUMERIC, "%d disk(s) numerically unhealthy and frozen, first disk %d (flags 0x%x)", nbad,
+                       first, h[first]);
     return FGX_OK;
   }
 
@@ -1349,6 +1368,9 @@ struct Batch : BatchBase {
     std::memcpy(&st, q, sizeof st); q += sizeof st;
     st.vpar = D.vpar;   // the velocities go into this batch's current buffer pair
     HIPCHK(hipMemcpyAsync(D.st + d, &st, sizeof st, hipMemcpyHostToDevice, stream));
+    // the health flags travel with the state: a disk saved unhealthy stays frozen
+    HIPCHK(hipMemcpyAsync(D.health + d, &st.health, sizeof(int), hipMemcpyHostToDevice, stream));
+    health_h[d] = st.health;
     for (real* r : rings) {
       HIPCHK(hipMemcpyAsync(r + dch[d].ro, q, nrr * sizeof(real), hipMemcpyHostToDevice, stream));
       q += nrr * sizeof(real);
@@ -1372,6 +1394,14 @@ struct Batch : BatchBase {
     const int v = on ? 1 : 0;
     HIPCHK(hipMemcpyAsync((char*)D.st + (size_t)d * sizeof(DiskState) + offsetof(DiskState, frozen), &v, sizeof(int),
                           hipMemcpyHostToDevice, stream));
+    if (!on) {
+      // thawing also clears the disk's health flags (its fields may have been rewritten)
+      static const int zero = 0;
+      HIPCHK(hipMemcpyAsync((char*)D.st + (size_t)d * sizeof(DiskState) + offsetof(DiskState, health), &zero,
+                            sizeof(int), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(D.health + d, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+      health_h[d] = 0;
+    }
     HIPCHK(hipStreamSynchronize(stream));
     return FGX_OK;
   }

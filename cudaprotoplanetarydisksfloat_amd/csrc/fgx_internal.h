@@ -97,6 +97,7 @@ struct DiskConst {
   double hill_radius, sigma0, eccentricity;
   double omega0;           // OmegaFrame at set-up: the frame of Substep1's profile tables
   int substeps_forced;     // SUBSTEPS key, 0 = CFL
+  int max_substeps;        // MAXSUBSTEPS: a larger CFL count marks the disk unhealthy
   int visc_dynamic;        // recompute the viscosity profile each substep
   double sg_eps, sg_rstep, sg_tstep;
   // self-gravity transforms: offsets of this disk's work rows (nrad*nsec

@@ -56,6 +56,7 @@ struct Dev {
   double* fpart;       // per ring tile: 2*FGX_NFORCE_T force partial sums
   double* rpart;       // per ring tile: generic partial sums (44 max)
   int* summary;        // [0] active disks, [1] max ncfl
+  int* health;         // per disk: FGX_HEALTH_* flags raised by the CFL and force pass
   const CellTile* ctiles;
   const RingTile* rtiles;
   const CellTile* stiles;   // strips (d, first ring, sector block) of strip_rings rings
@@ -366,10 +367,19 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   const long long fb = c.fo;
   const int lane = (int)threadIdx.x;
   // disk forces on the targets (k = 0 or k = 10 Hill cut), per ring in FP64
+  // numeric health of the density at the start of the substep (mode 0): a
+  // value that is not positive, or not a number, flags the disk; the first
+  // force target's sweep reads every cell anyway, else a sweep of its own
+  const real s0 = RT(sigmamed)[rb + i];
+  const int nt = s.ntarget;
+  bool dbad = false;
+  if (mode == 0 && (c.accretes || nt == 0)) {
+#pragma unroll 4
+    for (int j = lane; j < ns; j += 64) dbad |= !(s0 + (real)AT(D.dens, i, j) > (real)0);
+  }
   if (mode == 1 || !c.accretes) {
-    const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i], s0 = RT(sigmamed)[rb + i];
+    const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
     double* out = D.fpart + (size_t)(c.rt0 + i) * (2 * FGX_NFORCE_T);
-    const int nt = s.ntarget;
     for (int q = 0; q < nt; ++q) {
       const real tx = (real)s.tx[q], ty = (real)s.ty[q], sm2 = (real)s.tsm2[q], irh2 = (real)(1.0 / s.trh2[q]);
       const int kk = s.tk[q];
@@ -377,6 +387,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
 #pragma unroll 4
       for (int j = lane; j < ns; j += 64) {
         const real cm = surf * (s0 + (real)AT(D.dens, i, j));
+        if (q == 0) dbad |= !(cm > (real)0);
         const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
         const real d2 = dx * dx + dy * dy;
         const real dist2 = d2 + sm2;
@@ -392,6 +403,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
       if (lane == 0) { out[2 * q] = fx; out[2 * q + 1] = fy; }
     }
   }
+  if (mode == 0 && __any(dbad) && lane == 0) atomicOr(D.health + t.d, FGX_HEALTH_DENSITY);
   if (mode == 1 || c.substeps_forced > 0 || (D.sloppy && s.sub_in_dt > 0)) return;
   const double vr0 = RT(vref)[rb + i];
   double acc = 0;
@@ -413,6 +425,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   // least limit is the limit of the ring's largest x, bit for bit; NaN x are
   // skipped by both forms, and the 1e30 start survives an empty maximum
   real xmax = 0;
+  bool nbad = false;
 #pragma unroll 4
   for (int j = lane; j < ns; j += 64) {
     const real rs = AT(D.vtheta, i, j);
@@ -432,8 +445,10 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
     i4 = i4b > i4 ? i4b : i4;
     i4 *= (real)(4.0 * kCVNR * kCVNR);
     const real x = i1 * i1 + i2 * i2 + i3 * i3 + i4 * i4;
+    nbad |= x != x;
     xmax = x > xmax ? x : xmax;
   }
+  if (__any(nbad) && lane == 0) atomicOr(D.health + t.d, FGX_HEALTH_NAN);
   xmax = -wave_min(-xmax);
   if (lane == 0) {
     const real dt2 = (real)kCFLSECURITY / sqrt(xmax);
@@ -529,16 +544,35 @@ __device__ void substep_begin_block(const Dev<real>& D, int d, DiskState& s, rea
     s.masstaper = sn * sn;
   }
   const double DT = c.DT;
-  int n;
-  if (c.substeps_forced > 0) {
-    n = D.sloppy ? c.substeps_forced : c.substeps_forced - s.sub_in_dt;
-    if (n < 1) n = 1;
-  } else if (need_cfl) {
-    const real dT = (real)(DT - s.dtemp);
-    real nd = newdt;
-    if (dT < nd) nd = dT;
-    n = (int)ceil(dT / nd);
-  } else n = s.ncfl;
+  // numeric health: the flags of this substep's CFL and force pass, a time
+  // step that is not positive (NaN included), a count above the cap.  An
+  // unhealthy disk is frozen here, before its count or dt is used, and keeps
+  // its fields as they are; its velocities stay in the current buffer pair.
+  int h = D.health[d];
+  if (need_cfl && !(newdt > (real)0)) h |= FGX_HEALTH_TIMESTEP;
+  int n = 1;
+  if (!h) {
+    if (c.substeps_forced > 0) {
+      n = D.sloppy ? c.substeps_forced : c.substeps_forced - s.sub_in_dt;
+      if (n < 1) n = 1;
+    } else if (need_cfl) {
+      const real dT = (real)(DT - s.dtemp);
+      real nd = newdt;
+      if (dT < nd) nd = dT;
+      const real q = ceil(dT / nd);
+      if (!(q <= (real)c.max_substeps)) h |= FGX_HEALTH_SUBSTEPS;
+      else n = (int)q;
+    } else n = s.ncfl;
+  }
+  if (h) {
+    s.health |= h;
+    D.health[d] = s.health;
+    s.active = 0;
+    s.frozen = 1;
+    s.ncfl = 0;
+    s.vpar = D.vpar;
+    return;
+  }
   // SloppyCFL keeps the dt of the first substep (SourceEuler.cu:224-229)
   if (!(D.sloppy && s.sub_in_dt > 0)) s.dt = D.sloppy ? DT / (double)n : (DT - s.dtemp) / (double)n;
   s.ncfl = n;
@@ -780,9 +814,11 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep_setup(Dev<real> D, int part
   if (run) {
     if (part != 2) {
       substep_begin_block(D, d, ls, redr);
-      if (threadIdx.x == 0) ls.pending = 1;
+      __syncthreads();
+      run = ls.active;   // not when the disk was found unhealthy
+      if (run && threadIdx.x == 0) ls.pending = 1;
     }
-    if (part != 1) {
+    if (run && part != 1) {
       __syncthreads();
       planets_block(D, d, ls, redd, tot, rk);
       // the force targets of the disk's next substep, at the time that
@@ -826,6 +862,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_accrete(Dev<real> D) {
     const double xp = s.x[k], yp = s.y[k], mp = s.mass[k];
     const double rp = sqrt(xp * xp + yp * yp);
     const double rroche = pow(1.0 / 3.0 * mp, 1.0 / 3.0) * rp;
+    // the Roche box is formed from these: a planet that is not finite (or
+    // whose box angle would overflow) takes nothing
+    if (!(rp > 0 && rp < 1e30 && rroche < rp)) continue;
     if (threadIdx.x == 0) {
       int imin = 0, imax = nr - 1;
       while (imin < nr && (double)RT(rsup)[rb + imin] < rp - rroche) imin++;
@@ -1191,7 +1230,9 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
     s.nr_iangle = ia_sh[0]; s.nr_iangle2 = ia_sh[1];
   }
   __syncthreads();
-  const int ia = ia_sh[0], ia2 = ia_sh[1];
+  // the shifts reduced into [0, nsec) (the same sectors; a non-finite sound
+  // speed cannot make them index outside the ring)
+  const int ia = ((ia_sh[0] % ns) + ns) % ns, ia2 = ((ia_sh[1] % ns) + ns) % ns;
   const real sm1 = RT(sigmamed)[rb + 1], sm2 = RT(sigmamed)[rb + nr - 2];
   // energy residuals move between rings with the profile difference
   const double de01 = D.adiabatic ? RT(energymed_d)[rb + 1] - RT(energymed_d)[rb] : 0.0;

@@ -57,6 +57,7 @@ void params_defaults(fgx_params* p) {
   set_str(p->zmplus, sizeof p->zmplus, "NO");
   set_str(p->adiabatic_s, sizeof p->adiabatic_s, "NO");
   p->adiabaticindex = 1.4f;
+  p->ext_max_substeps = 100000;
   set_str(p->cooling_s, sizeof p->cooling_s, "NO");
   p->coolingtime0 = 6.28f;
   // src/Interpret.cu:23-31 flag defaults
@@ -138,6 +139,7 @@ static void search_variable(fgx_params* p, const char* name, const char* sv) {
   if (std::strcmp(name, "GASNAMES") == 0) p->ext_gas_names = ti;
   if (std::strcmp(name, "CHECKPOINT") == 0) p->ext_checkpoint = ti != 0 || std::strcmp(sv, "YES") == 0;
   if (std::strcmp(name, "STUBBEDPHYSICS") == 0) p->ext_stubbed_physics = ti != 0 || std::strcmp(sv, "YES") == 0;
+  if (std::strcmp(name, "MAXSUBSTEPS") == 0) p->ext_max_substeps = ti;
   // the executable's switches -c, -b, -f (Main.cu:122-170) as keys
   if (std::strcmp(name, "SLOPPYCFL") == 0) p->sloppy_cfl = ti;
   if (std::strcmp(name, "CENTRIFUGALBALANCE") == 0) p->centrifugal_balance = ti;
@@ -263,6 +265,7 @@ int params_get_key(const fgx_params* p, const char* key, char* out, size_t n) {
       {"DISKID", (double)p->ext_disk_id}, {"SUBSTEPS", (double)p->ext_substeps},
       {"SGSOFTENING", p->ext_sg_softening}, {"GASNAMES", (double)p->ext_gas_names},
       {"CHECKPOINT", (double)p->ext_checkpoint}, {"STUBBEDPHYSICS", (double)p->ext_stubbed_physics},
+      {"MAXSUBSTEPS", (double)p->ext_max_substeps},
       // derived flags
       {"LOGGRID", (double)p->log_grid}, {"FASTTRANSPORT", (double)p->fast_transport},
       {"OPENINNER", (double)p->open_inner}, {"NONREFLECTING", (double)p->nonreflecting},

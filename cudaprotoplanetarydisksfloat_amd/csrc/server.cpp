@@ -101,6 +101,18 @@ void run_batch(fgx_server* S, std::vector<int> ids) {
       for (int k = 0; k < n; ++k) any |= left[k] > 0;
       if (!any) break;
       rc = fgx_batch_step(b, 1);
+      if (rc == FGX_ENUMERIC) {
+        // the disks found numerically unhealthy were frozen by the step; their
+        // requests end with that status, the others went on
+        for (int k = 0; k < n; ++k) {
+          fgx_disk_info di{};
+          if (left[k] > 0 && fgx_batch_info(b, k, &di) == FGX_OK && di.health) {
+            left[k] = 0;
+            finish(k, FGX_ENUMERIC);
+          }
+        }
+        rc = FGX_OK;
+      }
       if (rc) {
         for (int k = 0; k < n; ++k)
           if (left[k] > 0) { left[k] = 0; finish(k, rc); }

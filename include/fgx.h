@@ -89,6 +89,8 @@ typedef struct fgx_params {
   float ext_sg_softening;      /* SGSOFTENING: >0 overrides SG epsilon/H     */
   int   ext_gas_names;         /* GASNAMES: 0 dens/ layout, 1 gasdens*.dat, 2 both */
   int   ext_checkpoint;        /* CHECKPOINT: fgx-run writes restart<N>.bin at each output */
+  int   ext_max_substeps;      /* MAXSUBSTEPS: a disk whose CFL count in a DT exceeds it is   */
+                               /* stopped as numerically unhealthy (default 100000)       */
   int   ext_stubbed_physics;   /* STUBBEDPHYSICS: run what the reference parses but leaves  */
                                /* stubbed, with FARGO's semantics: planet accretion     */
                                /* (src/Planet.cu:7-47), label advection                 */
@@ -131,8 +133,18 @@ typedef struct fgx_disk_info {
   int    last_ncfl;            /* last gastimestepcfl                         */
   long long substeps;          /* substeps completed since creation           */
   long long dt_steps;          /* AlgoGas calls completed since creation      */
-  int    health;               /* 0 ok, else FGX_ENUMERIC flags               */
+  int    health;               /* 0 ok, else FGX_HEALTH_* flags: the disk was   */
+                               /* stopped and frozen at the substep that set them */
 } fgx_disk_info;
+
+/* numeric health of a disk (no counterpart in the reference, whose crash
+ * detection is disabled, src/SourceEuler.cu:62-74, 268-276).  A disk that
+ * sets any flag is frozen where its substep starts; the other disks of its
+ * batch go on, and fgx_batch_step / fgx_disk_step return FGX_ENUMERIC. */
+#define FGX_HEALTH_DENSITY   1  /* a density not positive or not a number      */
+#define FGX_HEALTH_NAN       2  /* a velocity or sound speed not a number      */
+#define FGX_HEALTH_TIMESTEP  4  /* the CFL time step not positive             */
+#define FGX_HEALTH_SUBSTEPS  8  /* the CFL count of a DT above MAXSUBSTEPS     */
 
 typedef struct fgx_batch fgx_batch;
 typedef struct fgx_batch fgx_disk;   /* a disk is a batch of one */
