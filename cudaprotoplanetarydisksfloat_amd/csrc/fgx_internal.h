@@ -88,7 +88,7 @@ struct DiskConst {
   int outer_source;              // StubbedPhysics + OUTERSOURCEMASS: FARGO's outer mass source
   double outer_vr;               // its radial velocity, IMPOSEDDISKDRIFT * Rinf[nrad-1]^-SIGMASLOPE
   // constants of the substep start: Rmed[0]^(2 FLARINGINDEX), Rmed[nrad-1]^(2
-  // FLARINGINDEX) and the EvanescentBoundary constants (SideEuler.cu:227-233)
+  // FLARINGINDEX) and the EvanescentBoundary constants (SideEuler.cu:125-130)
   double pow_in_2f, pow_out_2f, ev_tin, ev_tout, ev_drmin, ev_drmax;
   double DT, masstaper, sigmaslope, flaringindex, aspectratio, adiabaticindex;
   double thicknesssmoothing, rochesmoothing, releasedate, releaseradius;

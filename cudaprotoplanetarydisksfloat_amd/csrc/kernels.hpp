@@ -579,14 +579,14 @@ __device__ void substep_begin_block(const Dev<real>& D, int d, DiskState& s, rea
   s.dtemp += s.dt;
   if (D.corotating) get_psys_info_dev(D, c, s, 1);
   const double tt = s.time - s.time0;
-  // ApplySubKeplerianBoundary values (SideEuler.cu:250-268), FP64; the
+  // ApplySubKeplerianBoundary values (SideEuler.cu:148-166), FP64; the
   // powers of the edge radii are the disk's constants
   const double r0 = RT(rmed_d)[rb], rn = RT(rmed_d)[rb + nr - 1];
   const double fac = 1.0 + c.sigmaslope - 2.0 * c.flaringindex;
   const double h0 = aspect_ratio_dev(c, r0, tt), hn = aspect_ratio_dev(c, rn, tt);
   s.vkep_in = 1.0 / r0 * (1.0 - fac * h0 * h0 * c.pow_in_2f);
   s.vkep_out = 1.0 / rn * (1.0 - fac * hn * hn * c.pow_out_2f);
-  // EvanescentBoundary constants (SideEuler.cu:227-233), formed on the host
+  // EvanescentBoundary constants (SideEuler.cu:125-130), formed on the host
   s.ev_tin = c.ev_tin;
   s.ev_tout = c.ev_tout;
   s.ev_drmin = c.ev_drmin;
@@ -1059,7 +1059,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* _
 }
 
 // UpdateVelocitiesKernel (Kernels.cu:142-160) + ApplySubKeplerianBoundary
-// (SideEuler.cu:250-272, Kernels.cu:1490-1503)
+// (SideEuler.cu:148-170, Kernels.cu:1490-1503)
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
   FGX_STRIP_BEGIN {
@@ -1085,7 +1085,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_visc_update(Dev<real> D) {
     }
     if (!D.evanescent) {
       // with self-gravity the Keplerian values include -r * <SG_Accr>_ring
-      // (ApplySubKeplerianBoundary, SideEuler.cu:159-166)
+      // (ApplySubKeplerianBoundary, SideEuler.cu:158-166)
       if (i == 0) {
         const double v = s.vkep_in - (D.self_gravity ? RT(rmed_d)[rb] * D.axisg[rb] : 0.0);
         AT(D.vthetaint, i, j) = (real)(sqrt(v) - RT(rmed_d)[rb] * s.omega_frame - RT(vref)[rb]);
@@ -1200,7 +1200,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_open_boundary(Dev<real> D) {
   }
 }
 
-// NonReflectingBoundary (SideEuler.cu:172-222) with ReduceCs, ReduceMean,
+// NonReflectingBoundary (SideEuler.cu:70-97) with ReduceCs, ReduceMean (99-120),
 // NonReflectingBoundaryKernel and MinusMeanKernel (Kernels.cu:640-723): one
 // block per disk
 template <class real>

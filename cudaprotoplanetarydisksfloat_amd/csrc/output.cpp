@@ -32,7 +32,7 @@ int write_raw(const std::string& dir, const std::string& name, const std::vector
   return FGX_OK;
 }
 
-// WriteDiskPolar (Output.cu:355-386) and the FARGO gas<name><N>.dat form
+// WriteDiskPolar (Output.cu:104-135) and the FARGO gas<name><N>.dat form
 int write_field(fgx_disk* d, const fgx_params& P, int field, const char* name, const char* gasname, int index) {
   const fgx::HostDisk* h = fgx::batch_host(d, 0);
   std::vector<float> v((size_t)h->nrad * h->nsec);
@@ -61,7 +61,7 @@ int fgx_disk_write_outputs(fgx_disk* d, int index) {
   if (!d) return set_error(FGX_EPARAM, "null handle");
   const fgx_params& P = fgx::batch_host(d, 0)->P;
   int r = FGX_OK;
-  // SendOutput (Output.cu:406-421)
+  // SendOutput (Output.cu:155-170)
   if (!P.is_disk) return FGX_OK;
   if (P.write_density && (r = write_field(d, P, FGX_DENS, "dens", "gasdens", index))) return r;
   if (P.write_velocity) {
@@ -89,7 +89,7 @@ int fgx_disk_write_logs(fgx_disk* d, int timestep) {
   const std::string od = outdir(P);
   const float lost = (float)info.lost_mass, ptime = (float)info.physical_time, om = (float)info.omega_frame;
   const float mdcp1 = (float)info.mdcp1, exces = (float)info.exces_mdcp;
-  // WriteBigPlanetSystemFile / WriteBigPlanetFile (Output.cu:321-353)
+  // WriteBigPlanetSystemFile / WriteBigPlanetFile (Output.cu:70-102)
   for (int k = 0; k < sys.nb; ++k) {
     std::string name = od + "bigplanet" + std::to_string(k) + ".dat";
     FILE* f = open_append(name);
@@ -147,7 +147,7 @@ int fgx_disk_write_setup_files(fgx_disk* d) {
     for (int i = 0; i <= h->nrad; ++i) std::fprintf(f, "%.18g\n", h->radii[i]);
     std::fclose(f);
   }
-  // WriteDim (Output.cu:388-404)
+  // WriteDim (Output.cu:137-153)
   {
     std::string name = od + "dims.dat";
     FILE* f = std::fopen(name.c_str(), "w");
@@ -156,7 +156,7 @@ int fgx_disk_write_setup_files(fgx_disk* d) {
                  P.nsec);
     std::fclose(f);
   }
-  // EmptyPlanetSystemFile (Output.cu:263-283)
+  // EmptyPlanetSystemFile (Output.cu:12-32)
   for (int k = 0; k < h->sys.nb; ++k) {
     std::string name = od + "planet" + std::to_string(k) + ".dat";
     FILE* f = std::fopen(name.c_str(), "w");

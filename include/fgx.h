@@ -198,13 +198,13 @@ int fgx_disk_planets(fgx_disk* d, fgx_planets* out);
 int fgx_disk_set_planets(fgx_disk* d, const fgx_planets* pl);
 
 /* ---- output files (src/Output.cu, src/Force.cu:76-138) -------------------- */
-/* SendOutput (src/Output.cu:406-421) for output number `index` */
+/* SendOutput (src/Output.cu:155-170) for output number `index` */
 int fgx_disk_write_outputs(fgx_disk* d, int index);
-/* WriteBigPlanetSystemFile (src/Output.cu:341-353) + UpdateLog
+/* WriteBigPlanetSystemFile (src/Output.cu:90-102) + UpdateLog
  * (src/Force.cu:76-138) for time step `timestep` */
 int fgx_disk_write_logs(fgx_disk* d, int timestep);
-/* WriteDim (src/Output.cu:388-404), used_rad.dat (src/SourceEuler.cu:159-171),
- * EmptyPlanetSystemFile (src/Output.cu:263-283) */
+/* WriteDim (src/Output.cu:137-153), used_rad.dat (src/SourceEuler.cu:159-171),
+ * EmptyPlanetSystemFile (src/Output.cu:12-32) */
 int fgx_disk_write_setup_files(fgx_disk* d);
 /* disk->point force, all eleven Hill-cut variants (ComputeForce,
  * src/Force.cu:31-67): out[44] laid out as the reference's GlobalForce

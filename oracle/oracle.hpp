@@ -643,7 +643,7 @@ struct Disk {
       gf[q] = s;
     }
   }
-  // ComputeAccel (SideEuler.cu:151-164)
+  // ComputeAccel (SideEuler.cu:49-62)
   void compute_accel(double x, double y, double rsm, double mass, double& ax, double& ay) {
     compute_force(x, y, rsm, mass, GlobalForce);
     if (P.exclude_hill) { ax = GlobalForce[10] + GlobalForce[21]; ay = GlobalForce[32] + GlobalForce[43]; }
@@ -796,7 +796,7 @@ struct Disk {
   }
 
   // ------------------------------------------------------------ boundaries
-  // ApplyBoundaryCondition (SideEuler.cu:274-284)
+  // ApplyBoundaryCondition (SideEuler.cu:172-182)
   void apply_boundary(double dt) {
     if (P.open_inner) open_boundary();
     if (P.nonreflecting) {
@@ -828,7 +828,7 @@ struct Disk {
       else Vrad[id(1, j)] = Vrad[id(2, j)];
     }
   }
-  // NonReflectingBoundary (SideEuler.cu:172-199) + kernels (Kernels.cu:640-723)
+  // NonReflectingBoundary (SideEuler.cu:70-120) + kernels (Kernels.cu:640-723)
   void nonreflecting_boundary() {
     double cs0 = ring_mean(SoundSpeed, 0), cs1 = ring_mean(SoundSpeed, 1);
     double csn1 = ring_mean(SoundSpeed, NRAD - 1), csn2 = ring_mean(SoundSpeed, NRAD - 2);
@@ -868,7 +868,7 @@ struct Disk {
       Energy[id(NRAD - 1, j)] += EnergyMed[NRAD - 1] - me2;
     }
   }
-  // EvanescentBoundary (SideEuler.cu:224-243) + kernel (Kernels.cu:1076-1124)
+  // EvanescentBoundary (SideEuler.cu:122-141) + kernel (Kernels.cu:1076-1124)
   void evanescent_boundary(double step) {
     double Tin = 2.0 * PI * std::pow((double)Rmed[0], 3. / 2);
     double Tout = 2.0 * PI * std::pow((double)Rmed[NRAD - 1], 3. / 2);
@@ -897,7 +897,7 @@ struct Disk {
       }
     }
   }
-  // ApplySubKeplerianBoundary (SideEuler.cu:250-272)
+  // ApplySubKeplerianBoundary (SideEuler.cu:148-170)
   void sub_keplerian_boundary() {
     double r0 = Rmed[0], rn = Rmed[NRAD - 1];
     double fac = (1.0 + P.sigmaslope - 2.0 * P.flaringindex);
