@@ -166,6 +166,71 @@ int fgx_disk_write_setup_files(fgx_disk* d) {
   return FGX_OK;
 }
 
+// -v: TellEverything (Interpret.cu:235-306).  The reference keeps each figure
+// in a float before printing it, and counts orbits at unit radius with G = 1.
+static void tell_everything(const fgx_params& P) {
+  const double pi = 3.14159265358979323846;
+  const double rmin = P.rmin, rmax = P.rmax, h = P.aspectratio, s0 = P.sigma0, sl = P.sigmaslope;
+  auto orbits = [&](float t) { return (float)(t / 2.0 / pi); };
+  auto outputs = [&](float t) { return (float)(t / P.dt / P.ninterm); };
+  float t;
+  std::printf("\nDisc properties:\n----------------\n");
+  std::printf("Inner Radius          : %g\n", rmin);
+  std::printf("Outer Radius          : %g\n", rmax);
+  std::printf("Aspect Ratio          : %g\n", h);
+  std::printf("VKep at inner edge    : %.3g\n", std::sqrt(1.0 / rmin));
+  std::printf("VKep at outer edge    : %.3g\n", std::sqrt(1.0 / rmax));
+  // disk masses of Sigma0 r^-slope between the edges and unit radius
+  t = (float)(2.0 * pi * s0 / (2.0 - sl) * (std::pow(rmax, 2.0 - sl) - std::pow(rmin, 2.0 - sl)));
+  std::printf("Initial Disk Mass             : %g\n", (double)t);
+  t = (float)(2.0 * pi * s0 / (2.0 - sl) * (1.0 - std::pow(rmin, 2.0 - sl)));
+  std::printf("Initial Mass inner to r=1.0  : %g \n", (double)t);
+  t = (float)(2.0 * pi * s0 / (2.0 - sl) * (std::pow(rmax, 2.0 - sl) - 1.0));
+  std::printf("Initial Mass outer to r=1.0  : %g \n", (double)t);
+  // sound crossing times, the integral of dr / (h r^-1/2)
+  std::printf("Travelling time for acoustic density waves :\n");
+  const double spans[3][2] = {{rmin, rmax}, {1.0, rmax}, {rmin, 1.0}};
+  const char* labels[3] = {" * From Rmin to Rmax  ", " * From r=1.0 to Rmax", " * From r=1.0 to Rmin"};
+  for (int k = 0; k < 3; ++k) {
+    t = (float)(2.0 / 3.0 / h * (std::pow(spans[k][1], 1.5) - std::pow(spans[k][0], 1.5)));
+    std::printf("%s: %.2g = %.2f orbits ~ %.1f outputs\n", labels[k], (double)t, (double)orbits(t), (double)outputs(t));
+  }
+  // r^3 is a float product there
+  t = (float)(2.0 * pi * std::sqrt((double)(P.rmin * P.rmin * P.rmin)));
+  std::printf("Orbital time at Rmin  : %.3g ~ %.2f outputs\n", (double)t, (double)outputs(t));
+  t = (float)(2.0 * pi * std::sqrt((double)(P.rmax * P.rmax * P.rmax)));
+  std::printf("Orbital time at Rmax  : %.3g ~ %.2f outputs\n", (double)t, (double)outputs(t));
+  std::printf("Sound speed :\n");
+  std::printf(" * At unit radius     : %.3g\n", h);
+  std::printf(" * At outer edge      : %.3g\n", h * std::sqrt(1.0 / rmax));
+  std::printf(" * At inner edge      : %.3g\n", h * std::sqrt(1.0 / rmin));
+  std::printf("\nGrid properties:\n----------------\n");
+  std::printf("Number of rings       : %d\n", P.nrad);
+  std::printf("Number of sectors     : %d\n", P.nsec);
+  std::printf("Total cells           : %d\n", P.nrad * P.nsec);
+  std::printf("\nOutputs properties:\n-------------------\n");
+  const float span = (float)(P.ninterm * P.dt);
+  std::printf("Time increment between outputs : %.3f = %.3f orbits\n", (double)span, (double)orbits(span));
+  std::printf("At each output #i, the following files are written:\n");
+  const int bytes = (int)(P.nrad * P.nsec * sizeof(float));
+  std::printf("gasdens[i].dat : %d bytes\n", bytes);
+  std::printf("gasvrad[i].dat : %d bytes\n", bytes);
+  std::printf("gasvtheta[i].dat : %d bytes\n", bytes);
+  if (P.adiabatic) std::printf("gasTemperature[i].dat : %d bytes\n", bytes);
+  if (P.advect_label) std::printf("gaslabel[i].dat : %d bytes\n", bytes);
+  std::printf("There will be in total %d outputs\n", P.ntot / P.ninterm);
+  const float total = (float)(P.ntot * P.dt);
+  std::printf("(which correspond to an elapsed time = %.3f or to %.2f orbits)\n", (double)total, (double)orbits(total));
+  const float nfiles = 3.0f + (P.adiabatic ? 1.0f : 0.0f) + (P.advect_label ? 1.0f : 0.0f);
+  t = nfiles * P.nrad * P.nsec * sizeof(float);
+  t *= (float)P.ntot / (float)P.ninterm;
+  t /= 1024.0 * 1024.0;
+  std::printf("So the code will produce ~%.2f Mbytes of data\n", (double)t);
+  std::printf("Check (eg by issuing a 'df' command) that you have enough disk space,\n");
+  std::printf("otherwise you will get a system full and the code will stop.\n");
+  std::fflush(stdout);
+}
+
 static int usage(const char* exe) {
   std::printf("Usage : %s [-abcdeimnptvz] [-(0-9)] [-s number] [-f scaling] parameters file\n", exe);
   return 1;
@@ -176,13 +241,14 @@ static int usage(const char* exe) {
 int fgx_run_main(int argc, char** argv) {
   if (argc == 1) return usage(argv[0]);
   std::string parfile;
-  int disable = 0, sloppy = 0, cb = 0, store = 0, restart = 0, override_out = 0, nb_restart = 0;
+  int disable = 0, sloppy = 0, cb = 0, store = 0, restart = 0, override_out = 0, nb_restart = 0, verbose = 0;
   float scaling = 1.0f;
   std::string newout;
   for (int i = 1; i < argc; ++i) {
     if (argv[i][0] == '-') {
       if (std::strspn(argv[i], "-secndovtpfamzib0123456789") != std::strlen(argv[i])) return usage(argv[0]);
       if (std::strchr(argv[i], 'n')) disable = 1;
+      if (std::strchr(argv[i], 'v')) verbose = 1;
       if (std::strchr(argv[i], 'c')) sloppy = 1;
       if (std::strchr(argv[i], 'b')) cb = 1;
       if (std::strchr(argv[i], 'i')) store = 1;
@@ -217,6 +283,7 @@ int fgx_run_main(int argc, char** argv) {
   if (r) { std::fprintf(stderr, "%s\n", fgx_last_error()); return 1; }
   if (override_out) std::snprintf(P.outputdir, sizeof P.outputdir, "%s", newout.c_str());
   P.sloppy_cfl = sloppy; P.centrifugal_balance = cb; P.scaling_factor = scaling;
+  if (verbose) tell_everything(P);
   if (disable) return 0;
   fgx_planets pl;
   if ((r = fgx_planets_load(P.planetconfig, &P, &pl))) { std::fprintf(stderr, "%s\n", fgx_last_error()); return 1; }
