@@ -89,6 +89,7 @@ struct BatchBase {
   virtual int freeze(int disk, int on) = 0;
   virtual int checkpoint(int disk, const char* path) = 0;
   virtual int restore(int disk, const char* path) = 0;
+  virtual int reset_disk(int disk, const fgx_params& p, const fgx_planets& pl) = 0;
 };
 
 // checkpoint file: header, per-disk state record, ring profiles carried from
@@ -285,6 +286,34 @@ struct Batch : BatchBase {
   int size() const override { return nd; }
   const HostDisk& host(int d) const override { return hd[d]; }
 
+  // the arrays that hold a share per disk, by how that share is laid out:
+  // 0 field ((nrad+1)*nsec at fo), 1 ring (nrad+1 at ro), 2 sector (nsec at
+  // so), 3 self-gravity work (nrad*nsec complex at sgw_o), 4 force partials
+  // (2*FGX_NFORCE_T per ring tile from rt0)
+  struct Region { void* p; size_t elem; int kind; };
+  std::vector<Region> regions;
+  template <class T>
+  int dalloc_share(T** p, size_t n, int kind) {
+    if (int r = dalloc(p, n)) return r;
+    regions.push_back(Region{(void*)*p, sizeof(T), kind});
+    return FGX_OK;
+  }
+  // zero disk d's share of every per-disk array, as a fresh batch holds it
+  int clear_share(int d) {
+    const DiskConst& c = dch[d];
+    for (const Region& g : regions) {
+      size_t off = 0, n = 0;
+      switch (g.kind) {
+        case 0: off = (size_t)c.fo; n = (size_t)(c.nrad + 1) * c.nsec; break;
+        case 1: off = (size_t)c.ro; n = (size_t)c.nrad + 1; break;
+        case 2: off = (size_t)c.so; n = (size_t)c.nsec; break;
+        case 3: off = (size_t)c.sgw_o; n = (size_t)c.nrad * c.nsec; break;
+        default: off = (size_t)c.rt0 * 2 * FGX_NFORCE_T; n = (size_t)c.nrad * 2 * FGX_NFORCE_T; break;
+      }
+      HIPCHK(hipMemsetAsync((char*)g.p + off * g.elem, 0, n * g.elem, stream));
+    }
+    return FGX_OK;
+  }
   template <class T>
   int dalloc(T** p, size_t n) {
     void* q = nullptr;
@@ -364,6 +393,45 @@ struct Batch : BatchBase {
   }
 #define LCHK(x) do { int r_ = (x); if (r_) return r_; } while (0)
 
+  // the constants of a disk that follow from its parameters and set-up (the
+  // layout offsets are the batch's)
+  static void fill_const(const HostDisk& h, DiskConst& c) {
+    const fgx_params& p = h.P;
+    c.accretes = 0;
+    c.nb = h.sys.nb;
+    for (int k = 0; k < c.nb; ++k) {
+      c.feel_disk[k] = h.sys.feel_disk[k]; c.feel_others[k] = h.sys.feel_others[k]; c.acc[k] = h.sys.acc[k];
+      if (p.ext_stubbed_physics && h.sys.acc[k] > 1e-10) c.accretes = 1;
+    }
+    c.outer_source = (p.ext_stubbed_physics && p.outer_source_mass) ? 1 : 0;
+    c.outer_vr = p.imposeddiskdrift * std::pow(h.rinf[h.nrad - 1], -(double)p.sigmaslope);
+    {
+      const double r0 = h.rmed[0], rn = h.rmed[h.nrad - 1];
+      c.pow_in_2f = std::pow(r0, 2.0 * (double)p.flaringindex);
+      c.pow_out_2f = std::pow(rn, 2.0 * (double)p.flaringindex);
+      c.ev_tin = 2.0 * kPI * std::pow(r0, 1.5);
+      c.ev_tout = 2.0 * kPI * std::pow(rn, 1.5);
+      c.ev_drmin = r0 * 1.25;
+      c.ev_drmax = rn * 0.84;
+    }
+    c.DT = p.dt; c.masstaper = p.masstaper; c.sigmaslope = p.sigmaslope; c.flaringindex = p.flaringindex;
+    c.aspectratio = p.aspectratio; c.adiabaticindex = p.adiabaticindex;
+    c.thicknesssmoothing = p.thicknesssmoothing; c.rochesmoothing = p.rochesmoothing;
+    c.releasedate = p.releasedate; c.releaseradius = p.releaseradius;
+    c.transitionradius = p.transitionradius; c.transitionratio = p.transitionratio;
+    c.transitionwidth = p.transitionwidth; c.lambdadoubling = p.lambdadoubling;
+    c.cavityradius = p.cavityradius; c.cavityratio = p.cavityratio; c.cavitywidth = p.cavitywidth;
+    c.viscosity = p.viscosity; c.alphaviscosity = p.alphaviscosity;
+    c.hill_radius = h.hill_radius; c.sigma0 = p.sigma0; c.eccentricity = p.eccentricity;
+    c.omega0 = h.omega_frame;
+    c.substeps_forced = p.ext_substeps;
+    c.max_substeps = p.ext_max_substeps > 0 ? p.ext_max_substeps : 0x7fffffff;
+    // the viscosity profile changes only through the ring-mean sound speed
+    // (adiabatic alpha viscosity) or the time-dependent transition scale
+    c.visc_dynamic = ((p.viscosity_alpha && p.adiabatic) || p.lambdadoubling != 0.0f) ? 1 : 0;
+    if (p.self_gravity) { c.sg_eps = h.sg_eps; c.sg_rstep = h.sg_rstep; c.sg_tstep = h.sg_tstep; }
+  }
+
   // ------------------------------------------------------------------ create
   int create(const std::vector<fgx_params>& ps, const std::vector<fgx_planets>& pls) {
     nd = (int)ps.size();
@@ -372,15 +440,7 @@ struct Batch : BatchBase {
     for (int d = 0; d < nd; ++d) {
       const fgx_params& p = ps[d];
       LCHK(params_validate(&p));
-      if (p.adiabatic != P0.adiabatic || p.open_inner != P0.open_inner || p.nonreflecting != P0.nonreflecting ||
-          p.evanescent != P0.evanescent || p.fast_transport != P0.fast_transport ||
-          p.self_gravity != P0.self_gravity || p.cooling != P0.cooling || p.indirect_term != P0.indirect_term ||
-          p.exclude_hill != P0.exclude_hill || p.sloppy_cfl != P0.sloppy_cfl || p.corotating != P0.corotating ||
-          p.guiding_center != P0.guiding_center || p.forced_circular != P0.forced_circular ||
-          p.roche_smoothing != P0.roche_smoothing || p.viscosity_alpha != P0.viscosity_alpha ||
-          p.ext_fix_taurp_corner != P0.ext_fix_taurp_corner || p.ext_fix_pp_distance != P0.ext_fix_pp_distance ||
-          p.is_disk != P0.is_disk ||
-          (p.advect_label && p.ext_stubbed_physics) != (P0.advect_label && P0.ext_stubbed_physics))
+      if (!batch_modes_equal(p, P0))
         return set_error(FGX_EPARAM, "disk %d of the batch differs from disk 0 in a mode flag", d);
       if (p.corotating && pls[d].nb == 0) return set_error(FGX_EPARAM, "a corotating frame needs a planet");
       if (!p.is_disk) return set_error(FGX_EPARAM, "DISK NO (planets only) is not supported");
@@ -421,43 +481,12 @@ struct Batch : BatchBase {
         rt.push_back(RingTile{d, i});
         for (int jb = 0; jb * FGX_TPB < h.nsec; ++jb) ct.push_back(CellTile{d, i, jb, 0});
       }
-      const fgx_params& p = h.P;
-      c.nb = h.sys.nb;
-      for (int k = 0; k < c.nb; ++k) {
-        c.feel_disk[k] = h.sys.feel_disk[k]; c.feel_others[k] = h.sys.feel_others[k]; c.acc[k] = h.sys.acc[k];
-        if (p.ext_stubbed_physics && h.sys.acc[k] > 1e-10) c.accretes = 1;
-      }
+      fill_const(h, c);
       accretion |= c.accretes != 0;
-      c.outer_source = (p.ext_stubbed_physics && p.outer_source_mass) ? 1 : 0;
-      c.outer_vr = p.imposeddiskdrift * std::pow(h.rinf[h.nrad - 1], -(double)p.sigmaslope);
       outer_source |= c.outer_source != 0;
-      {
-        const double r0 = h.rmed[0], rn = h.rmed[h.nrad - 1];
-        c.pow_in_2f = std::pow(r0, 2.0 * (double)p.flaringindex);
-        c.pow_out_2f = std::pow(rn, 2.0 * (double)p.flaringindex);
-        c.ev_tin = 2.0 * kPI * std::pow(r0, 1.5);
-        c.ev_tout = 2.0 * kPI * std::pow(rn, 1.5);
-        c.ev_drmin = r0 * 1.25;
-        c.ev_drmax = rn * 0.84;
-      }
-      c.DT = p.dt; c.masstaper = p.masstaper; c.sigmaslope = p.sigmaslope; c.flaringindex = p.flaringindex;
-      c.aspectratio = p.aspectratio; c.adiabaticindex = p.adiabaticindex;
-      c.thicknesssmoothing = p.thicknesssmoothing; c.rochesmoothing = p.rochesmoothing;
-      c.releasedate = p.releasedate; c.releaseradius = p.releaseradius;
-      c.transitionradius = p.transitionradius; c.transitionratio = p.transitionratio;
-      c.transitionwidth = p.transitionwidth; c.lambdadoubling = p.lambdadoubling;
-      c.cavityradius = p.cavityradius; c.cavityratio = p.cavityratio; c.cavitywidth = p.cavitywidth;
-      c.viscosity = p.viscosity; c.alphaviscosity = p.alphaviscosity;
-      c.hill_radius = h.hill_radius; c.sigma0 = p.sigma0; c.eccentricity = p.eccentricity;
-      c.omega0 = h.omega_frame;
-      c.substeps_forced = p.ext_substeps;
-      c.max_substeps = p.ext_max_substeps > 0 ? p.ext_max_substeps : 0x7fffffff;
-      // the viscosity profile changes only through the ring-mean sound speed
-      // (adiabatic alpha viscosity) or the time-dependent transition scale
-      c.visc_dynamic = ((p.viscosity_alpha && p.adiabatic) || p.lambdadoubling != 0.0f) ? 1 : 0;
       visc_dynamic |= c.visc_dynamic;
+      const fgx_params& p = h.P;
       if (p.self_gravity) {
-        c.sg_eps = h.sg_eps; c.sg_rstep = h.sg_rstep; c.sg_tstep = h.sg_tstep;
         c.sgw_o = (long long)SGW; c.sgk_o = (long long)SGK;
         c.tw_ns = (int)STW; c.tw_nr = (int)(STW + h.nsec);
         sg_factor(h.nsec, c.fac_ns, &c.nf_ns);
@@ -524,7 +553,7 @@ struct Batch : BatchBase {
     n_rtiles = (int)rt.size();
     // device arrays
     const int ad = P0.adiabatic;
-    auto f = [&](real** p, bool need) -> int { return dalloc(p, need ? F : 1); };
+    auto f = [&](real** p, bool need) -> int { return need ? dalloc_share(p, F, 0) : dalloc(p, 1); };
     LCHK(f(&D.dens, true)); LCHK(f(&D.vrad, true)); LCHK(f(&D.vtheta, true)); LCHK(f(&D.energy, ad));
     LCHK(f(&D.cs, ad)); LCHK(f(&D.vradint, true)); LCHK(f(&D.vthetaint, true));
     LCHK(f(&D.taurr, true)); LCHK(f(&D.taupp, true)); LCHK(f(&D.taurp, true));
@@ -535,14 +564,14 @@ struct Batch : BatchBase {
     LCHK(f(&D.label, lab)); LCHK(f(&D.t_lab, lab));
     const int sg = P0.self_gravity;
     LCHK(f(&D.sg_accr, sg)); LCHK(f(&D.sg_acct, sg));
-    LCHK(dalloc(&D.phi, F));
+    LCHK(dalloc_share(&D.phi, F, 0));
     if (sg) {
       while (sgct.size() % FGX_TILE_PAD) sgct.push_back(RingTile{-1, 0});
       n_sgctiles = (int)sgct.size();
       cxd *w, *kr, *kt, *tw;
       RingTile* sgt;
-      LCHK(dalloc(&w, SGW)); LCHK(dalloc(&kr, SGK)); LCHK(dalloc(&kt, SGK)); LCHK(dalloc(&tw, STW));
-      LCHK(dalloc(&D.axisg, R)); LCHK(dalloc(&sgt, sgct.size()));
+      LCHK(dalloc_share(&w, SGW, 3)); LCHK(dalloc(&kr, SGK)); LCHK(dalloc(&kt, SGK)); LCHK(dalloc(&tw, STW));
+      LCHK(dalloc_share(&D.axisg, R, 1)); LCHK(dalloc(&sgt, sgct.size()));
       D.sgw = w; D.sgkr = kr; D.sgkt = kt; D.sgtw = tw; D.sg_ctiles = sgt;
       HIPCHK(hipMemcpyAsync((void*)sgt, sgct.data(), sgct.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
       std::vector<cxd> twh(STW);
@@ -562,26 +591,26 @@ struct Batch : BatchBase {
     real** rings[] = {&D.rinf, &D.rsup, &D.rmed, &D.surf, &D.invsurf, &D.invrmed, &D.invrinf, &D.invdiffrmed,
                       &D.invdiffrsup, &D.dxtheta, &D.supp_torque, &D.sigmamed, &D.energymed, &D.coolingtime,
                       &D.qplusmed, &D.visc, &D.cs_ring, &D.csmean, &D.vmed, &D.newdt};
-    for (real** r : rings) LCHK(dalloc(r, R));
-    LCHK(dalloc(&D.nshift, R));
+    for (real** r : rings) LCHK(dalloc_share(r, R, 1));
+    LCHK(dalloc_share(&D.nshift, R, 1));
     {
       double *vref, *acc0, *u0, *gq0, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d;
-      LCHK(dalloc(&vref, R)); LCHK(dalloc(&acc0, R)); LCHK(dalloc(&u0, R)); LCHK(dalloc(&gq0, R));
-      LCHK(dalloc(&rinf_d, R)); LCHK(dalloc(&invrinf_d, R));
-      LCHK(dalloc(&cs2_d, R)); D.cs2_d = cs2_d;
+      LCHK(dalloc_share(&vref, R, 1)); LCHK(dalloc_share(&acc0, R, 1)); LCHK(dalloc_share(&u0, R, 1));
+      LCHK(dalloc_share(&gq0, R, 1)); LCHK(dalloc_share(&rinf_d, R, 1)); LCHK(dalloc_share(&invrinf_d, R, 1));
+      LCHK(dalloc_share(&cs2_d, R, 1)); D.cs2_d = cs2_d;
       double *sigmamed_d, *energymed_d;
-      LCHK(dalloc(&sigmamed_d, R)); LCHK(dalloc(&energymed_d, R));
+      LCHK(dalloc_share(&sigmamed_d, R, 1)); LCHK(dalloc_share(&energymed_d, R, 1));
       D.sigmamed_d = sigmamed_d; D.energymed_d = energymed_d;
-      LCHK(dalloc(&rmed_d, R)); LCHK(dalloc(&invrmed_d, R)); LCHK(dalloc(&invdiffrmed_d, R)); LCHK(dalloc(&D.vmoy, R));
-      LCHK(dalloc(&cosj_d, S)); LCHK(dalloc(&sinj_d, S));
+      LCHK(dalloc_share(&rmed_d, R, 1)); LCHK(dalloc_share(&invrmed_d, R, 1)); LCHK(dalloc_share(&invdiffrmed_d, R, 1));
+      LCHK(dalloc_share(&D.vmoy, R, 1)); LCHK(dalloc_share(&cosj_d, S, 2)); LCHK(dalloc_share(&sinj_d, S, 2));
       D.vref = vref; D.acc0 = acc0; D.u0 = u0; D.gq0 = gq0; D.rinf_d = rinf_d; D.invrinf_d = invrinf_d; D.rmed_d = rmed_d;
       D.invdiffrmed_d = invdiffrmed_d; D.cosj_d = cosj_d; D.sinj_d = sinj_d; D.invrmed_d = invrmed_d;
     }
-    LCHK(dalloc(&D.cosj, S)); LCHK(dalloc(&D.sinj, S)); LCHK(dalloc(&D.lost, S));
+    LCHK(dalloc_share(&D.cosj, S, 2)); LCHK(dalloc_share(&D.sinj, S, 2)); LCHK(dalloc_share(&D.lost, S, 2));
     DiskConst* dcd; DiskState* std_;
     LCHK(dalloc(&dcd, nd)); LCHK(dalloc(&std_, nd));
     D.dc = dcd; D.st = std_;
-    LCHK(dalloc(&D.fpart, (size_t)n_rtiles * 2 * FGX_NFORCE_T));
+    LCHK(dalloc_share(&D.fpart, (size_t)n_rtiles * 2 * FGX_NFORCE_T, 4));
     LCHK(dalloc(&D.rpart, (size_t)std::max(max_nrad, 1) * 44));
     LCHK(dalloc(&D.summary, 2 + FGX_SUMMARY_HIST + nd));
     LCHK(dalloc(&D.health, nd));
@@ -646,6 +675,7 @@ struct Batch : BatchBase {
     D.write_pressure = (!ad && P0.write_temperature) ? 1 : 0;
     D.advect_label = lab;
     D.ignore_active = 0;
+    D.only_disk = -1;
     int viscous = 0;
     for (int d = 0; d < nd; ++d)
       for (int i = 0; i < hd[d].nrad; ++i) viscous |= (hd[d].visc[i] != 0.0);
@@ -665,42 +695,11 @@ struct Batch : BatchBase {
     LCHK(dalloc(&d_ranges, max_ranges));
     if (const char* e = std::getenv("FGX_COMPACT")) compact_on = std::atoi(e) != 0;
     std::vector<DiskState> sth(nd);
-    for (int d = 0; d < nd; ++d) {
-      DiskState& s = sth[d];
-      std::memset(&s, 0, sizeof s);
-      const HostDisk& h = hd[d];
-      for (int k = 0; k < h.sys.nb; ++k) {
-        s.x[k] = h.sys.x[k]; s.y[k] = h.sys.y[k]; s.vx[k] = h.sys.vx[k]; s.vy[k] = h.sys.vy[k];
-        s.mass[k] = h.sys.mass[k];
-      }
-      s.omega_frame = h.omega_frame;
-      s.active = 0;
-    }
+    for (int d = 0; d < nd; ++d) sth[d] = initial_state(d);
     HIPCHK(hipMemcpyAsync((void*)std_, sth.data(), nd * sizeof(DiskState), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
     LCHK(upload_rings());
-    for (int d = 0; d < nd; ++d) {
-      // the density is kept as a residual of S0, the field-precision SigmaMed
-      {
-        std::vector<double> sres(hd[d].dens);
-        for (int i = 0; i < hd[d].nrad; ++i)
-          for (int j = 0; j < hd[d].nsec; ++j) sres[(size_t)i * hd[d].nsec + j] -= field_base(d, FGX_DENS, i);
-        LCHK(upload_field(d, D.dens, sres));
-      }
-      LCHK(upload_field(d, D.vrad, hd[d].vrad));
-      std::vector<double> res(hd[d].vtheta);
-      for (int i = 0; i < hd[d].nrad; ++i)
-        for (int j = 0; j < hd[d].nsec; ++j) res[(size_t)i * hd[d].nsec + j] -= vref_h[d][i];
-      LCHK(upload_field(d, D.vtheta, res));
-      if (ad) {
-        // the energy is kept as a residual of its ring profile EnergyMed
-        std::vector<double> eres(hd[d].energy);
-        for (int i = 0; i < hd[d].nrad; ++i)
-          for (int j = 0; j < hd[d].nsec; ++j) eres[(size_t)i * hd[d].nsec + j] -= hd[d].energymed[i];
-        LCHK(upload_field(d, D.energy, eres));
-      }
-      if (lab) LCHK(upload_field(d, D.label, hd[d].label));
-    }
+    for (int d = 0; d < nd; ++d) LCHK(upload_fields_disk(d));
     // initial pressure (InitEuler's ComputePressureField), used by the
     // isothermal temperature output
     D.ignore_active = 1;
@@ -747,6 +746,44 @@ struct Batch : BatchBase {
     return FGX_OK;
   }
 
+  // the state record of a disk at set-up
+  DiskState initial_state(int d) const {
+    DiskState s;
+    std::memset(&s, 0, sizeof s);
+    const HostDisk& h = hd[d];
+    for (int k = 0; k < h.sys.nb; ++k) {
+      s.x[k] = h.sys.x[k]; s.y[k] = h.sys.y[k]; s.vx[k] = h.sys.vx[k]; s.vy[k] = h.sys.vy[k];
+      s.mass[k] = h.sys.mass[k];
+    }
+    s.omega_frame = h.omega_frame;
+    s.active = 0;
+    return s;
+  }
+  // the initial fields of a disk, in the engine's representation
+  int upload_fields_disk(int d) {
+    // the density is kept as a residual of S0, the field-precision SigmaMed
+    {
+      std::vector<double> sres(hd[d].dens);
+      for (int i = 0; i < hd[d].nrad; ++i)
+        for (int j = 0; j < hd[d].nsec; ++j) sres[(size_t)i * hd[d].nsec + j] -= field_base(d, FGX_DENS, i);
+      LCHK(upload_field(d, D.dens, sres));
+    }
+    LCHK(upload_field(d, D.vrad, hd[d].vrad));
+    std::vector<double> res(hd[d].vtheta);
+    for (int i = 0; i < hd[d].nrad; ++i)
+      for (int j = 0; j < hd[d].nsec; ++j) res[(size_t)i * hd[d].nsec + j] -= vref_h[d][i];
+    LCHK(upload_field(d, D.vtheta, res));
+    if (D.adiabatic) {
+      // the energy is kept as a residual of its ring profile EnergyMed
+      std::vector<double> eres(hd[d].energy);
+      for (int i = 0; i < hd[d].nrad; ++i)
+        for (int j = 0; j < hd[d].nsec; ++j) eres[(size_t)i * hd[d].nsec + j] -= hd[d].energymed[i];
+      LCHK(upload_field(d, D.energy, eres));
+    }
+    if (D.advect_label) LCHK(upload_field(d, D.label, hd[d].label));
+    return FGX_OK;
+  }
+
   // The ring profile a field is stored as a residual of (vtheta: vref; the
   // energy: EnergyMed; the density: SigmaMed in the field precision), 0 for
   // the other fields and for the ghost ring
@@ -776,7 +813,11 @@ struct Batch : BatchBase {
   }
   int upload_rings() {
     vref_h.assign(nd, {});
-    for (int d = 0; d < nd; ++d) {
+    for (int d = 0; d < nd; ++d) LCHK(upload_rings_disk(d));
+    return FGX_OK;
+  }
+  int upload_rings_disk(int d) {
+    {
       const HostDisk& h = hd[d];
       const int o = dch[d].ro;
       const size_t n = (size_t)h.nrad + 1;
@@ -1381,13 +1422,74 @@ struct Batch : BatchBase {
     }
     HIPCHK(hipStreamSynchronize(stream));
     // fields derived from the state (pressure, and the adiabatic sound speed)
+    LCHK(derive_fields(d, false));
+    return FGX_OK;
+  }
+  // the fields derived from disk d's state alone (its pressure, adiabatic
+  // sound speed and, with `sg`, self-gravity field); other disks untouched
+  int derive_fields(int d, bool sg) {
     D.ignore_active = 1;
+    D.only_disk = d;
     int r = launch("pressure_init", 2, k_pressure_init<real>, n_ctiles, FGX_TPB, 0, D);
     if (!r && D.adiabatic) r = launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D);
+    if (!r && sg && D.self_gravity) r = self_gravity();
     D.ignore_active = 0;
+    D.only_disk = -1;
     LCHK(r);
     HIPCHK(hipStreamSynchronize(stream));
     return FGX_OK;
+  }
+  // slot d takes a new disk of the same grid and batch modes, set up as
+  // fgx_batch_create would set it up; no allocation, the other disks keep
+  // their state
+  int reset_disk(int d, const fgx_params& p, const fgx_planets& pl) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    LCHK(params_validate(&p));
+    if (p.nrad != hd[d].nrad || p.nsec != hd[d].nsec)
+      return set_error(FGX_EPARAM, "slot %d holds a %d x %d grid, not %d x %d", d, hd[d].nrad, hd[d].nsec, p.nrad,
+                       p.nsec);
+    if (!batch_modes_equal(p, hd[0].P)) return set_error(FGX_EPARAM, "the disk differs from the batch in a mode flag");
+    if (p.corotating && pl.nb == 0) return set_error(FGX_EPARAM, "a corotating frame needs a planet");
+    HostDisk h;
+    LCHK(setup_grid_and_density(p, pl, h));
+    std::vector<double> khr, kht, axisg;
+    if (p.self_gravity) LCHK(sg_host_setup(h, khr, kht, axisg));
+    setup_velocities(h, axisg);
+    const DiskConst& o = dch[d];
+    DiskConst c;
+    std::memset(&c, 0, sizeof c);
+    c.nrad = o.nrad; c.nsec = o.nsec; c.fo = o.fo; c.ro = o.ro; c.so = o.so; c.rt0 = o.rt0;
+    c.sgw_o = o.sgw_o; c.sgk_o = o.sgk_o; c.tw_ns = o.tw_ns; c.tw_nr = o.tw_nr; c.nf_ns = o.nf_ns; c.nf_nr = o.nf_nr;
+    std::memcpy(c.fac_ns, o.fac_ns, sizeof c.fac_ns);
+    std::memcpy(c.fac_nr, o.fac_nr, sizeof c.fac_nr);
+    fill_const(h, c);
+    // the launch sequence the batch was set up with must cover the disk
+    bool viscous = false;
+    for (int i = 0; i < h.nrad; ++i) viscous |= h.visc[i] != 0.0;
+    if ((c.accretes && !accretion) || (c.outer_source && !outer_source) || (c.visc_dynamic && !visc_dynamic) ||
+        ((viscous || c.visc_dynamic) && !D.viscous))
+      return set_error(FGX_EPARAM, "the disk needs a pass (accretion, outer source or viscosity) the batch was set up "
+                                   "without");
+    hd[d] = std::move(h);
+    dch[d] = c;
+    HIPCHK(hipMemcpyAsync((void*)(D.dc + d), &dch[d], sizeof(DiskConst), hipMemcpyHostToDevice, stream));
+    if (p.self_gravity) {
+      HIPCHK(hipMemcpyAsync((void*)(D.sgkr + c.sgk_o), khr.data(), khr.size() * sizeof(double), hipMemcpyHostToDevice,
+                            stream));
+      HIPCHK(hipMemcpyAsync((void*)(D.sgkt + c.sgk_o), kht.data(), kht.size() * sizeof(double), hipMemcpyHostToDevice,
+                            stream));
+    }
+    DiskState st = initial_state(d);
+    st.vpar = D.vpar;   // the fields go into this batch's current buffer pair
+    static const int zero = 0;
+    HIPCHK(hipMemcpyAsync((void*)(D.st + d), &st, sizeof st, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(D.health + d, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+    LCHK(clear_share(d));
+    HIPCHK(hipStreamSynchronize(stream));
+    health_h[d] = 0;
+    LCHK(upload_rings_disk(d));
+    LCHK(upload_fields_disk(d));
+    return derive_fields(d, true);
   }
   int freeze(int d, int on) override {
     if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
@@ -1470,6 +1572,10 @@ int fgx_batch_checkpoint(fgx_batch* b, int disk, const char* path) {
 int fgx_batch_restore(fgx_batch* b, int disk, const char* path) {
   if (!b) return set_error(FGX_EPARAM, "null handle");
   return b->impl->restore(disk, path);
+}
+int fgx_batch_reset_disk(fgx_batch* b, int disk, const fgx_params* p, const fgx_planets* pl) {
+  if (!b || !p || !pl) return set_error(FGX_EPARAM, "null argument");
+  return b->impl->reset_disk(disk, *p, *pl);
 }
 int fgx_batch_size(fgx_batch* b, int* n) {
   if (!b || !n) return set_error(FGX_EPARAM, "null argument");

@@ -18,6 +18,7 @@ int params_parse_text(const char* text, fgx_params* p);
 int params_set_key(fgx_params* p, const char* key, const char* value);
 int params_get_key(const fgx_params* p, const char* key, char* out, size_t n);
 int params_validate(const fgx_params* p);
+bool batch_modes_equal(const fgx_params& p, const fgx_params& P0);
 int planets_parse_text(const char* text, const fgx_params* p, const double* rmed,
                        const double* radii, fgx_planets* out);
 bool read_file(const char* path, std::string& out);

@@ -66,6 +66,7 @@ struct Dev {
   int adiabatic, fast_transport, open_inner, nonreflecting, evanescent, self_gravity;
   int cooling, indirect, exclude_hill, fix_taurp, fix_pp, fix_sgwrap, sloppy, corotating;
   int guiding, forced_circular, roche, write_pressure, ignore_active, viscous;
+  int only_disk;      // with ignore_active: -1 every disk, else that disk alone
   int advect_label;   // LABELADVECTION with StubbedPhysics
   // which of the buffer pairs (vrad, vtheta) / (vradint, vthetaint) the
   // batch's pointers have been swapped to since set-up, modulo 2 (the fused
@@ -229,12 +230,19 @@ __device__ __forceinline__ bool disk_running(const DiskConst& c, const DiskState
   return s.active && !(s.pending && !(s.dtemp < 0.999999999 * c.DT));
 }
 
+// the disks a kernel runs on: in a step the active ones; with ignore_active
+// (set-up, derived reads) every disk, or the one only_disk names
+template <class real>
+__device__ __forceinline__ bool skip_disk(const Dev<real>& D, const DiskState& s, int d) {
+  return D.ignore_active ? (D.only_disk >= 0 && d != D.only_disk) : !s.active;
+}
+
 #define FGX_CELL_PROLOGUE                                                   \
   const CellTile t = D.ctiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];   \
   if (t.d < 0) return;                                                      \
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);                           \
   const FGX_AS4 DiskState& s = cref(D.st + t.d);                            \
-  if (!D.ignore_active && !s.active) return;                                \
+  if (skip_disk(D, s, t.d)) return;                                \
   const int ns = c.nsec, nr = c.nrad;                                       \
   const int i = t.i;                                                        \
   const int j = t.jb * FGX_TPB + (int)threadIdx.x;                          \
@@ -248,7 +256,7 @@ __device__ __forceinline__ bool disk_running(const DiskConst& c, const DiskState
   if (t.d < 0) return;                                                      \
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);                           \
   const FGX_AS4 DiskState& s = cref(D.st + t.d);                            \
-  if (!D.ignore_active && !s.active) return;                                \
+  if (skip_disk(D, s, t.d)) return;                                \
   const int ns = c.nsec, nr = c.nrad;                                       \
   const int i = t.i;                                                        \
   const long long fb = c.fo;                                                \
@@ -261,7 +269,7 @@ __device__ __forceinline__ bool disk_running(const DiskConst& c, const DiskState
   if (t.d < 0) return;                                                      \
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);                           \
   const FGX_AS4 DiskState& s = cref(D.st + t.d);                            \
-  if (!D.ignore_active && !s.active) return;                                \
+  if (skip_disk(D, s, t.d)) return;                                \
   const int ns = c.nsec, nr = c.nrad;                                       \
   const int j = t.jb * FGX_TPB + (int)threadIdx.x;                          \
   if (j >= ns) return;                                                      \
@@ -1323,7 +1331,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_outer_source(Dev<real> D) {
   const int d = blockIdx.x;
   const DiskConst& c = D.dc[d];
   const DiskState& s = D.st[d];
-  if (!c.outer_source || (!D.ignore_active && !s.active)) return;
+  if (!c.outer_source || skip_disk(D, s, d)) return;
   __shared__ double redd[FGX_TPB / 64];
   const int ns = c.nsec, i = c.nrad - 1;
   const long long fb = c.fo;
@@ -1389,7 +1397,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_theta(Dev<real> D, const Ring
   if (t.d < 0) return;
   const DiskConst& c = D.dc[t.d];
   DiskState& s = D.st[t.d];
-  if (!D.ignore_active && !s.active) return;
+  if (skip_disk(D, s, t.d)) return;
   const int ns = c.nsec, nr = c.nrad;
   const int i = t.i;
   const long long fb = c.fo;
@@ -1532,7 +1540,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_vr(Dev<real> D) {
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
   const FGX_AS4 DiskState& s = cref(D.st + t.d);
-  if (!D.ignore_active && !s.active) return;
+  if (skip_disk(D, s, t.d)) return;
   const int ns = c.nsec, nr = c.nrad;
   const int j = t.jb * FGX_TPB + (int)threadIdx.x;
   if (j >= ns) return;

@@ -196,6 +196,20 @@ static int derive_flags(fgx_params* p) {
   return FGX_OK;
 }
 
+// the flags that choose a batch's launch sequence, and the batch-wide
+// switches: every disk of a batch shares them
+bool batch_modes_equal(const fgx_params& p, const fgx_params& P0) {
+  return !(p.adiabatic != P0.adiabatic || p.open_inner != P0.open_inner || p.nonreflecting != P0.nonreflecting ||
+           p.evanescent != P0.evanescent || p.fast_transport != P0.fast_transport ||
+           p.self_gravity != P0.self_gravity || p.cooling != P0.cooling || p.indirect_term != P0.indirect_term ||
+           p.exclude_hill != P0.exclude_hill || p.sloppy_cfl != P0.sloppy_cfl || p.corotating != P0.corotating ||
+           p.guiding_center != P0.guiding_center || p.forced_circular != P0.forced_circular ||
+           p.roche_smoothing != P0.roche_smoothing || p.viscosity_alpha != P0.viscosity_alpha ||
+           p.ext_fix_taurp_corner != P0.ext_fix_taurp_corner || p.ext_fix_pp_distance != P0.ext_fix_pp_distance ||
+           p.is_disk != P0.is_disk || p.ext_fix_sg_wrap != P0.ext_fix_sg_wrap ||
+           (p.advect_label && p.ext_stubbed_physics) != (P0.advect_label && P0.ext_stubbed_physics));
+}
+
 int params_validate(const fgx_params* p) {
   if (p->nrad < 4 || p->nsec < 4)
     return set_error(FGX_EPARAM, "NRAD and NSEC must be at least 4 (got %d x %d)", p->nrad, p->nsec);

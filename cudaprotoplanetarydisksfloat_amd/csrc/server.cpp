@@ -41,17 +41,6 @@ struct fgx_server {
 
 namespace {
 
-// requests batch together only when every mode flag a batch shares agrees
-bool same_modes(const fgx_params& a, const fgx_params& b) {
-  return a.adiabatic == b.adiabatic && a.open_inner == b.open_inner && a.nonreflecting == b.nonreflecting &&
-         a.evanescent == b.evanescent && a.fast_transport == b.fast_transport && a.self_gravity == b.self_gravity &&
-         a.cooling == b.cooling && a.indirect_term == b.indirect_term && a.exclude_hill == b.exclude_hill &&
-         a.sloppy_cfl == b.sloppy_cfl && a.corotating == b.corotating && a.guiding_center == b.guiding_center &&
-         a.forced_circular == b.forced_circular && a.roche_smoothing == b.roche_smoothing &&
-         a.viscosity_alpha == b.viscosity_alpha && a.ext_fix_taurp_corner == b.ext_fix_taurp_corner &&
-         a.ext_fix_pp_distance == b.ext_fix_pp_distance && a.is_disk == b.is_disk;
-}
-
 void run_batch(fgx_server* S, std::vector<int> ids) {
   const int n = (int)ids.size();
   std::vector<fgx_params> ps(n);
@@ -200,7 +189,7 @@ int fgx_server_run(fgx_server* s) {
       size_t used = first.footprint;
       size_t k = next + 1;
       while (k < order.size() && (int)group.size() < s->cfg.max_batch_disks && s->reqs[order[k]].arrival <= t &&
-             same_modes(s->reqs[order[k]].p, first.p) && s->in_use + used + s->reqs[order[k]].footprint <= s->cfg.hbm_budget) {
+             fgx::batch_modes_equal(s->reqs[order[k]].p, first.p) && s->in_use + used + s->reqs[order[k]].footprint <= s->cfg.hbm_budget) {
         group.push_back(order[k]);
         used += s->reqs[order[k]].footprint;
         ++k;

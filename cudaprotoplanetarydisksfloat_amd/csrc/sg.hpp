@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_cols(Dev<real> D) {
   if (t.d < 0) return;
   const DiskConst& c = D.dc[t.d];
   const DiskState& s = D.st[t.d];
-  if (!D.ignore_active && !s.active) return;
+  if (skip_disk(D, s, t.d)) return;
   const int ns = c.nsec, nr = c.nrad, n = 2 * nr;
   const int ka = t.i, kb = (ns - ka) % ns;
   const int nl = (ka == kb) ? 1 : 2;

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
   const FGX_AS4 DiskState& s = cref(D.st + t.d);
-  if (!D.ignore_active && !s.active) return;
+  if (skip_disk(D, s, t.d)) return;
   const int ns = c.nsec, nr = c.nrad;
   const long long fb = c.fo;
   const int rb = c.ro;

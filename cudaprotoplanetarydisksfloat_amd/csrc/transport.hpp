@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   if (t.d < 0) return;
   const DiskConst& c = D.dc[t.d];
   const DiskState& s = D.st[t.d];
-  if (!D.ignore_active && !s.active) return;
+  if (skip_disk(D, s, t.d)) return;
   const int ns = c.nsec, nr = c.nrad;
   const int j = t.jb * FGX_TPB + (int)threadIdx.x;
   if (j >= ns) return;
@@ -339,7 +339,7 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
   const FGX_AS4 DiskState& s = cref(D.st + t.d);
-  if (!D.ignore_active && !s.active) return;
+  if (skip_disk(D, s, t.d)) return;
   const int ns = c.nsec, i = t.i, rb = c.ro;
   const int w = MULTI ? (int)threadIdx.x >> 6 : 0, lane = (int)threadIdx.x & 63;
   const int T = (WM == 0) ? ns / K : 64;

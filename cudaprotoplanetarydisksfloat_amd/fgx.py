@@ -122,6 +122,7 @@ _sigs = {
     "fgx_run_main": ([_i, ctypes.POINTER(_cp)], _i),
     "fgx_batch_freeze": ([_vp, _i, _i], _i),
     "fgx_batch_checkpoint": ([_vp, _i, _cp], _i), "fgx_batch_restore": ([_vp, _i, _cp], _i),
+    "fgx_batch_reset_disk": ([_vp, _i, _vp, ctypes.POINTER(Planets)], _i),
     "fgx_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)], _i),
     "fgx_server_submit": ([_vp, _vp, ctypes.POINTER(Planets), _i, _d, ctypes.POINTER(_i)], _i),
     "fgx_server_run": ([_vp], _i),
@@ -266,6 +267,11 @@ class Batch:
 
     def restore(self, path, disk=0):
         _check(lib.fgx_batch_restore(self.h, disk, os.fsencode(path)))
+
+    def reset_disk(self, disk, params, planets):
+        """Slot `disk` takes a new disk of the same grid and modes (fgx_batch_reset_disk)."""
+        _check(lib.fgx_batch_reset_disk(self.h, disk, ctypes.cast(params.buf, ctypes.c_void_p), ctypes.byref(planets)))
+        self.params[disk] = params.copy()
 
     # single-disk conveniences (disk 0)
     def ring(self, name):

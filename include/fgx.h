@@ -252,6 +252,16 @@ int fgx_batch_freeze(fgx_batch* b, int disk, int frozen);
  * output when CHECKPOINT is YES. */
 int fgx_batch_checkpoint(fgx_batch* b, int disk, const char* path);
 int fgx_batch_restore(fgx_batch* b, int disk, const char* path);
+/* Slot `disk` of a live batch takes a new disk: parameters and planets of
+ * the same NRAD x NSEC grid and batch mode flags, set up exactly as
+ * fgx_batch_create would set it up (it then evolves bit-identically to that
+ * disk alone).  Nothing is allocated; the other disks keep their state and
+ * the frozen flag of the slot is cleared.  FGX_EPARAM when the grid or a mode
+ * differs, or when the disk needs a pass (accretion, outer mass source,
+ * viscosity) the batch was created without.  No counterpart in the reference,
+ * which runs one disk per process (src/Main.cu:87-401); the serving loop
+ * refills the slots of finished requests with it. */
+int fgx_batch_reset_disk(fgx_batch* b, int disk, const fgx_params* p, const fgx_planets* pl);
 /* bytes of device memory a disk of these parameters needs */
 int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes);
 
