@@ -74,7 +74,7 @@ def serving(n_req, precision, dt_range=(4, 24), snapshot_every=4):
     reqs = configs.serve_requests(n=n_req, rate=4.0, dt_range=dt_range)
     budget = int(torch.cuda.get_device_properties(0).total_memory * 0.9)
     srv = Server(precision=precision, hbm_budget=budget, max_batch_disks=64, snapshot_every=snapshot_every,
-                 batch_window=0.0)
+                 batch_window=0.0, max_live_batches=4, spare_slots=2)
     for par, cfg, n_dt, t in reqs:
         p, pl = configs.load(par, cfg)
         srv.submit(p, pl, n_dt, t)
@@ -82,13 +82,15 @@ def serving(n_req, precision, dt_range=(4, 24), snapshot_every=4):
     srv.run()
     wall = time.perf_counter() - t0
     infos = [srv.request(k) for k in range(len(reqs))]
+    st = srv.stats()
     srv.close()
     lat = np.array([i.finish - i.arrival for i in infos])
     cu = sum(i.cell_updates for i in infos)
     return {"requests": len(reqs), "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4), "max_s": round(float(lat.max()), 4),
             "cell_updates_per_s": cu / wall, "makespan_s": round(wall, 3),
-            "last_arrival_s": round(reqs[-1][3], 3), "batches": len({i.batch for i in infos}),
+            "last_arrival_s": round(reqs[-1][3], 3), "batches": st.batches, "slot_refills": st.refills,
+            "peak_live_batches": st.peak_live, "peak_bytes": int(st.peak_bytes),
             "snapshots": int(sum(i.snapshots for i in infos)), "self_gravity_requests":
                 sum(1 for par, _, _, _ in reqs if "SelfGravity\tYES" in par),
             "workload": "Poisson 4/s, grids 128x384..1024x3072, 1-3 planets, 20% SG, "

@@ -90,6 +90,8 @@ struct BatchBase {
   virtual int checkpoint(int disk, const char* path) = 0;
   virtual int restore(int disk, const char* path) = 0;
   virtual int reset_disk(int disk, const fgx_params& p, const fgx_planets& pl) = 0;
+  virtual int snapshot_queue(int disk) = 0;
+  virtual int snapshot_wait(int disk, int* had) = 0;
 };
 
 // checkpoint file: header, per-disk state record, ring profiles carried from
@@ -199,12 +201,16 @@ struct Batch : BatchBase {
     return FGX_OK;
   }
   int list_coarse_st = -1, list_coarse_src = -1, list_fine_st = -1, list_fine_src = -1, list_rings = -1;
+  int list_sgcols = -1, n_sgctiles_full = 0;
+  const RingTile* sg_ctiles_full = nullptr;
+  std::vector<char> frozen_h;   // disks frozen by the host (fgx_batch_freeze)
   std::vector<int> list_multi, list_groups;
   // the whole lists again (every DT starts and ends with them)
   void lists_full() {
     coarse = coarse_full; fine = fine_full;
     theta_multi = theta_multi_full; theta_multi_n = theta_multi_n_full; theta_groups = theta_groups_full;
     D.rtiles = rtiles_full; n_rtiles = n_rtiles_full;
+    if (list_sgcols >= 0) { D.sg_ctiles = sg_ctiles_full; n_sgctiles = n_sgctiles_full; }
     covered = nd; covered_cells = total_cells;
   }
   // narrow every list to the disks with act[d] != 0
@@ -235,6 +241,10 @@ struct Batch : BatchBase {
     fine.st = part(list_fine_st).first; fine.n_st = part(list_fine_st).second;
     fine.src = part(list_fine_src).first; fine.n_src = part(list_fine_src).second;
     D.rtiles = (const RingTile*)tlists[list_rings].part; n_rtiles = tlists[list_rings].n_part;
+    if (list_sgcols >= 0) {
+      D.sg_ctiles = (const RingTile*)tlists[list_sgcols].part;
+      n_sgctiles = tlists[list_sgcols].n_part;
+    }
     ThetaMulti M{};
     int first = 0;
     for (int g = 0; g < theta_multi_full.ng; ++g) {
@@ -274,8 +284,16 @@ struct Batch : BatchBase {
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
 
+  // asynchronous snapshots: per disk a pinned buffer of its stored fields and
+  // the event that marks the copy's end
+  std::vector<void*> snap_host;
+  std::vector<hipEvent_t> snap_ev;
+  std::vector<char> snap_pending;
+
   ~Batch() override {
     if (stream) hipStreamSynchronize(stream);
+    for (void* p : snap_host) if (p) hipHostFree(p);
+    for (hipEvent_t e : snap_ev) if (e) hipEventDestroy(e);
     for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : free_events) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
@@ -449,15 +467,27 @@ struct Batch : BatchBase {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc((void**)&h_summary, (2 + FGX_SUMMARY_HIST + nd) * sizeof(int)));
     hd.resize(nd);
-    for (int d = 0; d < nd; ++d) LCHK(setup_grid_and_density(ps[d], pls[d], hd[d]));
+    // a disk whose parameters and planets repeat an earlier disk's byte for
+    // byte (the serving loop's spare slots) takes that disk's host set-up
+    std::vector<int> same(nd, -1);
+    for (int d = 1; d < nd; ++d)
+      for (int e = 0; e < d && same[d] < 0; ++e)
+        if (same[e] < 0 && !std::memcmp(&ps[d], &ps[e], sizeof(fgx_params)) &&
+            !std::memcmp(&pls[d], &pls[e], sizeof(fgx_planets)))
+          same[d] = e;
+    for (int d = 0; d < nd; ++d)
+      if (same[d] < 0) LCHK(setup_grid_and_density(ps[d], pls[d], hd[d]));
     // self-gravity set-up (FP64, host): kernel transforms, the initial radial
     // SG profile for the planets' and the gas's circular velocities
     std::vector<std::vector<double>> khr(nd), kht(nd);
     for (int d = 0; d < nd; ++d) {
+      if (same[d] >= 0) continue;
       std::vector<double> axisg;
       if (hd[d].P.self_gravity) LCHK(sg_host_setup(hd[d], khr[d], kht[d], axisg));
       setup_velocities(hd[d], axisg);
     }
+    for (int d = 0; d < nd; ++d)
+      if (same[d] >= 0) { hd[d] = hd[same[d]]; khr[d] = khr[same[d]]; kht[d] = kht[same[d]]; }
     // layout
     dch.resize(nd);
     size_t SGW = 0, SGK = 0, STW = 0;
@@ -574,6 +604,9 @@ struct Batch : BatchBase {
       LCHK(dalloc_share(&D.axisg, R, 1)); LCHK(dalloc(&sgt, sgct.size()));
       D.sgw = w; D.sgkr = kr; D.sgkt = kt; D.sgtw = tw; D.sg_ctiles = sgt;
       HIPCHK(hipMemcpyAsync((void*)sgt, sgct.data(), sgct.size() * sizeof(RingTile), hipMemcpyHostToDevice, stream));
+      LCHK(add_list(sgct, (const RingTile*)sgt, list_sgcols));
+      sg_ctiles_full = sgt;
+      n_sgctiles_full = n_sgctiles;
       std::vector<cxd> twh(STW);
       for (int d = 0; d < nd; ++d) {
         const DiskConst& c = dch[d];
@@ -615,6 +648,7 @@ struct Batch : BatchBase {
     LCHK(dalloc(&D.summary, 2 + FGX_SUMMARY_HIST + nd));
     LCHK(dalloc(&D.health, nd));
     health_h.assign(nd, 0);
+    frozen_h.assign(nd, 0);
     CellTile* ctd; RingTile* rtd;
     LCHK(dalloc(&ctd, ct.size())); LCHK(dalloc(&rtd, rt.size()));
     D.ctiles = ctd; D.rtiles = rtd;
@@ -1029,12 +1063,21 @@ struct Batch : BatchBase {
   }
   int step(int n_dt) override {
     if (covered < nd) lists_full();   // after a step that stopped on an error
+    // the disks that take part in the DT: frozen ones (by the host, or for
+    // their numeric health) are left out of every tile list from its start
+    std::vector<int> act0(nd, 1);
+    int n_act0 = 0;
+    for (int d = 0; d < nd; ++d) {
+      act0[d] = !frozen_h[d] && !health_h[d];
+      n_act0 += act0[d];
+    }
     for (int k = 0; k < n_dt; ++k) {
       LCHK(launch("begin_dt", 0, k_begin_dt<real>, (nd + 63) / 64, 64, 0, D));
+      if (compact_on && n_act0 < nd) LCHK(lists_narrow(act0.data()));
       if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
       int plan = 0;
       // the first substeps of the DT run every disk that is not frozen
-      fine_plan.assign(1, (double)total_cells < fine_cells);
+      fine_plan.assign(1, (double)covered_cells < fine_cells);
       plan_pos = 0;
       for (int guard = 0;; ++guard) {
         if (guard > 10000000) return set_error(FGX_ENUMERIC, "substep loop did not terminate");
@@ -1412,6 +1455,7 @@ struct Batch : BatchBase {
     // the health flags travel with the state: a disk saved unhealthy stays frozen
     HIPCHK(hipMemcpyAsync(D.health + d, &st.health, sizeof(int), hipMemcpyHostToDevice, stream));
     health_h[d] = st.health;
+    frozen_h[d] = (char)(st.frozen != 0);
     for (real* r : rings) {
       HIPCHK(hipMemcpyAsync(r + dch[d].ro, q, nrr * sizeof(real), hipMemcpyHostToDevice, stream));
       q += nrr * sizeof(real);
@@ -1437,6 +1481,42 @@ struct Batch : BatchBase {
     D.only_disk = -1;
     LCHK(r);
     HIPCHK(hipStreamSynchronize(stream));
+    return FGX_OK;
+  }
+  // the fields a snapshot holds: Σ, vr, vθ and, adiabatic, e, as stored
+  // (residuals of the ring profiles that fgx_batch_read_ring returns)
+  std::vector<real*> snap_fields() {
+    std::vector<real*> v{D.dens, D.vrad, D.vtheta};
+    if (D.adiabatic) v.push_back(D.energy);
+    return v;
+  }
+  // queue a copy of disk d's fields into its pinned buffer behind the work
+  // already on the stream; the host does not wait
+  int snapshot_queue(int d) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    if (snap_host.empty()) { snap_host.assign(nd, nullptr); snap_ev.assign(nd, nullptr); snap_pending.assign(nd, 0); }
+    if (snap_pending[d]) HIPCHK(hipEventSynchronize(snap_ev[d]));   // the last copy, long done
+    const std::vector<real*> fs = snap_fields();
+    const size_t nf = (size_t)(hd[d].nrad + 1) * hd[d].nsec;
+    if (!snap_host[d]) {
+      HIPCHK(hipHostMalloc(&snap_host[d], fs.size() * nf * sizeof(real)));
+      HIPCHK(hipEventCreateWithFlags(&snap_ev[d], hipEventDisableTiming));
+    }
+    for (size_t q = 0; q < fs.size(); ++q)
+      HIPCHK(hipMemcpyAsync((real*)snap_host[d] + q * nf, fs[q] + dch[d].fo, nf * sizeof(real), hipMemcpyDeviceToHost,
+                            stream));
+    HIPCHK(hipEventRecord(snap_ev[d], stream));
+    snap_pending[d] = 1;
+    return FGX_OK;
+  }
+  // wait for disk d's queued copy (had = whether one was queued)
+  int snapshot_wait(int d, int* had) override {
+    if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
+    *had = 0;
+    if (snap_pending.empty() || !snap_pending[d]) return FGX_OK;
+    HIPCHK(hipEventSynchronize(snap_ev[d]));
+    snap_pending[d] = 0;
+    *had = 1;
     return FGX_OK;
   }
   // slot d takes a new disk of the same grid and batch modes, set up as
@@ -1487,6 +1567,8 @@ struct Batch : BatchBase {
     LCHK(clear_share(d));
     HIPCHK(hipStreamSynchronize(stream));
     health_h[d] = 0;
+    frozen_h[d] = 0;
+    if (!snap_pending.empty()) snap_pending[d] = 0;   // a copy of the slot's last disk has landed
     LCHK(upload_rings_disk(d));
     LCHK(upload_fields_disk(d));
     return derive_fields(d, true);
@@ -1494,6 +1576,7 @@ struct Batch : BatchBase {
   int freeze(int d, int on) override {
     if (d < 0 || d >= nd) return set_error(FGX_ERANGE, "disk index %d out of range", d);
     const int v = on ? 1 : 0;
+    frozen_h[d] = (char)v;
     HIPCHK(hipMemcpyAsync((char*)D.st + (size_t)d * sizeof(DiskState) + offsetof(DiskState, frozen), &v, sizeof(int),
                           hipMemcpyHostToDevice, stream));
     if (!on) {
@@ -1669,4 +1752,6 @@ int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes) {
 namespace fgx {
 const HostDisk* batch_host(fgx_batch* b, int d) { return &b->impl->host(d); }
 int batch_mdcp(fgx_batch* b, int d, double* out) { return b->impl->mdcp(d, out); }
+int batch_snapshot_queue(fgx_batch* b, int d) { return b->impl->snapshot_queue(d); }
+int batch_snapshot_wait(fgx_batch* b, int d, int* had) { return b->impl->snapshot_wait(d, had); }
 }  // namespace fgx

@@ -69,6 +69,10 @@ int sg_host_setup(HostDisk& h, std::vector<double>& khat_r, std::vector<double>&
 // engine accessors used by the output writers (engine.hip)
 const HostDisk* batch_host(fgx_batch* b, int d);
 int batch_mdcp(fgx_batch* b, int d, double* out);
+// serving: queue an asynchronous copy of a disk's stored fields into its
+// pinned host buffer, and wait for the last one queued
+int batch_snapshot_queue(fgx_batch* b, int d);
+int batch_snapshot_wait(fgx_batch* b, int d, int* had);
 
 }  // namespace fgx
 

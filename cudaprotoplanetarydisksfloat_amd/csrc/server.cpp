@@ -1,12 +1,24 @@
 // server.cpp -- serving loop over the batch engine (fgx_server_*).
 //
 // The reference runs one disk per process from main() (src/Main.cu:87-401).
-// Here many independent disk runs ("requests") arrive over time and share
-// one GPU: an admission thread hands each group of arrived requests to a
-// worker that builds a batch (own HIP stream) and advances it one DT at a
-// time, retiring requests as they finish and snapshotting fields on the
-// reference's output cadence.  Workers of different batches run
-// concurrently, so their kernels overlap on the device.
+// Here independent disk runs ("requests") arrive over time and share one GPU
+// through continuous batching:
+// - at most max_live_batches batches are live at once (no more than the
+//   device's hardware queues), each advanced one DT at a time by a worker
+//   thread on the batch's own HIP stream, so batches overlap on the device;
+// - a batch is created for a group of waiting requests that share the batch
+//   mode flags, with spare slots beside them: frozen disks of the group's
+//   grids, allocated with the batch;
+// - a request that has done its DTs is frozen and its slot goes back to the
+//   pool at once; at its batch's next DT boundary a waiting request of the
+//   same grid and modes takes the slot (fgx_batch_reset_disk: nothing is
+//   allocated, the disk evolves exactly as alone);
+// - a batch whose slots are all free stays live for later refills, and is
+//   released when a request needs its memory or its queue position, or when
+//   no request is left to come;
+// - snapshots are asynchronous copies into pinned host memory, queued behind
+//   the DT's kernels (the reference writes its outputs every NINTERM steps,
+//   src/Main.cu:348-371).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -26,107 +38,315 @@ struct fgx_server {
     double arrival = 0;
     fgx_request_info info{};
     size_t footprint = 0;
-    bool admitted = false;
+    std::vector<int> refused;   // batches whose slot set-up refused the request
     std::vector<double> final_fields[FGX_NFIELDS];
+  };
+  struct Slot {
+    int req = -1;         // request held, -1 = free (frozen)
+    bool fresh = false;   // assigned; its set-up is the worker's next task
+    bool initial = false; // the batch was created with this request's disk
+    int left = 0;         // DTs still to do
+    int nrad = 0, nsec = 0;
+  };
+  struct Live {
+    int id = 0;
+    fgx_params P0{};            // the batch's mode flags
+    std::vector<Slot> slots;
+    size_t foot = 0;            // device memory of all its slots
+    bool retire = false;        // take no more requests; end when idle
+    bool dead = false;          // the worker has ended and returned the memory
+    std::thread th;
   };
   fgx_server_config cfg{};
   std::vector<Request> reqs;
   std::mutex mu;
   std::condition_variable cv;
+  std::vector<std::unique_ptr<Live>> live;
+  std::vector<int> waiting;     // arrived, not yet in a slot; arrival order
   size_t in_use = 0;
-  int live = 0, batches = 0;
+  fgx_server_stats stats{};
   std::chrono::steady_clock::time_point t0;
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  int max_live() const { return cfg.max_live_batches > 0 ? cfg.max_live_batches : 4; }
 };
 
 namespace {
 
-void run_batch(fgx_server* S, std::vector<int> ids) {
-  const int n = (int)ids.size();
-  std::vector<fgx_params> ps(n);
-  std::vector<fgx_planets> pls(n);
-  size_t foot = 0;
-  int batch;
-  {
-    std::lock_guard<std::mutex> g(S->mu);
-    for (int k = 0; k < n; ++k) { ps[k] = S->reqs[ids[k]].p; pls[k] = S->reqs[ids[k]].pl; foot += S->reqs[ids[k]].footprint; }
-    batch = S->batches++;
-  }
+using Server = fgx_server;
+using Live = fgx_server::Live;
+
+bool idle(const Live& L) {
+  for (const auto& sl : L.slots)
+    if (sl.req >= 0) return false;
+  return true;
+}
+
+// a request back into the waiting queue at its arrival position
+void requeue(Server* S, int rid) {
+  auto it = std::upper_bound(S->waiting.begin(), S->waiting.end(), rid, [&](int a, int b) {
+    return S->reqs[a].arrival < S->reqs[b].arrival || (S->reqs[a].arrival == S->reqs[b].arrival && a < b);
+  });
+  S->waiting.insert(it, rid);
+}
+
+void worker(Server* S, Live* L, std::vector<fgx_params> ps, std::vector<fgx_planets> pls) {
+  const int n = (int)ps.size();
   fgx_batch* b = nullptr;
   int rc = fgx_batch_create(n, ps.data(), pls.data(), S->cfg.precision, &b);
-  std::vector<int> left(n);
-  std::vector<long long> cells(n);
-  for (int k = 0; k < n; ++k) {
-    left[k] = S->reqs[ids[k]].n_dt;
-    cells[k] = (long long)ps[k].nrad * ps[k].nsec;
-  }
-  std::vector<double> snap;
   const int fields[4] = {FGX_DENS, FGX_VRAD, FGX_VTHETA, FGX_ENERGY};
-  auto finish = [&](int k, int status) {
+  // record a request's end: final fields (device reads) first, then the
+  // record under the lock; the slot is frozen and freed by the caller
+  auto finish = [&](int k, int rid, int status) {
+    const Server::Request& r = S->reqs[rid];
     fgx_disk_info di{};
     if (b && status == FGX_OK) status = fgx_batch_info(b, k, &di);
     std::vector<double> keep[4];
     if (b && status == FGX_OK && S->cfg.keep_final)
       for (int f = 0; f < 4; ++f) {
-        if (fields[f] == FGX_ENERGY && !ps[k].adiabatic) continue;
-        keep[f].resize((size_t)(ps[k].nrad + 1) * ps[k].nsec);
+        if (fields[f] == FGX_ENERGY && !r.p.adiabatic) continue;
+        keep[f].resize((size_t)(r.p.nrad + 1) * r.p.nsec);
         if (fgx_batch_read_field(b, k, fields[f], FGX_F64, keep[f].data(), keep[f].size())) status = FGX_EDEVICE;
       }
     std::lock_guard<std::mutex> g(S->mu);
-    fgx_request_info& in = S->reqs[ids[k]].info;
+    fgx_request_info& in = S->reqs[rid].info;
     in.finish = S->now();
     in.status = status;
     in.substeps = di.substeps;
-    in.cell_updates = di.substeps * cells[k];
-    in.batch = batch;
+    in.cell_updates = di.substeps * (long long)r.p.nrad * r.p.nsec;
+    in.batch = L->id;
     in.batch_size = n;
-    for (int f = 0; f < 4; ++f) S->reqs[ids[k]].final_fields[fields[f]] = std::move(keep[f]);
+    in.slot = k;
+    for (int f = 0; f < 4; ++f) S->reqs[rid].final_fields[fields[f]] = std::move(keep[f]);
+    L->slots[k].req = -1;
+    L->slots[k].fresh = false;
+    S->cv.notify_all();
   };
   if (rc) {
-    for (int k = 0; k < n; ++k) finish(k, rc);
+    for (int k = 0; k < n; ++k)
+      if (L->slots[k].req >= 0) finish(k, L->slots[k].req, rc);
   } else {
-    for (int step = 1;; ++step) {
-      bool any = false;
-      for (int k = 0; k < n; ++k) any |= left[k] > 0;
-      if (!any) break;
+    // the group's requests start as created; the spare slots wait frozen
+    // (a request placed in one meanwhile is set up like any refill)
+    std::vector<char> spare(n, 0);
+    {
+      std::lock_guard<std::mutex> g(S->mu);
+      for (int k = 0; k < n; ++k) {
+        Server::Slot& sl = L->slots[k];
+        if (sl.initial) {
+          sl.fresh = false;
+          sl.left = S->reqs[sl.req].n_dt;
+        } else {
+          spare[k] = 1;
+        }
+        sl.initial = false;
+      }
+    }
+    for (int k = 0; k < n; ++k)
+      if (spare[k]) fgx_batch_freeze(b, k, 1);
+    std::vector<int> todo, act;
+    for (;;) {
+      todo.clear();
+      {
+        std::unique_lock<std::mutex> lk(S->mu);
+        bool any = false;
+        for (int k = 0; k < n; ++k) {
+          const Server::Slot& sl = L->slots[k];
+          if (sl.req >= 0 && sl.fresh) todo.push_back(k);
+          any |= sl.req >= 0;
+        }
+        if (!any) {
+          if (L->retire) break;
+          S->cv.wait_for(lk, std::chrono::milliseconds(20));   // idle in the pool
+          continue;
+        }
+      }
+      // refills: set up the assigned requests in their slots
+      for (int k : todo) {
+        const int rid = L->slots[k].req;
+        const Server::Request& r = S->reqs[rid];
+        rc = fgx_batch_reset_disk(b, k, &r.p, &r.pl);
+        std::lock_guard<std::mutex> g(S->mu);
+        Server::Slot& sl = L->slots[k];
+        if (rc == FGX_OK) {
+          sl.fresh = false;
+          sl.left = r.n_dt;
+          S->stats.refills++;
+        } else if (rc == FGX_EPARAM) {
+          // a pass the batch lacks: the request waits for another batch
+          S->reqs[rid].refused.push_back(L->id);
+          S->reqs[rid].info.admit = -1;
+          sl.req = -1;
+          sl.fresh = false;
+          requeue(S, rid);
+          S->cv.notify_all();
+        } else {
+          sl.fresh = false;
+          S->reqs[rid].info.status = rc;
+          S->reqs[rid].info.finish = S->now();
+          sl.req = -1;
+          S->cv.notify_all();
+        }
+      }
+      act.clear();
+      {
+        std::lock_guard<std::mutex> g(S->mu);
+        for (int k = 0; k < n; ++k)
+          if (L->slots[k].req >= 0 && !L->slots[k].fresh) act.push_back(k);
+      }
+      if (act.empty()) continue;
       rc = fgx_batch_step(b, 1);
       if (rc == FGX_ENUMERIC) {
-        // the disks found numerically unhealthy were frozen by the step; their
-        // requests end with that status, the others went on
-        for (int k = 0; k < n; ++k) {
+        // the disks found numerically unhealthy were frozen by the step;
+        // their requests end with that status, the others went on
+        std::vector<int> keep;
+        for (int k : act) {
           fgx_disk_info di{};
-          if (left[k] > 0 && fgx_batch_info(b, k, &di) == FGX_OK && di.health) {
-            left[k] = 0;
-            finish(k, FGX_ENUMERIC);
+          if (fgx_batch_info(b, k, &di) == FGX_OK && di.health) {
+            int had = 0;
+            fgx::batch_snapshot_wait(b, k, &had);
+            finish(k, L->slots[k].req, FGX_ENUMERIC);
+          } else {
+            keep.push_back(k);
           }
         }
+        act.swap(keep);
         rc = FGX_OK;
       }
       if (rc) {
-        for (int k = 0; k < n; ++k)
-          if (left[k] > 0) { left[k] = 0; finish(k, rc); }
-        break;
-      }
-      for (int k = 0; k < n; ++k) {
-        if (left[k] <= 0) continue;
-        if (--left[k] == 0) {
-          finish(k, FGX_OK);
+        for (int k : act) {
+          finish(k, L->slots[k].req, rc);
           fgx_batch_freeze(b, k, 1);
-        } else if (S->cfg.snapshot_every > 0 && step % S->cfg.snapshot_every == 0) {
-          // snapshot of the running request (WriteDiskPolar cadence)
-          snap.resize((size_t)(ps[k].nrad + 1) * ps[k].nsec);
-          for (int f = 0; f < 3; ++f) fgx_batch_read_field(b, k, fields[f], FGX_F64, snap.data(), snap.size());
-          std::lock_guard<std::mutex> g(S->mu);
-          S->reqs[ids[k]].info.snapshots++;
+        }
+        std::lock_guard<std::mutex> g(S->mu);
+        L->retire = true;   // the batch is in an unknown state
+        continue;
+      }
+      for (int k : act) {
+        const int rid = L->slots[k].req;
+        const Server::Request& r = S->reqs[rid];
+        const int left = --L->slots[k].left;
+        const int done = r.n_dt - left;
+        int had = 0;
+        if (left == 0) {
+          // the last snapshot has landed long ago; the slot returns to the pool
+          fgx::batch_snapshot_wait(b, k, &had);
+          if (had) { std::lock_guard<std::mutex> g(S->mu); S->reqs[rid].info.snapshots++; }
+          fgx_batch_freeze(b, k, 1);
+          finish(k, rid, FGX_OK);
+        } else if (S->cfg.snapshot_every > 0 && done % S->cfg.snapshot_every == 0) {
+          fgx::batch_snapshot_wait(b, k, &had);
+          if (had) { std::lock_guard<std::mutex> g(S->mu); S->reqs[rid].info.snapshots++; }
+          if (fgx::batch_snapshot_queue(b, k) == FGX_OK) {}
         }
       }
     }
   }
   if (b) fgx_batch_destroy(b);
   std::lock_guard<std::mutex> g(S->mu);
-  S->in_use -= foot;
-  S->live--;
+  S->in_use -= L->foot;
+  L->dead = true;
   S->cv.notify_all();
+}
+
+// place waiting requests into free slots of live batches; then form new
+// batches while the live count and the budget allow.  Called with the lock.
+void admit(Server* S, double t, bool more_to_come) {
+  // 1. free slots of live batches
+  for (size_t q = 0; q < S->waiting.size();) {
+    const int rid = S->waiting[q];
+    Server::Request& r = S->reqs[rid];
+    bool placed = false;
+    for (auto& Lp : S->live) {
+      Live& L = *Lp;
+      if (L.retire || L.dead || !fgx::batch_modes_equal(r.p, L.P0)) continue;
+      if (std::find(r.refused.begin(), r.refused.end(), L.id) != r.refused.end()) continue;
+      for (auto& sl : L.slots)
+        if (sl.req < 0 && sl.nrad == r.p.nrad && sl.nsec == r.p.nsec) {
+          sl.req = rid;
+          sl.fresh = true;
+          r.info.admit = t;
+          r.info.batch = L.id;
+          placed = true;
+          break;
+        }
+      if (placed) break;
+    }
+    if (placed) {
+      S->waiting.erase(S->waiting.begin() + q);
+      S->cv.notify_all();
+    } else {
+      ++q;
+    }
+  }
+  // 2. new batches
+  while (!S->waiting.empty()) {
+    int nlive = 0;
+    for (auto& Lp : S->live) nlive += !Lp->retire && !Lp->dead;
+    const int rid0 = S->waiting[0];
+    Server::Request& first = S->reqs[rid0];
+    if (S->cfg.batch_window > 0 && t < first.arrival + S->cfg.batch_window && nlive > 0) break;
+    // the group: the first waiting request and the next ones of its modes
+    std::vector<int> group{rid0};
+    size_t foot = first.footprint;
+    for (size_t q = 1; q < S->waiting.size() && (int)group.size() < S->cfg.max_batch_disks; ++q) {
+      const Server::Request& r = S->reqs[S->waiting[q]];
+      if (!fgx::batch_modes_equal(r.p, first.p)) continue;
+      if (S->in_use + foot + r.footprint > S->cfg.hbm_budget) break;
+      group.push_back(S->waiting[q]);
+      foot += r.footprint;
+    }
+    const bool room = nlive < S->max_live() && S->in_use + foot <= S->cfg.hbm_budget;
+    if (!room) {
+      // release an idle batch for the memory or the queue position
+      for (auto& Lp : S->live)
+        if (!Lp->retire && !Lp->dead && idle(*Lp)) {
+          Lp->retire = true;
+          S->cv.notify_all();
+          break;
+        }
+      break;
+    }
+    // spare slots: frozen copies of the group's grids, while the budget holds
+    std::vector<int> spare;
+    for (int q = 0; q < S->cfg.spare_slots && (int)(group.size() + spare.size()) < S->cfg.max_batch_disks; ++q) {
+      const int src = group[q % group.size()];
+      if (S->in_use + foot + S->reqs[src].footprint > S->cfg.hbm_budget) break;
+      spare.push_back(src);
+      foot += S->reqs[src].footprint;
+    }
+    auto L = std::make_unique<Live>();
+    L->id = S->stats.batches++;
+    L->P0 = first.p;
+    L->foot = foot;
+    std::vector<fgx_params> ps;
+    std::vector<fgx_planets> pls;
+    for (int rid : group) {
+      Server::Request& r = S->reqs[rid];
+      Server::Slot sl;
+      sl.req = rid; sl.fresh = true; sl.initial = true; sl.nrad = r.p.nrad; sl.nsec = r.p.nsec;
+      L->slots.push_back(sl);
+      ps.push_back(r.p); pls.push_back(r.pl);
+      r.info.admit = t;
+      r.info.batch = L->id;
+      S->waiting.erase(std::find(S->waiting.begin(), S->waiting.end(), rid));
+    }
+    for (int src : spare) {
+      Server::Slot sl;
+      sl.nrad = S->reqs[src].p.nrad; sl.nsec = S->reqs[src].p.nsec;
+      L->slots.push_back(sl);
+      ps.push_back(S->reqs[src].p); pls.push_back(S->reqs[src].pl);
+    }
+    S->in_use += foot;
+    S->stats.peak_bytes = std::max(S->stats.peak_bytes, S->in_use);
+    Live* raw = L.get();
+    S->live.push_back(std::move(L));
+    raw->th = std::thread(worker, S, raw, std::move(ps), std::move(pls));
+    S->stats.peak_live = std::max(S->stats.peak_live, nlive + 1);
+  }
+  // no request left to come or wait: the idle batches are done
+  if (!more_to_come && S->waiting.empty())
+    for (auto& Lp : S->live)
+      if (!Lp->retire) { Lp->retire = true; S->cv.notify_all(); }
 }
 
 }  // namespace
@@ -137,6 +357,7 @@ int fgx_server_create(const fgx_server_config* cfg, fgx_server** out) {
   if (!cfg || !out) return fgx::set_error(FGX_EPARAM, "null argument");
   if (cfg->precision != FGX_F32 && cfg->precision != FGX_F64) return fgx::set_error(FGX_EPARAM, "bad precision");
   if (cfg->hbm_budget == 0 || cfg->max_batch_disks < 1) return fgx::set_error(FGX_EPARAM, "bad server budget");
+  if (cfg->max_live_batches < 0 || cfg->spare_slots < 0) return fgx::set_error(FGX_EPARAM, "bad slot counts");
   auto* s = new fgx_server();
   s->cfg = *cfg;
   *out = s;
@@ -155,7 +376,7 @@ int fgx_server_submit(fgx_server* s, const fgx_params* p, const fgx_planets* pl,
   r.info.arrival = arrival;
   r.info.admit = r.info.finish = -1;
   r.info.n_dt = n_dt;
-  r.info.batch = -1;
+  r.info.batch = r.info.slot = -1;
   rc = fgx_footprint_bytes(p, s->cfg.precision, &r.footprint);
   if (rc) return rc;
   if (r.footprint > s->cfg.hbm_budget)
@@ -172,60 +393,46 @@ int fgx_server_run(fgx_server* s) {
   std::vector<int> order(s->reqs.size());
   for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return s->reqs[a].arrival < s->reqs[b].arrival; });
-  std::vector<std::thread> workers;
   s->t0 = std::chrono::steady_clock::now();
   std::unique_lock<std::mutex> lk(s->mu);
-  size_t next = 0;   // first request in arrival order not yet admitted
+  size_t next = 0;   // first request in arrival order not yet arrived
   for (;;) {
     const double t = s->now();
-    // admission in arrival order: a group is the first waiting request and
-    // the next arrived ones with the same modes, while the budget holds
-    while (next < order.size() && s->reqs[order[next]].arrival <= t) {
-      fgx_server::Request& first = s->reqs[order[next]];
-      // wait a little for companions to batch with, unless the window is over
-      if (s->cfg.batch_window > 0 && t < first.arrival + s->cfg.batch_window && s->live > 0) break;
-      if (s->in_use + first.footprint > s->cfg.hbm_budget) break;
-      std::vector<int> group{order[next]};
-      size_t used = first.footprint;
-      size_t k = next + 1;
-      while (k < order.size() && (int)group.size() < s->cfg.max_batch_disks && s->reqs[order[k]].arrival <= t &&
-             fgx::batch_modes_equal(s->reqs[order[k]].p, first.p) && s->in_use + used + s->reqs[order[k]].footprint <= s->cfg.hbm_budget) {
-        group.push_back(order[k]);
-        used += s->reqs[order[k]].footprint;
+    while (next < order.size() && s->reqs[order[next]].arrival <= t) s->waiting.push_back(order[next++]);
+    admit(s, t, next < order.size());
+    // join the batches that have ended
+    for (size_t k = 0; k < s->live.size();) {
+      if (s->live[k]->dead) {
+        std::unique_ptr<Live> L = std::move(s->live[k]);
+        s->live.erase(s->live.begin() + k);
+        lk.unlock();
+        L->th.join();
+        lk.lock();
+      } else {
         ++k;
       }
-      // requests skipped for a different mode stay in order behind this group
-      std::vector<int> rest;
-      for (size_t q = next + 1; q < k; ++q)
-        if (std::find(group.begin(), group.end(), order[q]) == group.end()) rest.push_back(order[q]);
-      for (int id : group) { s->reqs[id].admitted = true; s->reqs[id].info.admit = t; }
-      std::copy(rest.begin(), rest.end(), order.begin() + next + group.size());
-      std::copy(group.begin(), group.end(), order.begin() + next);
-      next += group.size();
-      s->in_use += used;
-      s->live++;
-      workers.emplace_back(run_batch, s, group);
     }
-    if (next >= order.size() && s->live == 0) break;
-    // sleep until the next arrival (or window end) or until a batch retires
+    if (next >= order.size() && s->waiting.empty() && s->live.empty()) break;
+    // sleep until the next arrival (or window end), or until a slot or a
+    // batch frees up
     double wake = 1e30;
-    if (next < order.size()) {
-      wake = s->reqs[order[next]].arrival;
-      if (s->cfg.batch_window > 0 && s->live > 0) wake += s->cfg.batch_window;
-    }
-    if (wake < 1e29) {
-      const double dt = std::max(0.0, wake - s->now());
-      s->cv.wait_for(lk, std::chrono::duration<double>(std::min(dt, 0.05)));
-    } else {
-      s->cv.wait_for(lk, std::chrono::milliseconds(50));
-    }
+    if (next < order.size()) wake = s->reqs[order[next]].arrival;
+    if (s->cfg.batch_window > 0 && !s->waiting.empty()) wake = std::min(wake, s->reqs[s->waiting[0]].arrival + s->cfg.batch_window);
+    const double dt = std::max(0.0, std::min(wake - s->now(), 0.05));
+    s->cv.wait_for(lk, std::chrono::duration<double>(dt));
   }
   lk.unlock();
-  for (auto& w : workers) w.join();
   int rc = FGX_OK;
   for (auto& r : s->reqs)
     if (r.info.status != FGX_OK && rc == FGX_OK) rc = r.info.status;
   return rc;
+}
+
+int fgx_server_stats_get(fgx_server* s, fgx_server_stats* out) {
+  if (!s || !out) return fgx::set_error(FGX_EPARAM, "null argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  *out = s->stats;
+  return FGX_OK;
 }
 
 int fgx_server_request(fgx_server* s, int id, fgx_request_info* out) {

@@ -203,88 +203,86 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
     const real sl = rad_vl(m.q[q], z.q[q], p.q[q], g.im, g.ip);
     return g.inner ? sl : (real)0;
   };
-  // prologue: rings i0-2 .. i0+1
-  Ring rm2, rm1, ra, rb1;
-  rad_load<real, LAB>(F, ns, nr, i0 - 2, j, jp, om, ad, rm2);
-  rad_load<real, LAB>(F, ns, nr, i0 - 1, j, jp, om, ad, rm1);
-  rad_load<real, LAB>(F, ns, nr, i0, j, jp, om, ad, ra);
-  rad_load<real, LAB>(F, ns, nr, i0 + 1, j, jp, om, ad, rb1);
-  real slA[NQ], slM[NQ], flux[NQ];
-  const SlopeRing gm = slope_ring(i0 - 1), ga = slope_ring(i0);
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    if (!qon(q)) { slA[q] = slM[q] = (real)0; continue; }
-    slM[q] = slope_at(gm, rm2, rm1, ra, q);
-    slA[q] = slope_at(ga, rm1, ra, rb1, q);
-  }
-  // flux through interface i0 (zero at the inner edge).  The ring spacings
-  // are formed before the quantity loop: read inside the two upwind arms,
-  // the table loads had turned each selection into a divergent branch.
-  {
-    const real v = ra.vr0;
-    const real hm = rmed[i0] - rmed[i0 - 1 < 0 ? 0 : i0 - 1], hp = rmed[i0 + 1] - rmed[i0];
-    real st[NQ];
+  // flux through the interface between ring lo and ring lo + 1, zero at the
+  // disk's edges.  The ring spacings are formed before the quantity loop:
+  // read inside the two upwind arms, the table loads had turned each
+  // selection into a divergent branch.
+  auto iface = [&](int lo, const Ring& L, const Ring& H, const real (&sL)[NQ], const real (&sH)[NQ],
+                   real (&f)[NQ]) {
+    const real v = H.vr0;
+    const bool edge = lo < 0 || lo + 1 >= nr;
+    const int l0 = lo < 0 ? 0 : lo, l1 = lo + 1 < 0 ? 0 : lo + 1, l2 = lo + 2 < 0 ? 0 : (lo + 2 <= nr ? lo + 2 : nr);
+    const real hm = rmed[l1] - rmed[l0], hp = rmed[l2] - rmed[l1];
+    const real a = edge ? (real)0 : dt * dth * RT(rinf)[rb + l1];
+    const real cm = (hm - v * dt) * (real)0.5, cp = (hp + v * dt) * (real)0.5;
+    const bool up = v > 0;
+    real st0 = 0;
+    f[0] = 0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      if (!qon(q)) { st[q] = 0; continue; }
-      st[q] = (v > 0) ? rm1.q[q] + (hm - v * dt) * (real)0.5 * slM[q] : ra.q[q] - (hp + v * dt) * (real)0.5 * slA[q];
+      if (!qon(q)) { f[q] = 0; continue; }
+      // both upwind values, then a selection (a conditional expression over
+      // the two formulas compiles to a divergent branch per quantity)
+      const real from_lo = L.q[q] + cm * sL[q], from_hi = H.q[q] - cp * sH[q];
+      const real st = up ? from_lo : from_hi;
+      if (q == 0) st0 = st;
+      else f[q] = edge ? (real)0 : a * st * st0 * v;
     }
-    const real a = dt * dth * RT(rinf)[rb + i0];
-    flux[0] = 0;
+  };
+  // The walk starts two rings before the strip: those two passes only form
+  // the slopes of rings i0-1 and i0 and the flux through interface i0, with
+  // the same instructions as every later ring, so no value depends on where
+  // a strip begins (a separate prologue rounds them differently once the
+  // compiler fuses its products into the flux arithmetic).
+  Ring ra, rb1;
+  rad_load<real, LAB>(F, ns, nr, i0 - 2, j, jp, om, ad, ra);
+  rad_load<real, LAB>(F, ns, nr, i0 - 1, j, jp, om, ad, rb1);
+  real slA[NQ], flux[NQ];
 #pragma unroll
-    for (int q = 1; q < NQ; ++q) flux[q] = (i0 == 0 || !qon(q)) ? (real)0 : a * st[q] * st[0] * v;
-  }
+  for (int q = 0; q < NQ; ++q) slA[q] = flux[q] = (real)0;
   const auto isurf = RT(invsurf) + rb;
-  // ring i+2 is fetched one iteration ahead, before ring i-1's stores: the
-  // vector memory counter retires loads and stores in issue order
-  RadRaw<real> nxt;
-  rad_fetch<real, LAB>(F, ns, nr, i0 + 2, j, jp, ad, nxt);
-  for (int i = i0; i < i1; ++i) {
+  // ring i+2 is fetched two iterations ahead, before the stores of rings
+  // i-2 and i-1: the vector memory counter retires loads and stores in issue
+  // order, and four rings are in flight when the walk starts
+  RadRaw<real> nxt, nxt2;
+  rad_fetch<real, LAB>(F, ns, nr, i0, j, jp, ad, nxt);
+  rad_fetch<real, LAB>(F, ns, nr, i0 + 1, j, jp, ad, nxt2);
+  for (int i = i0 - 2; i < i1; ++i) {
     const RadRaw<real> cur = nxt;
-    if (i + 1 < i1) rad_fetch<real, LAB>(F, ns, nr, i + 3, j, jp, ad, nxt);
+    nxt = nxt2;
+    if (i + 2 < i1) rad_fetch<real, LAB>(F, ns, nr, i + 4, j, jp, ad, nxt2);
     Ring rc;
     rad_derive<real, LAB>(F, nr, i + 2, om, ad, cur, rc);
-    // interface i+1 (zero flux at the outer edge); the ring values this
-    // iteration needs are read together, ahead of their use
-    const real v = rb1.vr0;
-    const bool edge = (i + 1 == nr);
+    // slope of ring i+1, flux through interface i+1 (zero at the outer edge)
     const SlopeRing gb = slope_ring(i + 1);
-    const real rinf1 = RT(rinf)[rb + (edge ? i : i + 1)];
-    const real is = isurf[i];
-    const double rd = F.rmed_d[i], vref_i = F.vref[i];
-    const real a = edge ? (real)0 : dt * dth * rinf1;
-    const real hm = rmed[i + 1] - rmed[i], hp = rmed[i + 2 <= nr ? i + 2 : nr] - rmed[i + 1];
     real fnew[NQ], slB[NQ];
-    real st0 = 0;
-    fnew[0] = 0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      if (!qon(q)) { slB[q] = 0; fnew[q] = 0; continue; }
-      slB[q] = slope_at(gb, ra, rb1, rc, q);
-      const real st = (v > 0) ? ra.q[q] + (hm - v * dt) * (real)0.5 * slA[q] : rb1.q[q] - (hp + v * dt) * (real)0.5 * slB[q];
-      if (q == 0) st0 = st;
-      else fnew[q] = edge ? (real)0 : a * st * st0 * v;
+    for (int q = 0; q < NQ; ++q) slB[q] = qon(q) ? slope_at(gb, ra, rb1, rc, q) : (real)0;
+    iface(i, ra, rb1, slA, slB, fnew);
+    if (i >= i0) {
+      const real is = isurf[i];
+      const double rd = F.rmed_d[i], vref_i = F.vref[i];
+      // cell i: quantity + (flux in - flux out) / Surf
+      const real inc = (flux[1] - fnew[1]) * is;
+      const real sg = ra.sg;
+      const int oc = i * ns + j;
+      o_dens[oc] = ra.sr + inc;   // a residual of S0, as the field
+      if (i == 0 && D.open_inner) D.lost[c.so + j] = flux[1] - fnew[1];
+      o_rmp[oc] = sg * ra.vr1 + (flux[2] - fnew[2]) * is;
+      o_rmm[oc] = sg * ra.vr0 + (flux[3] - fnew[3]) * is;
+      // angular momenta leave as residuals of density * l0(ring), with the
+      // density taken before its field rounding, so the azimuthal velocity
+      // rebuilt from them carries no trace of that rounding:
+      //   (L + dL) - (Sigma + dSigma) l0 = Sigma (l - l0) + dL - dSigma l0,
+      // formed from its small terms only (the O(1) products Sigma l and
+      // Sigma l0 would each carry a rounding of the size of a vtheta ulp)
+      const double l0i = (vref_i + rd * om) * rd;
+      const double dsl = (double)inc * l0i;
+      o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - dsl);
+      o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - dsl);
+      if (ad) o_e[oc] = ra.e + (flux[6] - fnew[6]) * is;
+      if constexpr (LAB) o_lab[oc] = ra.x + (flux[7] - fnew[7]) * is;
     }
-    // cell i: quantity + (flux in - flux out) / Surf
-    const real inc = (flux[1] - fnew[1]) * is;
-    const real sg = ra.sg;
-    const int oc = i * ns + j;
-    o_dens[oc] = ra.sr + inc;   // a residual of S0, as the field
-    if (i == 0 && D.open_inner) D.lost[c.so + j] = flux[1] - fnew[1];
-    o_rmp[oc] = sg * ra.vr1 + (flux[2] - fnew[2]) * is;
-    o_rmm[oc] = sg * ra.vr0 + (flux[3] - fnew[3]) * is;
-    // angular momenta leave as residuals of density * l0(ring), with the
-    // density taken before its field rounding, so the azimuthal velocity
-    // rebuilt from them carries no trace of that rounding:
-    //   (L + dL) - (Sigma + dSigma) l0 = Sigma (l - l0) + dL - dSigma l0,
-    // formed from its small terms only (the O(1) products Sigma l and
-    // Sigma l0 would each carry a rounding of the size of a vtheta ulp)
-    const double l0i = (vref_i + rd * om) * rd;
-    const double dsl = (double)inc * l0i;
-    o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - dsl);
-    o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - dsl);
-    if (ad) o_e[oc] = ra.e + (flux[6] - fnew[6]) * is;
-    if constexpr (LAB) o_lab[oc] = ra.x + (flux[7] - fnew[7]) * is;
     // slide the window
 #pragma unroll
     for (int q = 0; q < NQ; ++q) { flux[q] = fnew[q]; slA[q] = slB[q]; }

@@ -77,7 +77,8 @@ class ServerConfig(ctypes.Structure):
     """fgx_server_config (include/fgx.h)."""
     _fields_ = [("precision", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("max_batch_disks", ctypes.c_int), ("snapshot_every", ctypes.c_int),
-                ("batch_window", ctypes.c_double), ("keep_final", ctypes.c_int)]
+                ("batch_window", ctypes.c_double), ("keep_final", ctypes.c_int),
+                ("max_live_batches", ctypes.c_int), ("spare_slots", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -85,7 +86,13 @@ class RequestInfo(ctypes.Structure):
     _fields_ = [("arrival", ctypes.c_double), ("admit", ctypes.c_double), ("finish", ctypes.c_double),
                 ("substeps", ctypes.c_longlong), ("cell_updates", ctypes.c_longlong),
                 ("n_dt", ctypes.c_int), ("snapshots", ctypes.c_int), ("batch", ctypes.c_int),
-                ("batch_size", ctypes.c_int), ("status", ctypes.c_int)]
+                ("batch_size", ctypes.c_int), ("status", ctypes.c_int), ("slot", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    """fgx_server_stats: batches created, slot refills, peak live batches and bytes."""
+    _fields_ = [("batches", ctypes.c_int), ("refills", ctypes.c_int), ("peak_live", ctypes.c_int),
+                ("peak_bytes", ctypes.c_size_t)]
 
 
 _vp, _cp, _sz, _i, _d = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
@@ -128,6 +135,7 @@ _sigs = {
     "fgx_server_run": ([_vp], _i),
     "fgx_server_request": ([_vp, _i, ctypes.POINTER(RequestInfo)], _i),
     "fgx_server_read_field": ([_vp, _i, _i, _i, _vp, _sz], _i),
+    "fgx_server_stats_get": ([_vp, ctypes.POINTER(ServerStats)], _i),
     "fgx_server_destroy": ([_vp], _i),
 }
 for _n, (_a, _r) in _sigs.items():
@@ -346,8 +354,9 @@ class Server:
     concurrent batches; per-request admit/finish times come back."""
 
     def __init__(self, precision=F32, hbm_budget=256 << 30, max_batch_disks=64, snapshot_every=100,
-                 batch_window=0.0, keep_final=False):
-        cfg = ServerConfig(precision, hbm_budget, max_batch_disks, snapshot_every, batch_window, int(keep_final))
+                 batch_window=0.0, keep_final=False, max_live_batches=4, spare_slots=2):
+        cfg = ServerConfig(precision, hbm_budget, max_batch_disks, snapshot_every, batch_window, int(keep_final),
+                           max_live_batches, spare_slots)
         self.h = ctypes.c_void_p()
         _check(lib.fgx_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
         self.shapes = []
@@ -366,6 +375,11 @@ class Server:
         info = RequestInfo()
         _check(lib.fgx_server_request(self.h, rid, ctypes.byref(info)))
         return info
+
+    def stats(self):
+        out = ServerStats()
+        _check(lib.fgx_server_stats_get(self.h, ctypes.byref(out)))
+        return out
 
     def field(self, rid, name):
         nr, ns = self.shapes[rid]

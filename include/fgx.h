@@ -267,13 +267,18 @@ int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes);
 
 /* ---- serving loop (a stream of run requests; no counterpart in the reference)
  * Requests (one disk each: parameters, planets, number of DTs, arrival time
- * in seconds from the start of fgx_server_run) are admitted in arrival order
- * while the summed device footprint stays within hbm_budget; requests that
- * arrive together and share the batch mode flags form one batch.  Each live
- * batch advances on its own stream and host thread, one DT at a time; a
- * request that has done its DTs is frozen and its finish time recorded;
- * every snapshot_every DTs the fields of each running request are copied to
- * host memory (the reference writes its outputs every NINTERM steps,
+ * in seconds from the start of fgx_server_run) are served by continuous
+ * batching.  At most max_live_batches batches are live, each advanced one DT
+ * at a time on its own stream and host thread.  A new batch holds a group of
+ * waiting requests that share the batch mode flags, plus spare_slots frozen
+ * slots of the group's grids, while the summed device footprint of the live
+ * batches stays within hbm_budget.  A request that has done its DTs is frozen
+ * and its slot returns to the pool at once: at the batch's next DT boundary
+ * a waiting request of the same grid and modes takes it (fgx_batch_reset_disk,
+ * no allocation).  A batch with every slot free is released when a waiting
+ * request needs its memory or place, or when nothing is left to come.  Every
+ * snapshot_every DTs of a request its fields are copied asynchronously into
+ * pinned host memory (the reference writes its outputs every NINTERM steps,
  * src/Main.cu:348-371).  A request's fields evolve exactly as they would
  * alone. */
 typedef struct fgx_server fgx_server;
@@ -284,18 +289,28 @@ typedef struct fgx_server_config {
   int    snapshot_every;   /* DTs between snapshots (0: none)                  */
   double batch_window;     /* seconds to wait for more arrivals to batch with  */
   int    keep_final;       /* keep each request's final fields for read-back   */
+  int    max_live_batches; /* batches live at once (0: 4, the hardware queues)  */
+  int    spare_slots;      /* frozen slots a new batch holds beside its group   */
 } fgx_server_config;
 typedef struct fgx_request_info {
   double arrival, admit, finish;      /* seconds from the start of the run   */
   long long substeps, cell_updates;   /* CFL substeps done, cells x substeps */
   int    n_dt, snapshots, batch, batch_size, status;   /* status: FGX_OK or error */
+  int    slot;                        /* the slot of its batch it ran in     */
 } fgx_request_info;
+typedef struct fgx_server_stats {
+  int    batches;       /* batches created                                 */
+  int    refills;       /* requests set up in a freed slot of a live batch */
+  int    peak_live;     /* most batches live at once                       */
+  size_t peak_bytes;    /* most device memory the live batches held        */
+} fgx_server_stats;
 int fgx_server_create(const fgx_server_config* cfg, fgx_server** out);
 int fgx_server_submit(fgx_server* s, const fgx_params* p, const fgx_planets* pl, int n_dt,
                       double arrival, int* id);
 int fgx_server_run(fgx_server* s);      /* serves every submitted request    */
 int fgx_server_request(fgx_server* s, int id, fgx_request_info* out);
 int fgx_server_read_field(fgx_server* s, int id, int field, int dtype, void* host, size_t count);
+int fgx_server_stats_get(fgx_server* s, fgx_server_stats* out);
 int fgx_server_destroy(fgx_server* s);
 
 /* ---- profiling: average device time per kernel class ---------------------

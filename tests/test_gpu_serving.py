@@ -69,3 +69,37 @@ def test_serving_budget_serialises(gpu):
     for a, b in zip(infos, infos[1:]):
         assert b.admit >= a.finish - 1e-6
     srv.close()
+
+
+def test_continuous_batching_refills_slots(gpu):
+    """Eight staggered requests of two grids, at most two live batches with
+    a spare slot each: later requests take the slots that earlier ones
+    leave, so the requests run in fewer batches than there are requests, and
+    each still ends bit-identical to its solo run."""
+    shapes = [(32, 96), (48, 144), (32, 96), (32, 96), (48, 144), (32, 96), (48, 144), (32, 96)]
+    srv = Server(precision=F32, hbm_budget=1 << 30, max_batch_disks=2, snapshot_every=2, keep_final=True,
+                 max_live_batches=2, spare_slots=1)
+    loaded = []
+    for k, (nr, ns) in enumerate(shapes):
+        par, cfg = configs.plain_disk(nrad=nr, nsec=ns, disk_id=40 + k)
+        p, pl = configs.load(par, cfg, SIGMA0=6e-4 * (1 + 0.1 * k), ASPECTRATIO=0.05 + 0.002 * k)
+        n_dt = 2 + k % 3
+        loaded.append((p, pl, n_dt))
+        srv.submit(p, pl, n_dt, 0.04 * k)
+    srv.run()
+    st = srv.stats()
+    print("batches", st.batches, "refills", st.refills, "peak live", st.peak_live, "peak bytes", st.peak_bytes)
+    assert st.peak_live <= 2
+    assert st.batches < len(shapes) and st.refills >= 1
+    for rid, (p, pl, n_dt) in enumerate(loaded):
+        info = srv.request(rid)
+        assert info.status == 0 and info.admit >= info.arrival - 1e-6 and info.finish > info.admit
+        assert 0 <= info.batch < st.batches and 0 <= info.slot < 2
+        assert info.snapshots == (n_dt - 1) // 2
+        alone = Batch(p, pl, F32)
+        alone.step(n_dt)
+        assert alone.info().substeps == info.substeps
+        for name in ("dens", "vrad", "vtheta"):
+            assert np.array_equal(srv.field(rid, name), alone.field(name)), (rid, name)
+        alone.close()
+    srv.close()
