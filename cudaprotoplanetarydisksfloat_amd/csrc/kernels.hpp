@@ -137,13 +137,40 @@ __device__ __forceinline__ real dens_work(real sg) {
   if constexpr (sizeof(real) == 4) return sg >= 1e-12f ? 1.0f : sg * __builtin_amdgcn_rcpf(sg + 1e-20f);
   else return sg >= 3e-4 ? 1.0 : sg / (sg + 1e-20);
 }
-// Substep1's azimuthal pressure gradient (times dx_theta): the FP64 pressure
-// difference over the mean density; the quotient is rounded to the field
-// precision anyway, so FP32 forms it with a reciprocal
+// Substep1's radial velocity update (Kernels.cu:52-68): the acceleration
+// -gradP/Sigma - gradPhi + vt^2/Rinf is the ring profile's own, acc0 (formed
+// at set-up in extended precision, where the O(1) gravity and centrifugal
+// terms cancel), plus the change that the departures from the profiles make:
+//   dgp  = ((p_i - p_i-1) - gq0 (sr_i + sr_i-1)) 2 / (Sigma_i + Sigma_i-1) / dRmed
+//   dphi = (phi_i - phi_i-1) / dRmed
+//   sv   = the mean of the four vtheta residuals + Rinf (OmegaFrame - OmegaFrame at set-up)
+//   acc  = acc0 - dgp - dphi + sv (2 u0 + sv) / Rinf
+// with p the pressure departure, sr the density residual and gq0, u0 the
+// profile's pressure-gradient coefficient and interface velocity.  Every term
+// is a small departure, so the field precision carries it: FP32 forms it in
+// FP32 (the ring constants rounded once from their FP64 tables, the quotient
+// with a reciprocal), FP64 in FP64.  The potential difference is taken in
+// FP64.
 template <class real>
-__device__ __forceinline__ real azim_gradp(double p, double pl, real sig, real sigl) {
-  if constexpr (sizeof(real) == 4) return (real)(p - pl) * (real)2 * __builtin_amdgcn_rcpf(sig + sigl);
-  else return (real)((p - pl) * 2.0 / ((double)sig + (double)sigl));
+struct S1Ring { real cs2, idr, acc0, u0, gq0, invrinf, rinfdom; };
+template <class real>
+__device__ __forceinline__ S1Ring<real> s1_ring(const Dev<real>& D, int r, double dom) {
+  return S1Ring<real>{(real)D.cs2_d[r], (real)D.invdiffrmed_d[r], (real)D.acc0[r], (real)D.u0[r], (real)D.gq0[r],
+                      (real)D.invrinf_d[r], (real)(D.rinf_d[r] * dom)};
+}
+template <class real>
+__device__ __forceinline__ real substep1_vr(real vr, real p, real pm, real sr, real srm, real sg, real sgm, double phi,
+                                            double phim, real vsum, const S1Ring<real>& q, real dt) {
+  const real dgp = qdiv((p - pm - q.gq0 * (sr + srm)) * (real)2, sg + sgm) * q.idr;
+  const real dphi = (real)(phi - phim) * q.idr;
+  const real sv = vsum / (real)4 + q.rinfdom;
+  return vr + dt * (q.acc0 - dgp - dphi + sv * ((real)2 * q.u0 + sv) * q.invrinf);
+}
+// Substep1's azimuthal pressure gradient (times dx_theta): the difference of
+// the pressure departures over the mean density (FP32 with a reciprocal)
+template <class real>
+__device__ __forceinline__ real azim_gradp(real p, real pl, real sig, real sigl) {
+  return qdiv((p - pl) * (real)2, sig + sigl);
 }
 // block -> tile: the hardware hands blocks to the 8 XCDs round-robin (block
 // b to XCD b & 7).  Groups of FGX_XCD_GROUP consecutive tiles (neighbouring
@@ -324,13 +351,17 @@ __device__ __forceinline__ FieldRef<T> field_at(T* base, int soff, unsigned voff
 // evaluation; the star's -1/r is handled per ring (its radial difference is
 // part of the ring-profile acceleration acc0, its azimuthal difference
 // vanishes)
-// 1/sqrt(x) in FP64: the hardware estimate refined by two Newton steps
-// (relative error below 1e-16; the estimate alone is good to ~2^-29)
+// 1/sqrt(x) in FP64: the hardware estimate (good to ~2^-29) refined by
+// Newton steps: two for FP64 fields (relative error below 1e-16), one for
+// FP32 fields, whose potential differences need far less (below 2^-50)
+template <class real>
 __device__ __forceinline__ double rsqrt_d(double x) {
   double y = __builtin_amdgcn_rsq(x);
   double e = fma(-x * y, y, 1.0);
-  y = fma(0.5 * y, e, y);
-  e = fma(-x * y, y, 1.0);
+  if constexpr (sizeof(real) == 8) {
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+  }
   return fma(0.5 * y, e, y);
 }
 
@@ -341,7 +372,7 @@ __device__ __forceinline__ double potential_planets_xy(const Dev<real>& D, const
   for (int k = 0; k < c.nb; ++k) {
     const double xp = s.pot_x[k], yp = s.pot_y[k], mp = s.pot_m[k];
     const double dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
-    double p = -mp * rsqrt_d(dist + s.pot_sm2[k]);
+    double p = -mp * rsqrt_d<real>(dist + s.pot_sm2[k]);
     if (D.indirect) p += mp * s.pot_inv3[k] * (x * xp + y * yp);
     pot += p;
   }
@@ -981,41 +1012,38 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
   FGX_STRIP_BEGIN {
     const double dt = s.dt;
     const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-    // the pressure's departure from the ring profile's, in FP64 from the
-    // residual fields: (Sigma - S0) cs^2 with the exact isothermal cs^2 of the
-    // ring, or (gamma - 1)(e - EnergyMed)
+    // the pressure's departure from the ring profile's, in the field
+    // precision from the residual fields: (Sigma - S0) cs^2 with the ring's
+    // isothermal cs^2, or (gamma - 1)(e - EnergyMed)
     const double g1 = c.adiabaticindex - 1.0;
-    auto press = [&](int ii, int jj) -> double {
-      if (D.adiabatic) return g1 * (double)AT(D.energy, ii, jj);
-      return (double)AT(D.dens, ii, jj) * RT(cs2_d)[rb + ii];
+    const real g1r = (real)g1;
+    auto press = [&](int ii, int jj) -> real {
+      if (D.adiabatic) return g1r * AT(D.energy, ii, jj);
+      return AT(D.dens, ii, jj) * (real)RT(cs2_d)[rb + ii];
     };
     const real sr = AT(D.dens, i, j);
     const real sig = s_whole(D, rb, i, sr);
-    const double p = press(i, j);
+    const real p = press(i, j);
     const double phi = AT(D.phi, i, j);
-    if (D.write_pressure) AT(D.pressure, i, j) = (real)((double)RT(sigmamed)[rb + i] * RT(cs2_d)[rb + i] + p);
-    // radial: the ring profiles' own acceleration acc0 (formed at set-up in
-    // extended precision, where the O(1) gravity and centrifugal terms cancel)
-    // plus the change the departures from the profiles make
+    if (D.write_pressure)
+      AT(D.pressure, i, j) = (real)((double)RT(sigmamed)[rb + i] * RT(cs2_d)[rb + i] + (double)p);
+    // radial: the ring profiles' own acceleration acc0 plus the change the
+    // departures from the profiles make (substep1_vr)
     if (i > 0) {
       const real srm = AT(D.dens, i - 1, j);
       const real sigm = s_whole(D, rb, i - 1, srm);
-      const double pm = press(i - 1, j);
+      const real pm = press(i - 1, j);
       const double phim = AT(D.phi, i - 1, j);
-      const double idr = RT(invdiffrmed_d)[rb + i];
-      const double dgp = (p - pm - RT(gq0)[rb + i] * ((double)sr + (double)srm)) * 2.0 / ((double)sig + (double)sigm) * idr;
-      const double dphi = (phi - phim) * idr;
-      const double sv = ((double)AT(D.vtheta, i, j) + (double)AT(D.vtheta, i, jp) + (double)AT(D.vtheta, i - 1, j) +
-                         (double)AT(D.vtheta, i - 1, jp)) / 4.0 + RT(rinf_d)[rb + i] * (s.omega_frame - c.omega0);
-      const double acc = RT(acc0)[rb + i] - dgp - dphi + sv * (2.0 * RT(u0)[rb + i] + sv) * RT(invrinf_d)[rb + i];
-      AT(D.vradint, i, j) = (real)((double)AT(D.vrad, i, j) + dt * acc);
+      const real vsum = AT(D.vtheta, i, j) + AT(D.vtheta, i, jp) + AT(D.vtheta, i - 1, j) + AT(D.vtheta, i - 1, jp);
+      const S1Ring<real> q = s1_ring(D, rb + i, s.omega_frame - c.omega0);
+      AT(D.vradint, i, j) = substep1_vr((real)AT(D.vrad, i, j), p, pm, sr, srm, sig, sigm, phi, phim, vsum, q, (real)dt);
     } else {
       AT(D.vradint, i, j) = 0;
     }
     // azimuthal (the star's potential is the same at j and j-1)
     const real invdxt = (real)1 / RT(dxtheta)[rb + i];
     const real sigl = s_whole(D, rb, i, (real)AT(D.dens, i, jm));
-    const double pl = press(i, jm);
+    const real pl = press(i, jm);
     const double phil = AT(D.phi, i, jm);
     const real gradp = azim_gradp(p, pl, sig, sigl) * invdxt;
     const real gradphi = (real)(phi - phil) * invdxt;

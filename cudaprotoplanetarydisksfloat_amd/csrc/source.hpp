@@ -67,8 +67,12 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   // acc0, u0, gq0: the ring profile's own radial acceleration, interface
   // velocity and pressure-gradient coefficient (Substep1 below); S0 the density
   // profile of which the density field is a residual
+  // cr_: Substep1's departure terms are formed in the field precision, from
+  // the FP64 tables rounded once here: cs^2, 1/(Rmed_i - Rmed_i-1), acc0, u0,
+  // gq0, 1/Rinf and Rinf * (the frame's change since set-up)
   __shared__ double cd_[10][FGX_SRC_STAGE];  // cs2_d rmed_d invdiffrmed_d acc0 vref rinf_d invrinf_d invrmed_d u0 gq0
   __shared__ real cf_[11][FGX_SRC_STAGE];    // 1/dxtheta supp_torque invrmed invdiffrsup rsup rinf visc invrinf rmed invdiffrmed S0
+  __shared__ real cr_[7][FGX_SRC_STAGE];     // cs2 idr acc0 u0 gq0 invrinf rinf*dom
   const int rlo = i0 - 4;
   {
     const int r = rlo + lane;
@@ -80,6 +84,9 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
       cd_[8][lane] = RT(u0)[rb + r];
       cd_[9][lane] = RT(gq0)[rb + r];
       cf_[10][lane] = RT(sigmamed)[rb + r];
+      const S1Ring<real> q = s1_ring(D, rb + r, s.omega_frame - c.omega0);
+      cr_[0][lane] = q.cs2; cr_[1][lane] = q.idr; cr_[2][lane] = q.acc0; cr_[3][lane] = q.u0;
+      cr_[4][lane] = q.gq0; cr_[5][lane] = q.invrinf; cr_[6][lane] = q.rinfdom;
       cd_[4][lane] = RT(vref)[rb + r];
       cd_[5][lane] = RT(rinf_d)[rb + r];
       cd_[6][lane] = RT(invrinf_d)[rb + r];
@@ -99,10 +106,11 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   __syncthreads();
 #define FGX_CD(q, r) cd_[q][(r) - rlo]
 #define FGX_CF(q, r) cf_[q][(r) - rlo]
+#define FGX_CR(q, r) cr_[q][(r) - rlo]
 
   // ring k-1 inputs of stage A (sgA1 the whole density, srA1 its residual)
-  real sgA1 = 0, srA1 = 0, vtA1 = 0, vtpA1 = 0;
-  double pA1 = 0, phiA1 = 0;
+  real sgA1 = 0, srA1 = 0, vtA1 = 0, vtpA1 = 0, pA1 = 0;
+  double phiA1 = 0;
   // density and energy of rings k-1 .. k-4 (index 1..4), A's outputs of
   // rings k-1, k-2, the stresses of rings k-2, k-3, C's outputs of rings
   // k-3, k-4
@@ -131,14 +139,14 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     double pot = 0;
     if (nb > 0) {
       const double dist = (x - px0) * (x - px0) + (y - py0) * (y - py0);
-      double p = -pm0 * rsqrt_d(dist + ps0);
+      double p = -pm0 * rsqrt_d<real>(dist + ps0);
       if (indirect) p += pm0 * pi0 * (x * px0 + y * py0);
       pot += p;
     }
     for (int q = 1; q < nb; ++q) {
       const double xp = s.pot_x[q], yp = s.pot_y[q], mp = s.pot_m[q];
       const double dist = (x - xp) * (x - xp) + (y - yp) * (y - yp);
-      double p = -mp * rsqrt_d(dist + s.pot_sm2[q]);
+      double p = -mp * rsqrt_d<real>(dist + s.pot_sm2[q]);
       if (indirect) p += mp * s.pot_inv3[q] * (x * xp + y * yp);
       pot += p;
     }
@@ -147,9 +155,8 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   };
   // the pressure's departure from the ring profile's: (Sigma - S0) cs^2, or
   // (gamma - 1)(e - EnergyMed), from the residual fields
-  auto pressure = [&](real sr, real e, int k) -> double {
-    return ad ? g1 * (double)e : (double)sr * FGX_CD(0, k);
-  };
+  const real g1r = (real)g1;
+  auto pressure = [&](real sr, real e, int k) -> real { return ad ? g1r * e : sr * FGX_CR(0, k); };
   const int ks = (i0 - 2 > 0) ? i0 - 2 : 0;
   if (ks > 0) {   // prime stage A with ring ks-1
     const int k = ks - 1;
@@ -194,25 +201,19 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
       const real vr = cvr;
       const real vt = cvt, vtp = cvtp;
       if (ad) e0 = ce;
-      const double p = pressure(sr, e0, k);
+      const real p = pressure(sr, e0, k);
       const double phi = potential(FGX_CD(1, k) * cj, FGX_CD(1, k) * sj);
-      if (D.write_pressure && out && k >= i0 && k < i1) AT(D.pressure, k, j) = (real)((double)FGX_CF(10, k) * FGX_CD(0, k) + p);
+      if (D.write_pressure && out && k >= i0 && k < i1)
+        AT(D.pressure, k, j) = (real)((double)FGX_CF(10, k) * FGX_CD(0, k) + (double)p);
       if (k > 0) {
-        // Substep1's radial acceleration -gradP/Sigma - gradPhi + vt^2/Rinf,
-        // formed as the ring profile's own (acc0, exact, from the set-up) plus
-        // the change the departures from the profile make: the O(1) gravity
-        // and centrifugal terms then cancel in the set-up's extended
-        // precision instead of in the field arithmetic
-        const double idr = FGX_CD(2, k);
-        const double dgp = (p - pA1 - FGX_CD(9, k) * ((double)sr + (double)srA1)) * 2.0 / ((double)sg0 + (double)sgA1) * idr;
-        const double dphi = (phi - phiA1) * idr;
-        const double sv = ((double)vt + (double)vtp + (double)vtA1 + (double)vtpA1) / 4.0 + FGX_CD(5, k) * dom;
-        vri0 = (real)((double)vr + dtd * (FGX_CD(3, k) - dgp - dphi + sv * (2.0 * FGX_CD(8, k) + sv) * FGX_CD(6, k)));
+        const S1Ring<real> q{FGX_CR(0, k), FGX_CR(1, k), FGX_CR(2, k), FGX_CR(3, k), FGX_CR(4, k), FGX_CR(5, k),
+                             FGX_CR(6, k)};
+        vri0 = substep1_vr(vr, p, pA1, sr, srA1, sg0, sgA1, phi, phiA1, vt + vtp + vtA1 + vtpA1, q, dt);
       }
       // azimuthal (the star's potential is the same at j and j-1)
       const real invdxt = FGX_CF(0, k);
       const real sigl = lane_from_left(sg0);
-      const double pl = lane_from_left(p);
+      const real pl = lane_from_left(p);
       const double phil = lane_from_left(phi);
       const real gradp = azim_gradp(p, pl, sg0, sigl) * invdxt;
       const real gradphi = (real)(phi - phil) * invdxt;
@@ -324,5 +325,6 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
 }
 #undef FGX_CD
 #undef FGX_CF
+#undef FGX_CR
 
 }  // namespace fgx
