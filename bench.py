@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s batched vs plain at 1e-5 rel L2; %HBM roofline; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_GINSTR = 256 * 2.4   # wave64 VALU instructions per ns: one per CU per clock
 
 
 def _cells(batch):
@@ -117,24 +118,38 @@ def cpu_baseline(loaded, budget_s=20.0):
                 break
     dt = time.perf_counter() - t0
     return {"value": total / dt, "unit": "cell-updates/s", "cores": oracle.lib().orc_num_threads(),
-            "kind": "port", "sample": f"{ndisk} disk(s) of the workload, {ndt} DTs in all, "
-                                      f"{total} cell-updates in {dt:.1f} s"}
+            "kind": "port", "cpu_model": _cpu_model(),
+            "sample": f"{ndisk} disk(s) of the workload, {ndt} DTs in all, {total} cell-updates in {dt:.1f} s"}
+
+
+def _cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's processor string."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or None
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_batch64_fp32.txt")
 
 
 def measured_traffic(kernel, args):
-    """HBM bytes per launch of a kernel class, FETCH_SIZE + WRITE_SIZE from
-    the two rocprofv3 --pmc runs of this workload recorded in profiles/
-    (scaled by a 256 MiB copy kernel's counts); None for other workloads."""
+    """HBM bytes and VALU wave-instructions per launch of a kernel class:
+    FETCH_SIZE + WRITE_SIZE from the two rocprofv3 --pmc runs of this workload
+    recorded in profiles/ (scaled by a 256 MiB copy kernel's counts), and
+    SQ_INSTS_VALU from the SQ counter run; Nones for other workloads."""
     if args.workload != "batch" or args.precision != "f32" or not os.path.exists(TRAFFIC_FILE):
-        return None, None
+        return None, None, None
     for line in open(TRAFFIC_FILE):
         f = line.split()
-        if len(f) >= 5 and f[0] == kernel:
-            return int(float(f[3]) + float(f[4])), os.path.relpath(TRAFFIC_FILE, os.path.dirname(TRAFFIC_FILE) + "/..")
-    return None, None
+        if len(f) >= 6 and f[0] == kernel:
+            src = os.path.relpath(TRAFFIC_FILE, os.path.dirname(TRAFFIC_FILE) + "/..")
+            return int(float(f[3]) + float(f[4])), int(float(f[5])), src
+    return None, None, None
 
 
 def main():
@@ -201,10 +216,20 @@ def main():
     d = kern[dom]
     achieved = d["alg_bytes_per_launch"] / (d["avg_us"] * 1e-6) / 1e9
     sub_ms = sum(v["ms"] for v in kern.values())
-    traffic, traffic_src = measured_traffic(dom, args)
+    traffic, valu, traffic_src = measured_traffic(dom, args)
+    hbm_frac = achieved / HBM_PEAK_GBS
+    # the other roof of these kernels: VALU issue, one wave64 instruction per
+    # CU per clock (4 SIMDs, 4 clocks each), 256 CUs at 2.4 GHz
+    valu_issue = None
+    if valu:
+        rate = valu / (d["avg_us"] * 1e-6)
+        valu_issue = {"instr_per_launch": valu, "achieved": round(rate / 1e9, 1), "peak": VALU_PEAK_GINSTR,
+                      "unit": "G wave-instr/s", "frac": round(rate / 1e9 / VALU_PEAK_GINSTR, 4),
+                      "source": traffic_src}
+    binding = "valu_issue" if valu_issue and valu_issue["frac"] > hbm_frac else "hbm"
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": traffic_src,
+                "unit": "GB/s", "frac": round(hbm_frac, 4), "traffic": traffic,
+                "traffic_source": traffic_src, "valu_issue": valu_issue, "binding": binding,
                 "alg_bytes_per_launch": round(d["alg_bytes_per_launch"]),
                 "kernel_ms_per_step": round(sub_ms / args.steps, 3),
                 "avg_us": round(d["avg_us"], 2),
