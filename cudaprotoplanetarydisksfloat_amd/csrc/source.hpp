@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   real c0A1 = 0;   // column-0 density of ring k-1 (whole)
   if (ks > 0) c0A1 = FGX_CF(10, ks - 1) + (real)AT(D.dens, ks - 1, 0);
   fetch(ks);
-  for (int k = ks; k <= ke; ++k) {
+  auto ring_step = [&](const int k) {
     // ---------------------------------------------------------- stage A
     real sg0 = 0, e0 = 0, vri0 = 0, vti0 = 0;
     const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne, cc0 = nc0;
@@ -321,6 +321,12 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     vri2 = vri1; vri1 = vri0; vti2 = vti1; vti1 = vti0;
     trr2 = trr1; trr1 = trr0; tpp2 = tpp1; tpp1 = tpp0; trp1 = trp0;
     vrc2 = vrc1; vrc1 = vrc0; vtc1 = vtc0;
+  };
+  // two rings per trip, so that the register shifts of period two cost no
+  // moves; an odd count runs one more ring past ke, which stores nothing
+  for (int k = ks; k <= ke; k += 2) {
+    ring_step(k);
+    ring_step(k + 1);
   }
 }
 #undef FGX_CD

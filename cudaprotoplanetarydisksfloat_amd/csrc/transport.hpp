@@ -69,10 +69,25 @@ struct RadRing {
                   // density * (vtheta residual) * R, l0 = (vref + R*Omega) * R
 };
 
-// the disk's fields seen from its first element; offsets in 32 bits
+// a disk's field as a buffer resource (scalar registers): a ring's offset
+// goes into the scalar offset and the sector's into a 32-bit vector offset
+// fixed for the whole walk, so a ring's loads and stores cost no vector
+// address arithmetic.  Offsets are bytes; a disk's field is far below 2 GiB.
+template <class T>
+struct FieldBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ T ld(int soff, unsigned voff) const { return FieldRef<T>{r, soff, voff}.get(); }
+  __device__ __forceinline__ void st(int soff, unsigned voff, T v) const { FieldRef<T>{r, soff, voff}.put(v); }
+};
+template <class T>
+__device__ __forceinline__ FieldBuf<T> field_buf(const T* base) {
+  return FieldBuf<T>{__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, 0x7fffffff, FGX_BUF_WORD3)};
+}
+
+// the disk's fields seen from its first element
 template <class real>
 struct RadFields {
-  const real *dens, *vrad, *vtheta, *energy, *label;
+  FieldBuf<real> dens, vrad, vtheta, energy, label;
   const FGX_AS4 double *rmed_d, *vref;
   const FGX_AS4 real* emed;   // ring profile the stored energy is a residual of
   const FGX_AS4 real* smed;   // S0, the ring profile the stored density is a residual of
@@ -84,18 +99,19 @@ template <class real>
 struct RadRaw {
   real sg, vr0, vr1, vtp, vtj, e, lab;
 };
+// vj, vjp: the byte offsets of sectors j and j+1 (cyclic) within a ring
 template <class real, bool LAB = false>
-__device__ __forceinline__ void rad_fetch(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, bool ad,
-                                          RadRaw<real>& w) {
+__device__ __forceinline__ void rad_fetch(const RadFields<real>& f, int ns, int nr, int k, unsigned vj, unsigned vjp,
+                                          bool ad, RadRaw<real>& w) {
   k = k < 0 ? 0 : (k > nr - 1 ? nr - 1 : k);
-  const int o = k * ns;
-  w.sg = f.dens[o + j];
-  w.vr0 = f.vrad[o + j];
-  w.vr1 = f.vrad[o + ns + j];
-  w.vtp = f.vtheta[o + jp];
-  w.vtj = f.vtheta[o + j];
-  w.e = ad ? f.energy[o + j] : (real)0;
-  w.lab = LAB ? f.label[o + j] : (real)0;
+  const int o = k * ns * (int)sizeof(real), o1 = o + ns * (int)sizeof(real);
+  w.sg = f.dens.ld(o, vj);
+  w.vr0 = f.vrad.ld(o, vj);
+  w.vr1 = f.vrad.ld(o1, vj);
+  w.vtp = f.vtheta.ld(o, vjp);
+  w.vtj = f.vtheta.ld(o, vj);
+  w.e = ad ? f.energy.ld(o, vj) : (real)0;
+  w.lab = LAB ? f.label.ld(o, vj) : (real)0;
 }
 
 template <class real, bool LAB = false>
@@ -139,10 +155,10 @@ __device__ __forceinline__ void rad_derive(const RadFields<real>& f, int nr, int
 }
 
 template <class real, bool LAB = false>
-__device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int nr, int k, int j, int jp, double om,
-                                         bool ad, RadRing<real, LAB>& r) {
+__device__ __forceinline__ void rad_load(const RadFields<real>& f, int ns, int nr, int k, unsigned vj, unsigned vjp,
+                                         double om, bool ad, RadRing<real, LAB>& r) {
   RadRaw<real> w;
-  rad_fetch<real, LAB>(f, ns, nr, k, j, jp, ad, w);
+  rad_fetch<real, LAB>(f, ns, nr, k, vj, vjp, ad, w);
   rad_derive<real, LAB>(f, nr, k, om, ad, w, r);
 }
 
@@ -179,16 +195,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   const auto idr = RT(invdiffrmed) + rb;
   // quantities in use: 0..5 always, 6 (energy) if adiabatic, 7 (label) if LAB
   auto qon = [&](int q) { return q < 6 || q == 7 || ad; };
-  const RadFields<real> F{D.dens + fb, D.vrad + fb, D.vtheta + fb, D.energy + (ad ? fb : 0),
-                          LAB ? D.label + fb : nullptr, RT(rmed_d) + rb, RT(vref) + rb, RT(energymed) + rb,
-                          RT(sigmamed) + rb};
-  real* const o_dens = D.t_dens + fb;
-  real* const o_rmp = D.t_rmp + fb;
-  real* const o_rmm = D.t_rmm + fb;
-  real* const o_tmp = D.t_tmp + fb;
-  real* const o_tmm = D.t_tmm + fb;
-  real* const o_e = D.t_e + (ad ? fb : 0);
-  real* const o_lab = LAB ? D.t_lab + fb : nullptr;
+  const RadFields<real> F{field_buf(D.dens + fb), field_buf(D.vrad + fb), field_buf(D.vtheta + fb),
+                          field_buf(D.energy + (ad ? fb : 0)), field_buf(LAB ? D.label + fb : D.dens + fb),
+                          RT(rmed_d) + rb, RT(vref) + rb, RT(energymed) + rb, RT(sigmamed) + rb};
+  const FieldBuf<real> o_dens = field_buf(D.t_dens + fb);
+  const FieldBuf<real> o_rmp = field_buf(D.t_rmp + fb);
+  const FieldBuf<real> o_rmm = field_buf(D.t_rmm + fb);
+  const FieldBuf<real> o_tmp = field_buf(D.t_tmp + fb);
+  const FieldBuf<real> o_tmm = field_buf(D.t_tmm + fb);
+  const FieldBuf<real> o_e = field_buf(D.t_e + (ad ? fb : 0));
+  const FieldBuf<real> o_lab = field_buf(LAB ? D.t_lab + fb : D.t_dens + fb);
+  const unsigned vj = (unsigned)j * (unsigned)sizeof(real), vjp = (unsigned)jp * (unsigned)sizeof(real);
 
   using Ring = RadRing<real, LAB>;
   // a ring's slope coefficients, read once for all quantities (read inside
@@ -235,8 +252,8 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   // a strip begins (a separate prologue rounds them differently once the
   // compiler fuses its products into the flux arithmetic).
   Ring ra, rb1;
-  rad_load<real, LAB>(F, ns, nr, i0 - 2, j, jp, om, ad, ra);
-  rad_load<real, LAB>(F, ns, nr, i0 - 1, j, jp, om, ad, rb1);
+  rad_load<real, LAB>(F, ns, nr, i0 - 2, vj, vjp, om, ad, ra);
+  rad_load<real, LAB>(F, ns, nr, i0 - 1, vj, vjp, om, ad, rb1);
   real slA[NQ], flux[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) slA[q] = flux[q] = (real)0;
@@ -245,12 +262,12 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
   // i-2 and i-1: the vector memory counter retires loads and stores in issue
   // order, and four rings are in flight when the walk starts
   RadRaw<real> nxt, nxt2;
-  rad_fetch<real, LAB>(F, ns, nr, i0, j, jp, ad, nxt);
-  rad_fetch<real, LAB>(F, ns, nr, i0 + 1, j, jp, ad, nxt2);
+  rad_fetch<real, LAB>(F, ns, nr, i0, vj, vjp, ad, nxt);
+  rad_fetch<real, LAB>(F, ns, nr, i0 + 1, vj, vjp, ad, nxt2);
   for (int i = i0 - 2; i < i1; ++i) {
     const RadRaw<real> cur = nxt;
     nxt = nxt2;
-    if (i + 2 < i1) rad_fetch<real, LAB>(F, ns, nr, i + 4, j, jp, ad, nxt2);
+    if (i + 2 < i1) rad_fetch<real, LAB>(F, ns, nr, i + 4, vj, vjp, ad, nxt2);
     Ring rc;
     rad_derive<real, LAB>(F, nr, i + 2, om, ad, cur, rc);
     // slope of ring i+1, flux through interface i+1 (zero at the outer edge)
@@ -259,17 +276,17 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) slB[q] = qon(q) ? slope_at(gb, ra, rb1, rc, q) : (real)0;
     iface(i, ra, rb1, slA, slB, fnew);
-    if (i >= i0) {
+    if (i >= i0 && i < i1) {
       const real is = isurf[i];
       const double rd = F.rmed_d[i], vref_i = F.vref[i];
       // cell i: quantity + (flux in - flux out) / Surf
       const real inc = (flux[1] - fnew[1]) * is;
       const real sg = ra.sg;
-      const int oc = i * ns + j;
-      o_dens[oc] = ra.sr + inc;   // a residual of S0, as the field
+      const int oc = i * ns * (int)sizeof(real);
+      o_dens.st(oc, vj, ra.sr + inc);   // a residual of S0, as the field
       if (i == 0 && D.open_inner) D.lost[c.so + j] = flux[1] - fnew[1];
-      o_rmp[oc] = sg * ra.vr1 + (flux[2] - fnew[2]) * is;
-      o_rmm[oc] = sg * ra.vr0 + (flux[3] - fnew[3]) * is;
+      o_rmp.st(oc, vj, sg * ra.vr1 + (flux[2] - fnew[2]) * is);
+      o_rmm.st(oc, vj, sg * ra.vr0 + (flux[3] - fnew[3]) * is);
       // angular momenta leave as residuals of density * l0(ring), with the
       // density taken before its field rounding, so the azimuthal velocity
       // rebuilt from them carries no trace of that rounding:
@@ -278,10 +295,10 @@ __global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const 
       // Sigma l0 would each carry a rounding of the size of a vtheta ulp)
       const double l0i = (vref_i + rd * om) * rd;
       const double dsl = (double)inc * l0i;
-      o_tmp[oc] = (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - dsl);
-      o_tmm[oc] = (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - dsl);
-      if (ad) o_e[oc] = ra.e + (flux[6] - fnew[6]) * is;
-      if constexpr (LAB) o_lab[oc] = ra.x + (flux[7] - fnew[7]) * is;
+      o_tmp.st(oc, vj, (real)(ra.tp + (double)((flux[4] - fnew[4]) * is) - dsl));
+      o_tmm.st(oc, vj, (real)(ra.tm + (double)((flux[5] - fnew[5]) * is) - dsl));
+      if (ad) o_e.st(oc, vj, ra.e + (flux[6] - fnew[6]) * is);
+      if constexpr (LAB) o_lab.st(oc, vj, ra.x + (flux[7] - fnew[7]) * is);
     }
     // slide the window
 #pragma unroll
