@@ -165,13 +165,18 @@ def main():
                     help="requests of the serving phase (p50/p99 completion time); 0 skips it")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run with the engine's own tiling instead of the autotuner's choice")
+    ap.add_argument("--tiling-cache", default=None,
+                    help="the autotuner's record file: a matching record is used without timing")
+    ap.add_argument("--tune-only", action="store_true", help="run the autotuner, print its report and stop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); disks shard across GPUs "
               f"by disk, which this build does not launch", file=sys.stderr)
         sys.exit(2)
     import torch
-    from cudaprotoplanetarydisksfloat_amd import F32, F64, Batch
+    from cudaprotoplanetarydisksfloat_amd import F32, F64, Batch, autotune
 
     prec = F32 if args.precision == "f32" else F64
     if args.copy_mib:
@@ -185,7 +190,19 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     loaded, desc = _workload(args.workload)
-    batch = Batch([p for p, _ in loaded], [pl for _, pl in loaded], prec)
+    # the tile shapes: the autotuner times candidate tilings on scratch
+    # batches of the same disks (outside the timed region; any tiling gives
+    # the same fields), or reads its record from --tiling-cache
+    tiling, tuner = None, None
+    if not args.no_tune:
+        tiling, rep = autotune([p for p, _ in loaded], [pl for _, pl in loaded], prec, n_dt=2,
+                               cache=args.tiling_cache)
+        tuner = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.as_dict().items()}
+        if args.tune_only:
+            print(json.dumps({"tiling": tiling.as_dict(), "tuner": tuner}))
+            return
+    batch = Batch([p for p, _ in loaded], [pl for _, pl in loaded], prec, tiling=tiling)
+    desc = dict(desc, tiling=batch.tiling().as_dict())
     batch.step(args.warmup)
     torch.cuda.synchronize()
     before = _substeps(batch)
@@ -262,7 +279,7 @@ def main():
             "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
             "config": dict(desc, cell_updates=updates, cells=ncell), "roofline": roofline,
-            "cpu_baseline": cpu, "plain": plain, "serving": serve}
+            "cpu_baseline": cpu, "plain": plain, "serving": serve, "tuner": tuner}
     print(json.dumps(line))
 
 

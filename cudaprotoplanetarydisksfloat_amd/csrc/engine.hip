@@ -92,6 +92,7 @@ struct BatchBase {
   virtual int reset_disk(int disk, const fgx_params& p, const fgx_planets& pl) = 0;
   virtual int snapshot_queue(int disk) = 0;
   virtual int snapshot_wait(int disk, int* had) = 0;
+  virtual int tiling(fgx_tiling* out) const = 0;
 };
 
 // checkpoint file: header, per-disk state record, ring profiles carried from
@@ -451,7 +452,10 @@ struct Batch : BatchBase {
   }
 
   // ------------------------------------------------------------------ create
-  int create(const std::vector<fgx_params>& ps, const std::vector<fgx_planets>& pls) {
+  // the tiling the batch was created with (every field resolved)
+  fgx_tiling tiling_used{};
+  int tiling(fgx_tiling* out) const override { *out = tiling_used; return FGX_OK; }
+  int create(const std::vector<fgx_params>& ps, const std::vector<fgx_planets>& pls, const fgx_tiling* tl) {
     nd = (int)ps.size();
     if (nd < 1) return set_error(FGX_EPARAM, "empty batch");
     const fgx_params& P0 = ps[0];
@@ -536,7 +540,9 @@ struct Batch : BatchBase {
         tiles16 += std::ceil(hd[d].nrad / 16.0) * std::ceil(hd[d].nsec / (double)FGX_TPB);
       strip_rings = 16;
       while (strip_rings > 2 && tiles16 * 16.0 / strip_rings < 2048) strip_rings /= 2;
-      if (const char* e = std::getenv("FGX_STRIP_RINGS")) strip_rings = std::min(std::max(std::atoi(e), 2), 64);
+      if (tl && tl->strip_rings > 0) strip_rings = tl->strip_rings;
+      else if (const char* e = std::getenv("FGX_STRIP_RINGS")) strip_rings = std::atoi(e);
+      strip_rings = std::min(std::max(strip_rings, 2), 64);
     }
     std::vector<CellTile> st;
     std::vector<std::vector<RingTile>> tg(17 * 17);   // index K*17 + W (K = 0: shared-memory kernel)
@@ -554,7 +560,8 @@ struct Batch : BatchBase {
     // fused source pass: on by default, FGX_FUSED_SOURCE=0 keeps the
     // separate kernels; FGX_SOURCE_RINGS sets the strip height
     if (const char* e = std::getenv("FGX_FUSED_SOURCE")) fused_source = std::atoi(e) != 0;
-    if (const char* e = std::getenv("FGX_SOURCE_RINGS")) src_rings = std::atoi(e);
+    if (tl && tl->source_rings > 0) src_rings = tl->source_rings;
+    else if (const char* e = std::getenv("FGX_SOURCE_RINGS")) src_rings = std::atoi(e);
     src_rings = std::min(std::max(src_rings, 1), FGX_SRC_RINGS_MAX);   // k_source stages rs + 8 rings
     std::vector<CellTile> srt;
     for (int d = 0; d < nd; ++d)
@@ -563,9 +570,13 @@ struct Batch : BatchBase {
     while (srt.size() % FGX_TILE_PAD) srt.push_back(CellTile{-1, 0, 0, 0});
     n_srctiles = (int)srt.size();
     // the fine tiling: 2-ring strips, 4-ring source strips
-    int fine_src = std::min(4, src_rings);
-    if (const char* e = std::getenv("FGX_FINE_SOURCE_RINGS")) fine_src = std::min(std::max(std::atoi(e), 1), src_rings);
-    const int fine_strip = std::min(2, strip_rings);
+    int fine_src = 4;
+    if (tl && tl->fine_source_rings > 0) fine_src = tl->fine_source_rings;
+    else if (const char* e = std::getenv("FGX_FINE_SOURCE_RINGS")) fine_src = std::atoi(e);
+    fine_src = std::min(std::max(fine_src, 1), src_rings);
+    int fine_strip = 2;
+    if (tl && tl->fine_strip_rings > 0) fine_strip = tl->fine_strip_rings;
+    fine_strip = std::min(std::max(fine_strip, 2), strip_rings);
     std::vector<CellTile> stf, srtf;
     for (int d = 0; d < nd; ++d) {
       total_cells += (long long)hd[d].nrad * hd[d].nsec;
@@ -576,7 +587,9 @@ struct Batch : BatchBase {
     }
     while (stf.size() % FGX_TILE_PAD) stf.push_back(CellTile{-1, 0, 0, 0});
     while (srtf.size() % FGX_TILE_PAD) srtf.push_back(CellTile{-1, 0, 0, 0});
-    if (const char* e = std::getenv("FGX_FINE_CELLS")) fine_cells = std::atof(e);
+    if (tl && tl->fine_cells != 0.0) fine_cells = tl->fine_cells;
+    else if (const char* e = std::getenv("FGX_FINE_CELLS")) fine_cells = std::atof(e);
+    tiling_used = fgx_tiling{strip_rings, src_rings, fine_src, fine_strip, fine_cells};
     while (ct.size() % FGX_TILE_PAD) ct.push_back(CellTile{-1, 0, 0, 0});
     while (rt.size() % FGX_TILE_PAD) rt.push_back(RingTile{-1, 0});
     n_ctiles = (int)ct.size();
@@ -1602,7 +1615,7 @@ struct fgx_batch {
 using fgx::set_error;
 
 static int make_batch(const std::vector<fgx_params>& ps, const std::vector<fgx_planets>& pls, int precision,
-                      fgx_batch** out) {
+                      fgx_batch** out, const fgx_tiling* tl = nullptr) {
   if (!out) return set_error(FGX_EPARAM, "null output handle");
   *out = nullptr;
   int ndev = 0;
@@ -1613,11 +1626,11 @@ static int make_batch(const std::vector<fgx_params>& ps, const std::vector<fgx_p
   if (precision == FGX_F64) {
     auto* impl = new fgx::Batch<double>();
     b->impl.reset(impl);
-    r = impl->create(ps, pls);
+    r = impl->create(ps, pls, tl);
   } else if (precision == FGX_F32) {
     auto* impl = new fgx::Batch<float>();
     b->impl.reset(impl);
-    r = impl->create(ps, pls);
+    r = impl->create(ps, pls, tl);
   } else {
     return set_error(FGX_EPARAM, "precision must be FGX_F32 or FGX_F64");
   }
@@ -1635,6 +1648,15 @@ int fgx_disk_create(const fgx_params* p, const fgx_planets* pl, int precision, f
 int fgx_batch_create(int n, const fgx_params* p, const fgx_planets* pl, int precision, fgx_batch** out) {
   if (n < 1 || !p || !pl) return set_error(FGX_EPARAM, "bad batch arguments");
   return make_batch(std::vector<fgx_params>(p, p + n), std::vector<fgx_planets>(pl, pl + n), precision, out);
+}
+int fgx_batch_create_tiled(int n, const fgx_params* p, const fgx_planets* pl, int precision, const fgx_tiling* t,
+                           fgx_batch** out) {
+  if (n < 1 || !p || !pl) return set_error(FGX_EPARAM, "bad batch arguments");
+  return make_batch(std::vector<fgx_params>(p, p + n), std::vector<fgx_planets>(pl, pl + n), precision, out, t);
+}
+int fgx_batch_tiling(fgx_batch* b, fgx_tiling* out) {
+  if (!b || !out) return set_error(FGX_EPARAM, "null argument");
+  return b->impl->tiling(out);
 }
 int fgx_disk_step(fgx_disk* d, int n) {
   if (!d) return set_error(FGX_EPARAM, "null handle");
@@ -1754,4 +1776,12 @@ const HostDisk* batch_host(fgx_batch* b, int d) { return &b->impl->host(d); }
 int batch_mdcp(fgx_batch* b, int d, double* out) { return b->impl->mdcp(d, out); }
 int batch_snapshot_queue(fgx_batch* b, int d) { return b->impl->snapshot_queue(d); }
 int batch_snapshot_wait(fgx_batch* b, int d, int* had) { return b->impl->snapshot_wait(d, had); }
+int device_name(char* out, size_t n) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return set_error(FGX_EDEVICE, "no HIP device available");
+  std::snprintf(out, n, "%s:%s:%d", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return FGX_OK;
+}
 }  // namespace fgx

@@ -73,6 +73,8 @@ int batch_mdcp(fgx_batch* b, int d, double* out);
 // pinned host buffer, and wait for the last one queued
 int batch_snapshot_queue(fgx_batch* b, int d);
 int batch_snapshot_wait(fgx_batch* b, int d, int* had);
+// the name of the current HIP device (the autotuner keys its records on it)
+int device_name(char* out, size_t n);
 
 }  // namespace fgx
 

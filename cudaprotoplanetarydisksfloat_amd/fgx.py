@@ -73,6 +73,29 @@ class DiskInfo(ctypes.Structure):
                 ("dt_steps", ctypes.c_longlong), ("health", ctypes.c_int)]
 
 
+class Tiling(ctypes.Structure):
+    """fgx_tiling: strip heights of the batch kernels and the short-strip
+    threshold (0 = the engine's choice)."""
+    _fields_ = [("strip_rings", ctypes.c_int), ("source_rings", ctypes.c_int),
+                ("fine_source_rings", ctypes.c_int), ("fine_strip_rings", ctypes.c_int),
+                ("fine_cells", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneReport(ctypes.Structure):
+    """fgx_tune_report: what the autotuner timed and chose."""
+    _fields_ = [("from_cache", ctypes.c_int), ("candidates", ctypes.c_int),
+                ("default_rate", ctypes.c_double), ("best_rate", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("signature", ctypes.c_char * 128)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["signature"] = self.signature.decode()
+        return d
+
+
 class ServerConfig(ctypes.Structure):
     """fgx_server_config (include/fgx.h)."""
     _fields_ = [("precision", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
@@ -107,6 +130,11 @@ _sigs = {
     "fgx_planets_parse": ([_cp, _vp, ctypes.POINTER(Planets)], _i),
     "fgx_disk_create": ([_vp, ctypes.POINTER(Planets), _i, ctypes.POINTER(_vp)], _i),
     "fgx_batch_create": ([_i, _vp, ctypes.POINTER(Planets), _i, ctypes.POINTER(_vp)], _i),
+    "fgx_batch_create_tiled": ([_i, _vp, ctypes.POINTER(Planets), _i, ctypes.POINTER(Tiling),
+                                ctypes.POINTER(_vp)], _i),
+    "fgx_batch_tiling": ([_vp, ctypes.POINTER(Tiling)], _i),
+    "fgx_autotune": ([_i, _vp, ctypes.POINTER(Planets), _i, _i, _cp, ctypes.POINTER(Tiling),
+                      ctypes.POINTER(TuneReport)], _i),
     "fgx_disk_step": ([_vp, _i], _i), "fgx_batch_step": ([_vp, _i], _i),
     "fgx_disk_destroy": ([_vp], _i), "fgx_batch_destroy": ([_vp], _i),
     "fgx_batch_size": ([_vp, ctypes.POINTER(_i)], _i),
@@ -214,20 +242,43 @@ def planets_from_file(path, params):
     return pl
 
 
-class Batch:
-    """N disks advanced together; a single disk is a batch of one."""
+def _disk_arrays(params, planets):
+    n = len(params)
+    sz = lib.fgx_sizeof_params()
+    arr = ctypes.create_string_buffer(sz * n)
+    for k, p in enumerate(params):
+        ctypes.memmove(ctypes.addressof(arr) + k * sz, p.buf, sz)
+    return n, arr, (Planets * n)(*planets)
 
-    def __init__(self, params, planets, precision=F32):
+
+def autotune(params, planets, precision=F32, n_dt=2, cache=None):
+    """fgx_autotune: the tiling under which these disks advance fastest, and
+    the tuner's report (a cache file hit times nothing)."""
+    if isinstance(params, Params):
+        params, planets = [params], [planets]
+    n, arr, pls = _disk_arrays(params, planets)
+    t, rep = Tiling(), TuneReport()
+    _check(lib.fgx_autotune(n, arr, pls, precision, n_dt, cache.encode() if cache else None,
+                            ctypes.byref(t), ctypes.byref(rep)))
+    return t, rep
+
+
+class Batch:
+    """N disks advanced together; a single disk is a batch of one.
+    tiling: a Tiling (or dict of its fields) for the batch kernels, None for
+    the engine's own choice."""
+
+    def __init__(self, params, planets, precision=F32, tiling=None):
         if isinstance(params, Params):
             params, planets = [params], [planets]
-        n = len(params)
-        sz = lib.fgx_sizeof_params()
-        arr = ctypes.create_string_buffer(sz * n)
-        for k, p in enumerate(params):
-            ctypes.memmove(ctypes.addressof(arr) + k * sz, p.buf, sz)
-        pls = (Planets * n)(*planets)
+        n, arr, pls = _disk_arrays(params, planets)
         h = ctypes.c_void_p()
-        _check(lib.fgx_batch_create(n, arr, pls, precision, ctypes.byref(h)))
+        if tiling is None:
+            _check(lib.fgx_batch_create(n, arr, pls, precision, ctypes.byref(h)))
+        else:
+            if isinstance(tiling, dict):
+                tiling = Tiling(**tiling)
+            _check(lib.fgx_batch_create_tiled(n, arr, pls, precision, ctypes.byref(tiling), ctypes.byref(h)))
         self.h = h
         self.n = n
         self.precision = precision
@@ -247,6 +298,12 @@ class Batch:
 
     def step(self, n_dt=1):
         _check(lib.fgx_batch_step(self.h, n_dt))
+
+    def tiling(self):
+        """The tiling the batch runs with, every field resolved."""
+        t = Tiling()
+        _check(lib.fgx_batch_tiling(self.h, ctypes.byref(t)))
+        return t
 
     def freeze(self, disk, frozen=True):
         """Skip (or resume) one disk in every later DT."""

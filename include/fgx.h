@@ -265,6 +265,46 @@ int fgx_batch_reset_disk(fgx_batch* b, int disk, const fgx_params* p, const fgx_
 /* bytes of device memory a disk of these parameters needs */
 int fgx_footprint_bytes(const fgx_params* p, int precision, size_t* bytes);
 
+/* ---- tile shapes and their autotuner (no counterpart in the reference, whose
+ * kernels launch one thread per cell, src/SourceEuler.cu:208-301)
+ * The batch kernels walk strips of rings; how tall the strips are, and below
+ * how many active cells a substep switches to the short-strip tiling, set how
+ * well a batch fills the device.  The fields do not depend on the tiling: any
+ * tiling gives the same bits.  A zero field means the engine's own choice for
+ * the batch (and then an FGX_STRIP_RINGS / FGX_SOURCE_RINGS /
+ * FGX_FINE_SOURCE_RINGS / FGX_FINE_CELLS environment variable, if set). */
+typedef struct fgx_tiling {
+  int    strip_rings;        /* radial sweep strips: rings per strip, 2..64       */
+  int    source_rings;       /* fused source pass strips: rings, 1..24            */
+  int    fine_source_rings;  /* source strips of the short-strip tiling, 1..source_rings */
+  int    fine_strip_rings;   /* radial strips of the short-strip tiling, 1..strip_rings */
+  double fine_cells;         /* short strips below this many active cells; <0: never */
+} fgx_tiling;
+typedef struct fgx_tune_report {
+  int    from_cache;     /* 1: the tiling came from the cache file, nothing was timed */
+  int    candidates;     /* tilings timed                                        */
+  double default_rate;   /* cell-updates/s of the engine's own tiling            */
+  double best_rate;      /* cell-updates/s of the tiling returned                 */
+  double seconds;        /* wall time the tuning took                            */
+  char   signature[128]; /* the workload key the choice is recorded under        */
+} fgx_tune_report;
+/* fgx_batch_create with a tiling (NULL: the engine's choice) */
+int fgx_batch_create_tiled(int n, const fgx_params* p, const fgx_planets* pl, int precision,
+                           const fgx_tiling* t, fgx_batch** out);
+/* the tiling a batch runs with, every field resolved */
+int fgx_batch_tiling(fgx_batch* b, fgx_tiling* out);
+/* Autotuner: the tiling under which this batch of disks advances fastest.
+ * Scratch batches of the same disks are created with candidate tilings (a
+ * coordinate search from the engine's own choice over the four strip heights
+ * and the short-strip threshold) and timed over n_dt DTs after one warm-up DT;
+ * a candidate replaces the incumbent only when it is more than 1% faster.
+ * The choice is recorded in cache_path (one line per workload signature:
+ * precision, disk shapes, mode flags and the device name); a later call with a
+ * matching signature reads it back without timing anything.  cache_path may be
+ * NULL (no cache).  Needs device memory for one scratch batch at a time. */
+int fgx_autotune(int n, const fgx_params* p, const fgx_planets* pl, int precision, int n_dt,
+                 const char* cache_path, fgx_tiling* out, fgx_tune_report* report);
+
 /* ---- serving loop (a stream of run requests; no counterpart in the reference)
  * Requests (one disk each: parameters, planets, number of DTs, arrival time
  * in seconds from the start of fgx_server_run) are served by continuous

@@ -43,3 +43,17 @@ def test_create_without_device_fails_cleanly():
     with pytest.raises(FgxError) as e:
         Batch(p, pl)
     assert e.value.code == -2
+
+
+def test_autotune_without_device_fails_cleanly(tmp_path):
+    from cudaprotoplanetarydisksfloat_amd import FgxError, autotune, configs, device_count
+    if device_count() > 0:
+        pytest.skip("a device is present")
+    p, pl = configs.load(*configs.plain_disk(nrad=16, nsec=32))
+    cache = tmp_path / "tiling.txt"
+    with pytest.raises(FgxError) as e:
+        autotune([p], [pl], cache=str(cache))
+    assert e.value.code == -2
+    assert not cache.exists()   # nothing recorded for a failed tuning
+    with pytest.raises(FgxError):
+        autotune([p], [pl], n_dt=0)
