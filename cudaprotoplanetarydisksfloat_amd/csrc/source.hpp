@@ -174,12 +174,15 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   // the density of column 0 of the ring (the corner density of the
   // reference's TAURP, used by stage B one ring later): a load used in the
   // iteration that issues it would make the wave wait for every load in
-  // flight, since loads retire in issue order.  Rings past the outer edge
-  // load the last ring (their values are never used): a fixed number of
-  // loads per iteration lets the compiler wait for exactly the ring it needs.
+  // flight, since loads retire in issue order.  Rings past the last one
+  // stage A reads (i1+1, or the outer edge) load that ring again, which the
+  // caches still hold (their values are never used): a fixed number of loads
+  // per iteration lets the compiler wait for exactly the ring it needs,
+  // without fetching rings no stage reads.
   real nsg, nvr, nvt, nvtp, ne, nc0;
+  const int klast = (i1 + 1 < nr) ? i1 + 1 : nr - 1;
   auto fetch = [&](int k) {
-    const int kk = k < nr ? k : nr - 1;
+    const int kk = k < klast ? k : klast;
     nsg = AT(D.dens, kk, j);
     nvr = AT(D.vrad, kk, j);
     nvt = AT(D.vtheta, kk, j);
