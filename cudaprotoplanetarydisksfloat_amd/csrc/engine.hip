@@ -1033,12 +1033,20 @@ struct Batch : BatchBase {
     if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_stiles, FGX_TPB, 0, D));
     if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
     LCHK(boundary());
-    if (fused_source && !D.self_gravity) {
+    if (fused_source) {
       // the viscosity profile only depends on the sound speed, which the
       // source stages do not change, so it can be formed first
       if (D.viscous && visc_dynamic)
         LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
-      LCHK(launch("source", ad ? 7 : 5, k_source<real>, n_srctiles, 64, 0, D, srctiles, src_rings));
+      if (D.self_gravity) {
+        // the self-gravity field only depends on the density, which the
+        // source stages read but do not change; the pass adds it after
+        // Substep1, reading two more fields
+        LCHK(self_gravity());
+        LCHK(launch("source", ad ? 9 : 7, k_source<real, true>, n_srctiles, 64, 0, D, srctiles, src_rings));
+      } else {
+        LCHK(launch("source", ad ? 7 : 5, k_source<real, false>, n_srctiles, 64, 0, D, srctiles, src_rings));
+      }
       // the new velocities were written to the VradInt / VthetaInt arrays
       std::swap(D.vrad, D.vradint);
       std::swap(D.vtheta, D.vthetaint);
@@ -1125,7 +1133,7 @@ struct Batch : BatchBase {
       LCHK(launch("finish_dt", 0, k_finish_dt<real>, nd, 64, 0, D));
       if (covered < nd) lists_full();
       use_tiles(false);
-      if (fused_source && !D.self_gravity) LCHK(launch("vel_fixup", 0, k_vel_fixup<real>, n_stiles, FGX_TPB, 0, D));
+      if (fused_source) LCHK(launch("vel_fixup", 0, k_vel_fixup<real>, n_stiles, FGX_TPB, 0, D));
     }
     HIPCHK(hipStreamSynchronize(stream));
     if (prof_on) prof_collect();

@@ -166,6 +166,19 @@ __device__ __forceinline__ real substep1_vr(real vr, real p, real pm, real sr, r
   const real sv = vsum / (real)4 + q.rinfdom;
   return vr + dt * (q.acc0 - dgp - dphi + sv * ((real)2 * q.u0 + sv) * q.invrinf);
 }
+// Update_sgvelocityKernel's two updates (Kernels.cu:1024-1041) in FP64: the
+// radial acceleration interpolated to the inner interface of ring i, and the
+// azimuthal one averaged over sectors j-1 and j.  Contraction is off, so that
+// the separate update kernel and the fused source pass round them alike.
+__device__ __forceinline__ double sg_vrad_new(double v, double dt, double ri, double rm1, double rm, double ar,
+                                              double ar1, double idr) {
+#pragma clang fp contract(off)
+  return v + dt * ((ri - rm1) * ar + (rm - ri) * ar1) * idr;
+}
+__device__ __forceinline__ double sg_vtheta_new(double v, double dt, double at, double am) {
+#pragma clang fp contract(off)
+  return v + 0.5 * dt * (at + am);
+}
 // Substep1's azimuthal pressure gradient (times dx_theta): the difference of
 // the pressure departures over the mean density (FP32 with a reciprocal)
 template <class real>

@@ -206,13 +206,10 @@ template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_sg_update(Dev<real> D) {
   FGX_STRIP_BEGIN {
     const double dt = s.dt;
-    if (i > 0) {
-      const double ri = RT(rinf_d)[rb + i];
-      const double inc = dt * ((ri - RT(rmed_d)[rb + i - 1]) * (double)AT(D.sg_accr, i, j) +
-                               (RT(rmed_d)[rb + i] - ri) * (double)AT(D.sg_accr, i - 1, j)) *
-                         RT(invdiffrmed_d)[rb + i];
-      AT(D.vradint, i, j) = (real)((double)AT(D.vradint, i, j) + inc);
-    }
+    if (i > 0)
+      AT(D.vradint, i, j) = (real)sg_vrad_new((double)AT(D.vradint, i, j), dt, RT(rinf_d)[rb + i], RT(rmed_d)[rb + i - 1],
+                                              RT(rmed_d)[rb + i], (double)AT(D.sg_accr, i, j),
+                                              (double)AT(D.sg_accr, i - 1, j), RT(invdiffrmed_d)[rb + i]);
     // (j-1)%nsec at j = 0 is -1 in the reference: linear index i*nsec-1, the
     // previous ring's last sector (periodic on ring 0, where it would read
     // before the array; ext_fix_sg_wrap makes every ring periodic)
@@ -220,7 +217,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_update(Dev<real> D) {
     if (j > 0) am = (double)AT(D.sg_acct, i, j - 1);
     else if (D.fix_sgwrap || i == 0) am = (double)AT(D.sg_acct, i, ns - 1);
     else am = (double)AT(D.sg_acct, i - 1, ns - 1);
-    AT(D.vthetaint, i, j) = (real)((double)AT(D.vthetaint, i, j) + 0.5 * dt * ((double)AT(D.sg_acct, i, j) + am));
+    AT(D.vthetaint, i, j) = (real)sg_vtheta_new((double)AT(D.vthetaint, i, j), dt, (double)AT(D.sg_acct, i, j), am);
   }
 }
 

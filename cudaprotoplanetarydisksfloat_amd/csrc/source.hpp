@@ -34,7 +34,11 @@ namespace fgx {
 #define FGX_SRC_STAGE 32  // rings of coefficients staged per strip: rs + 8
 #define FGX_SRC_RINGS_MAX (FGX_SRC_STAGE - 8)
 
-template <class real>
+// SG: self-gravitating disks.  Their acceleration field (formed from the
+// density before the launch) is added to VradInt and VthetaInt right after
+// Substep1, as Update_sgvelocityKernel (Kernels.cu:1024-1041) does between
+// Substep1 and the viscous stresses, in FP64.
+template <class real, bool SG>
 __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
   const CellTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
@@ -179,7 +183,7 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
   // caches still hold (their values are never used): a fixed number of loads
   // per iteration lets the compiler wait for exactly the ring it needs,
   // without fetching rings no stage reads.
-  real nsg, nvr, nvt, nvtp, ne, nc0;
+  real nsg, nvr, nvt, nvtp, ne, nc0, nar = 0, nat = 0;
   const int klast = (i1 + 1 < nr) ? i1 + 1 : nr - 1;
   auto fetch = [&](int k) {
     const int kk = k < klast ? k : klast;
@@ -189,14 +193,26 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     nvtp = AT(D.vtheta, kk, jp);
     ne = ad ? (real)AT(D.energy, kk, j) : (real)0;
     nc0 = AT(D.dens, kk, 0);
+    if constexpr (SG) {
+      nar = AT(D.sg_accr, kk, j);
+      nat = AT(D.sg_acct, kk, j);
+    }
   };
+  // the self-gravity acceleration of ring k-1 at this lane's sector
+  real arA1 = 0, atA1 = 0;
+  if constexpr (SG) {
+    if (ks > 0) {
+      arA1 = AT(D.sg_accr, ks - 1, j);
+      atA1 = AT(D.sg_acct, ks - 1, j);
+    }
+  }
   real c0A1 = 0;   // column-0 density of ring k-1 (whole)
   if (ks > 0) c0A1 = FGX_CF(10, ks - 1) + (real)AT(D.dens, ks - 1, 0);
   fetch(ks);
   auto ring_step = [&](const int k) {
     // ---------------------------------------------------------- stage A
     real sg0 = 0, e0 = 0, vri0 = 0, vti0 = 0;
-    const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne, cc0 = nc0;
+    const real csg = nsg, cvr = nvr, cvt = nvt, cvtp = nvtp, ce = ne, cc0 = nc0, car = nar, cat = nat;
     fetch(k + 1);
     if (k < nr && k <= i1 + 1) {
       const real sr = csg;
@@ -222,6 +238,20 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
       const real gradphi = (real)(phi - phil) * invdxt;
       real v = vt - (real)dtd * (gradp + gradphi);
       v += (real)dtd * FGX_CF(1, k);
+      if constexpr (SG) {
+        // the radial acceleration interpolated to the inner interface; the
+        // azimuthal one averaged with sector j-1, which at j = 0 the
+        // reference takes from ring k-1's last sector (a linear index of
+        // -1 past the ring start) unless the quirk fix asks for ring k's
+        if (k > 0)
+          vri0 = (real)sg_vrad_new((double)vri0, dtd, FGX_CD(5, k), FGX_CD(1, k - 1), FGX_CD(1, k), (double)car,
+                                   (double)arA1, FGX_CD(2, k));
+        const real atl = lane_from_left(cat), atl1 = lane_from_left(atA1);
+        const real am = (j == 0 && k > 0 && !D.fix_sgwrap) ? atl1 : atl;
+        v = (real)sg_vtheta_new((double)v, dtd, (double)cat, (double)am);
+        arA1 = car;
+        atA1 = cat;
+      }
       vti0 = v;
       sgA1 = sg0; srA1 = sr; vtA1 = vt; vtpA1 = vtp; pA1 = p; phiA1 = phi;
     }
@@ -325,11 +355,16 @@ __global__ void __launch_bounds__(64) FGX_SRC_ATTR k_source(Dev<real> D, const C
     trr2 = trr1; trr1 = trr0; tpp2 = tpp1; tpp1 = tpp0; trp1 = trp0;
     vrc2 = vrc1; vrc1 = vrc0; vtc1 = vtc0;
   };
-  // two rings per trip, so that the register shifts of period two cost no
-  // moves; an odd count runs one more ring past ke, which stores nothing
-  for (int k = ks; k <= ke; k += 2) {
-    ring_step(k);
-    ring_step(k + 1);
+  // FP32: two rings per trip, so that the register shifts of period two
+  // cost no moves (an odd count runs one more ring past ke, which stores
+  // nothing); FP64 keeps one ring per trip, where two would cost occupancy
+  if constexpr (sizeof(real) == 4) {
+    for (int k = ks; k <= ke; k += 2) {
+      ring_step(k);
+      ring_step(k + 1);
+    }
+  } else {
+    for (int k = ks; k <= ke; ++k) ring_step(k);
   }
 }
 #undef FGX_CD

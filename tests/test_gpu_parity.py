@@ -194,14 +194,18 @@ def test_mass_conserved_closed(gpu):
 
 @pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("case", ["plain", "plain_log_alpha", "adiabatic_evanescent", "adiabatic_cooling_open",
-                                  "nonreflecting", "two_planets", "guiding_center"])
+                                  "nonreflecting", "two_planets", "guiding_center", "self_gravity",
+                                  "self_gravity_radix5"])
 def test_fused_source_equals_separate_kernels(gpu, monkeypatch, case, precision):
-    """The one-pass source kernel (potential, Substep1, stresses, viscous
-    update, Substep2) does the separate kernels' arithmetic operation for
-    operation: after CFL-driven steps every field is bit-identical."""
+    """The one-pass source kernel (potential, Substep1, the self-gravity
+    update, stresses, viscous update, Substep2) does the separate kernels'
+    arithmetic operation for operation: after CFL-driven steps every field
+    is bit-identical."""
     par, cfg, keys = _case(case)
     keys = dict(keys)
-    keys.setdefault("NRAD", 70)   # several strips and a ragged last one
+    # several strips and a ragged last one; self-gravity needs 2*NRAD to
+    # factor into 2, 3 and 5 (150 = 2*3*5*5)
+    keys.setdefault("NRAD", 75 if case.startswith("self_gravity") else 70)
     p, pl = configs.load(par, cfg, DT=DT_SMALL, **keys)
     out = {}
     for fused in ("1", "0"):
