@@ -256,8 +256,16 @@ def main():
 
     plain = None
     if not args.no_plain and len(loaded) > 1:
-        # the same disks one at a time through the single-disk path
-        solo = [Batch(p, pl, prec) for p, pl in loaded]
+        # the same disks one at a time through the single-disk path, each with
+        # the tiling the autotuner picks for one disk of its grid
+        solo_tiling = {}
+        if not args.no_tune:
+            for p, pl in loaded:
+                shape = (int(p.get("NRAD")), int(p.get("NSEC")))
+                if shape not in solo_tiling:
+                    solo_tiling[shape] = autotune([p], [pl], prec, n_dt=2, cache=args.tiling_cache)[0]
+        solo = [Batch(p, pl, prec, tiling=solo_tiling.get((int(p.get("NRAD")), int(p.get("NSEC")))))
+                for p, pl in loaded]
         for b in solo:
             b.step(1)
         torch.cuda.synchronize()
@@ -269,7 +277,8 @@ def main():
         pwall = time.perf_counter() - t0
         pup = sum(_updates(b, bf) for b, bf in zip(solo, befores))
         plain = {"value": pup / pwall, "unit": "cell-updates/s", "wall_s": round(pwall, 3),
-                 "batched_over_plain": round(value / (pup / pwall), 3)}
+                 "batched_over_plain": round(value / (pup / pwall), 3),
+                 "tiling": "tuned per grid" if solo_tiling else "engine's own"}
         for b in solo:
             b.close()
 
