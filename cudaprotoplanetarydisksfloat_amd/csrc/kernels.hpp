@@ -681,7 +681,7 @@ __device__ double get_psys_info_dev(const Dev<real>& D, const DiskConst& c, Disk
 
 // DerivMotionRK5 (RungeKutta.cu:9-46)
 template <class real>
-__device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const double* mass, const double* q, double* der,
+__device__ __forceinline__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const double* mass, const double* q, double* der,
                               double* dist, int nb, double dt) {
   const double *x = q, *y = q + nb, *vx = q + 2 * nb, *vy = q + 3 * nb;
   for (int i = 0; i < nb; ++i) dist[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
@@ -709,6 +709,45 @@ __device__ void deriv_rk5_dev(const Dev<real>& D, const DiskConst& c, const doub
     der[i + 3 * nb] = ay;
   }
   for (int i = 0; i < 4 * nb; ++i) der[i] *= dt;
+}
+
+// The Cash-Karp stages of AdvanceSystemRK5 (RungeKutta.cu:55-74): q1 = q0
+// advanced by dt.  With a compile-time planet count (NB > 0) every stage
+// array is a register array and the six derivative evaluations and stage
+// sums are straight-line code, with no LDS round trip between dependent
+// steps (one planet: 21.5 -> 16.4 us per setup launch of the 512x1536 disk);
+// NB = 0 takes the count at run time, with the stages in the LDS scratch
+// `rk`.  Both do the same operations in the same order.
+template <class real, int NB>
+__device__ __forceinline__ void rk5_stages(const Dev<real>& D, const DiskConst& c, const DiskState& s, int nb,
+                                           double dt, double* rk, double* q1out) {
+  constexpr int S = NB > 0 ? 4 * NB : 1;
+  double q0r[S], q1r[S], k1r[S], k2r[S], k3r[S], k4r[S], k5r[S], k6r[S], distr[NB > 0 ? NB : 1];
+  const int L = 4 * FGX_MAX_PLANETS;
+  double *q0 = NB ? q0r : rk, *q1 = NB ? q1r : rk + L, *k1 = NB ? k1r : rk + 2 * L, *k2 = NB ? k2r : rk + 3 * L;
+  double *k3 = NB ? k3r : rk + 4 * L, *k4 = NB ? k4r : rk + 5 * L, *k5 = NB ? k5r : rk + 6 * L;
+  double *k6 = NB ? k6r : rk + 7 * L, *dist = NB ? distr : rk + 8 * L;
+  if (NB) nb = NB;
+  for (int k = 0; k < nb; ++k) {
+    q0[k] = s.x[k]; q0[k + nb] = s.y[k]; q0[k + 2 * nb] = s.vx[k]; q0[k + 3 * nb] = s.vy[k];
+  }
+  const int n4 = 4 * nb;
+  deriv_rk5_dev(D, c, s.mass, q0, k1, dist, nb, dt);
+  for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.2 * k1[m];
+  deriv_rk5_dev(D, c, s.mass, q1, k2, dist, nb, dt);
+  for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.075 * k1[m] + 0.225 * k2[m];
+  deriv_rk5_dev(D, c, s.mass, q1, k3, dist, nb, dt);
+  for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.3 * k1[m] - 0.9 * k2[m] + 1.2 * k3[m];
+  deriv_rk5_dev(D, c, s.mass, q1, k4, dist, nb, dt);
+  for (int m = 0; m < n4; ++m)
+    q1[m] = q0[m] - 11.0 / 54.0 * k1[m] + 2.5 * k2[m] - 70.0 / 27.0 * k3[m] + 35.0 / 27.0 * k4[m];
+  deriv_rk5_dev(D, c, s.mass, q1, k5, dist, nb, dt);
+  for (int m = 0; m < n4; ++m)
+    q1[m] = q0[m] + 1631.0 / 55296.0 * k1[m] + 175.0 / 512.0 * k2[m] + 575.0 / 13824.0 * k3[m] +
+            44275.0 / 110592.0 * k4[m] + 253.0 / 4096.0 * k5[m];
+  deriv_rk5_dev(D, c, s.mass, q1, k6, dist, nb, dt);
+  for (int m = 0; m < n4; ++m)
+    q1out[m] = q0[m] + 37.0 / 378.0 * k1[m] + 250.0 / 621.0 * k3[m] + 125.0 / 594.0 * k4[m] + 512.0 / 1771.0 * k6[m];
 }
 
 // reduction of the ring partial forces, FillForcesArrays' potential
@@ -758,33 +797,14 @@ __device__ void planets_block(const Dev<real>& D, int d, DiskState& s, double* r
   }
   // AdvanceSystemRK5 (Pframeforce.cu:94-157)
   if (nb > 0) {
-    // the Cash-Karp stages in LDS (rk: 8 x 4*FGX_MAX_PLANETS doubles plus
-    // FGX_MAX_PLANETS distances) rather than in per-thread scratch memory
-    const int S = 4 * FGX_MAX_PLANETS;
-    double *q0 = rk, *q1 = rk + S;
+    // the advanced state: one planet runs the stages in registers, more in
+    // the LDS scratch (rk: 8 x 4*FGX_MAX_PLANETS doubles plus FGX_MAX_PLANETS
+    // distances); register stages for two or three planets raised the
+    // kernel's registers enough to slow the batch's one-planet launches
+    double* q1 = rk + 4 * FGX_MAX_PLANETS;
     if (!D.forced_circular) {
-      double *k1 = rk + 2 * S, *k2 = rk + 3 * S, *k3 = rk + 4 * S, *k4 = rk + 5 * S, *k5 = rk + 6 * S, *k6 = rk + 7 * S;
-      double* dist = rk + 8 * S;
-      for (int k = 0; k < nb; ++k) {
-        q0[k] = s.x[k]; q0[k + nb] = s.y[k]; q0[k + 2 * nb] = s.vx[k]; q0[k + 3 * nb] = s.vy[k];
-      }
-      const int n4 = 4 * nb;
-      deriv_rk5_dev(D, c, s.mass, q0, k1, dist, nb, dt);
-      for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.2 * k1[m];
-      deriv_rk5_dev(D, c, s.mass, q1, k2, dist, nb, dt);
-      for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.075 * k1[m] + 0.225 * k2[m];
-      deriv_rk5_dev(D, c, s.mass, q1, k3, dist, nb, dt);
-      for (int m = 0; m < n4; ++m) q1[m] = q0[m] + 0.3 * k1[m] - 0.9 * k2[m] + 1.2 * k3[m];
-      deriv_rk5_dev(D, c, s.mass, q1, k4, dist, nb, dt);
-      for (int m = 0; m < n4; ++m)
-        q1[m] = q0[m] - 11.0 / 54.0 * k1[m] + 2.5 * k2[m] - 70.0 / 27.0 * k3[m] + 35.0 / 27.0 * k4[m];
-      deriv_rk5_dev(D, c, s.mass, q1, k5, dist, nb, dt);
-      for (int m = 0; m < n4; ++m)
-        q1[m] = q0[m] + 1631.0 / 55296.0 * k1[m] + 175.0 / 512.0 * k2[m] + 575.0 / 13824.0 * k3[m] +
-                44275.0 / 110592.0 * k4[m] + 253.0 / 4096.0 * k5[m];
-      deriv_rk5_dev(D, c, s.mass, q1, k6, dist, nb, dt);
-      for (int m = 0; m < n4; ++m)
-        q1[m] = q0[m] + 37.0 / 378.0 * k1[m] + 250.0 / 621.0 * k3[m] + 125.0 / 594.0 * k4[m] + 512.0 / 1771.0 * k6[m];
+      if (nb == 1) rk5_stages<real, 1>(D, c, s, nb, dt, rk, q1);
+      else rk5_stages<real, 0>(D, c, s, nb, dt, rk, q1);
     }
     for (int k = 1 - (s.time >= c.releasedate ? 1 : 0); k < nb; ++k) {
       if (!D.forced_circular) {
