@@ -349,7 +349,7 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   // WM: 2 = W arcs of one full wave each, 1 = one full wave (nsec = 64*K),
   // 0 = one wave of T < 64 lanes
   constexpr bool MULTI = WM == 2;
-  __shared__ real ex[2][4][16];
+  __shared__ real ex[2][12][16];
   __shared__ double redw[16];
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
@@ -613,6 +613,128 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
     apply_u(X, F, c2);
   };
 
+  // FP32: the four momenta (RadMomP, RadMomM, ThetaMomP, ThetaMomM) go
+  // through both passes in lockstep: each takes exactly `both`'s operations,
+  // and each step's halo exchange carries all four, one barrier across arcs
+  // where `both` takes one per quantity (16 -> 4 per ring).
+  auto halo3x4 = [&](const real (&X)[4][K], real (&l1)[4], real (&l2)[4], real (&r1)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      l1[q] = lane_from_left(X[q][K - 1]);
+      l2[q] = lane_from_left((K >= 2) ? X[q][K - 2] : X[q][0]);
+      r1[q] = lane_from_right(X[q][0]);
+    }
+    if constexpr (MULTI) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const real e1 = lane_read(X[q][K - 1], 63), e2 = lane_read((K >= 2) ? X[q][K - 2] : X[q][0], 63);
+        const real er = lane_read(X[q][0], 0);
+        ex[xb][3 * q][w] = e1; ex[xb][3 * q + 1][w] = e2; ex[xb][3 * q + 2][w] = er;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const real n1 = ex[xb][3 * q][wl], n2 = ex[xb][3 * q + 1][wl], nr = ex[xb][3 * q + 2][wr];
+        l1[q] = lane == 0 ? n1 : l1[q];
+        l2[q] = lane == 0 ? n2 : l2[q];
+        r1[q] = lane == 63 ? nr : r1[q];
+      }
+      xb ^= 1;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        fix_single(l1[q], l2[q], r1[q], X[q][K - 1], (K >= 2) ? X[q][K - 2] : X[q][0], X[q][0]);
+    }
+  };
+  auto star4 = [&](const real (&X)[4][K], unsigned pos, const real (&cf)[K], real (&out)[4][K]) {
+    real xl1[4], xl2[4], xr1[4];
+    halo3x4(X, xl1, xl2, xr1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if constexpr (K == 1) xl2[q] = left_of(xl1[q]);
+      real slp = slope(xl2[q], xl1[q], X[q][0]);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const real xm = (k == 0) ? xl1[q] : X[q][k - 1];
+        const real xp = (k == K - 1) ? xr1[q] : X[q][k + 1];
+        const real sl = slope(xm, X[q][k], xp);
+        out[q][k] = ((pos >> k) & 1u) ? xm + cf[k] * slp : X[q][k] - cf[k] * sl;
+        slp = sl;
+      }
+    }
+  };
+  auto star_u4 = [&](const real (&X)[4][K], bool pos, real cf, real (&out)[4][K]) {
+    real xl1[4], xl2[4], xr1[4];
+    halo3x4(X, xl1, xl2, xr1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if constexpr (K == 1) xl2[q] = left_of(xl1[q]);
+      real slp = slope(xl2[q], xl1[q], X[q][0]);
+      if (pos) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const real xm = (k == 0) ? xl1[q] : X[q][k - 1];
+          const real xp = (k == K - 1) ? xr1[q] : X[q][k + 1];
+          const real sl = slope(xm, X[q][k], xp);
+          out[q][k] = xm + cf * slp;
+          slp = sl;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const real xm = (k == 0) ? xl1[q] : X[q][k - 1];
+          const real xp = (k == K - 1) ? xr1[q] : X[q][k + 1];
+          out[q][k] = X[q][k] - cf * slope(xm, X[q][k], xp);
+        }
+      }
+    }
+  };
+  auto apply4 = [&](real (&X)[4][K], real (&F)[4][K], const real (&SV)[K]) {
+    real fr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) F[q][k] *= SV[k];
+      fr[q] = lane_from_right(F[q][0]);
+    }
+    if constexpr (MULTI) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ex[xb][q][w] = lane_read(F[q][0], 0);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const real nr = ex[xb][q][wr];
+        fr[q] = lane == 63 ? nr : fr[q];
+      }
+      xb ^= 1;
+    } else if constexpr (WM == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const real e = lane_read(F[q][0], 0);
+        if (lane == T - 1) fr[q] = e;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < K; ++k) X[q][k] += F[q][k] - ((k == K - 1) ? fr[q] : F[q][k + 1]);
+  };
+  auto both4 = [&](real (&M)[4][K]) {
+    real A[4][K], F[4][K];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < K; ++k) A[q][k] = M[q][k] * invD1[k];
+    star4(A, pos1, cf1, F);
+    apply4(M, F, SV1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < K; ++k) A[q][k] = M[q][k] * invD2[k];
+    star_u4(A, pos2, cf2, F);
+    apply4(M, F, SV2);
+  };
+
   // pass-1 velocity (ComputeResiduals); STANDARD transport folds the
   // constant part in and the second pass moves nothing
 #pragma unroll
@@ -651,14 +773,29 @@ __device__ __forceinline__ void theta_arc_ring(const Dev<real>& D, const RingTil
   }
   // quantities in the order RadMomP, RadMomM, [energy], ThetaMomP (held),
   // ThetaMomM (held); the energy is stored as a residual of its ring profile
-  real* dsts[3] = {D.t_rmp, D.t_rmm, D.energy};
   real H[K], Qn[K];
+  if constexpr (sizeof(real) == 4) {
+    constexpr int m2 = AD ? 3 : 2, m3 = AD ? 4 : 3;
+    real M[4][K];
 #pragma unroll
-  for (int o = 0; o < 5; ++o) {
-    if (o >= nq) break;
-    if (ad && o == 2) both_energy(Qa[o]);
-    else both(Qa[o]);
-    if (o < oh) store_shift(dsts[o], Qa[o]);
+    for (int k = 0; k < K; ++k) { M[0][k] = Qa[0][k]; M[1][k] = Qa[1][k]; M[2][k] = Qa[m2][k]; M[3][k] = Qa[m3][k]; }
+    both4(M);
+#pragma unroll
+    for (int k = 0; k < K; ++k) { Qa[0][k] = M[0][k]; Qa[1][k] = M[1][k]; Qa[m2][k] = M[2][k]; Qa[m3][k] = M[3][k]; }
+    if (ad) both_energy(Qa[2]);
+    store_shift(D.t_rmp, Qa[0]);
+    store_shift(D.t_rmm, Qa[1]);
+    if (ad) store_shift(D.energy, Qa[2]);
+  } else {
+    // FP64 takes the quantities one at a time: four in lockstep spill
+    real* dsts[3] = {D.t_rmp, D.t_rmm, D.energy};
+#pragma unroll
+    for (int o = 0; o < 5; ++o) {
+      if (o >= nq) break;
+      if (ad && o == 2) both_energy(Qa[o]);
+      else both(Qa[o]);
+      if (o < oh) store_shift(dsts[o], Qa[o]);
+    }
   }
   if constexpr (LAB) both(Lq);
 #pragma unroll
