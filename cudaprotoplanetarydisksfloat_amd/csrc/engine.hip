@@ -19,6 +19,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <cstdint>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -293,6 +294,7 @@ struct Batch : BatchBase {
 
   ~Batch() override {
     if (stream) hipStreamSynchronize(stream);
+    graphs_clear();
     for (void* p : snap_host) if (p) hipHostFree(p);
     for (hipEvent_t e : snap_ev) if (e) hipEventDestroy(e);
     for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
@@ -377,6 +379,17 @@ struct Batch : BatchBase {
   template <class K, class... A>
   int launch(const char* name, double bytes_per_cell, K kern, int grid, int block, size_t shmem, A... args) {
     if (grid <= 0) return FGX_OK;
+    if (launch_mode == 1) {
+      // recording: the launch's kernel, shape and argument bytes go into the
+      // substep's key, nothing is launched
+      const void* kp = (const void*)kern;
+      rec_mix(&kp, sizeof kp);
+      rec_mix(&grid, sizeof grid);
+      rec_mix(&block, sizeof block);
+      rec_mix(&shmem, sizeof shmem);
+      (rec_mix(&args, sizeof(args)), ...);
+      return FGX_OK;
+    }
     Pending pd{};
     const bool timed = prof_on && !in_group;
     if (timed) {
@@ -398,6 +411,76 @@ struct Batch : BatchBase {
     if (timed) pending.push_back(pd);
     return FGX_OK;
   }
+  // ---------------------------------------------------- substep graphs
+  // With graphs on, a substep's launches run as one instantiated hipGraph.
+  // The substep is first run with launches only recorded: its kernels,
+  // shapes and argument bytes key a cache of graphs, and the host-side state
+  // (velocity buffer parity, tiling, the plan position) advances exactly as
+  // the launches would have advanced it.  On a miss the same substep is
+  // captured from the same starting state and instantiated.  Two substeps
+  // with the same key launch the same kernels with the same arguments, so
+  // replaying the cached graph is the same work; the tile lists and disk
+  // state the kernels read are device data, read when the graph runs.
+  bool graphs_on = false;
+  int launch_mode = 0;   // 0 launch, 1 record the key only
+  uint64_t rec_hash = 0;
+  std::unordered_map<uint64_t, hipGraphExec_t> graph_cache;
+  long long graph_hits = 0, graph_builds = 0;
+  void rec_mix(const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t k = 0; k < n; ++k) { rec_hash ^= b[k]; rec_hash *= 1099511628211ull; }
+  }
+  void graphs_clear() {
+    for (auto& kv : graph_cache) hipGraphExecDestroy(kv.second);
+    graph_cache.clear();
+  }
+  struct HostTiles {
+    const CellTile *st, *src;
+    int n_st, strip_rings, n_src, src_rings;
+  };
+  HostTiles tiles_save() const { return HostTiles{stiles, srctiles, n_stiles, strip_rings, n_srctiles, src_rings}; }
+  void tiles_restore(const HostTiles& t) {
+    stiles = t.st; srctiles = t.src; n_stiles = t.n_st; strip_rings = t.strip_rings;
+    n_srctiles = t.n_src; src_rings = t.src_rings;
+  }
+  int substep_graphed() {
+    const Dev<real> D0 = D;
+    const int pp0 = plan_pos;
+    const HostTiles t0 = tiles_save();
+    launch_mode = 1;
+    rec_hash = 1469598103934665603ull;
+    int r = substep();
+    launch_mode = 0;
+    if (r) return r;
+    auto it = graph_cache.find(rec_hash);
+    if (it == graph_cache.end()) {
+      const Dev<real> D1 = D;
+      const int pp1 = plan_pos;
+      const HostTiles t1 = tiles_save();
+      D = D0; plan_pos = pp0; tiles_restore(t0);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      r = substep();
+      const hipError_t e = hipStreamEndCapture(stream, &g);
+      if (r) { if (g) hipGraphDestroy(g); return r; }
+      if (e != hipSuccess) return set_error(FGX_EDEVICE, "substep capture failed: %s", hipGetErrorString(e));
+      hipGraphExec_t x = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (ei != hipSuccess) return set_error(FGX_EDEVICE, "substep graph instantiation failed: %s", hipGetErrorString(ei));
+      if (graph_cache.size() >= 512) graphs_clear();
+      it = graph_cache.emplace(rec_hash, x).first;
+      ++graph_builds;
+      // the capture advanced the host state as the recording did
+      if (plan_pos != pp1 || std::memcmp(&D, &D1, sizeof D) != 0 || tiles_save().st != t1.st)
+        return set_error(FGX_EDEVICE, "substep capture changed the host state differently from its recording");
+    } else {
+      ++graph_hits;
+    }
+    HIPCHK(hipGraphLaunch(it->second, stream));
+    return FGX_OK;
+  }
+
   // several launches timed as one kernel of the profile
   bool in_group = false, grp_started = false;
   Pending grp{};
@@ -741,6 +824,7 @@ struct Batch : BatchBase {
     HIPCHK(hipHostMalloc((void**)&h_ranges, max_ranges * sizeof(CompactRange)));
     LCHK(dalloc(&d_ranges, max_ranges));
     if (const char* e = std::getenv("FGX_COMPACT")) compact_on = std::atoi(e) != 0;
+    if (const char* e = std::getenv("FGX_GRAPH")) graphs_on = std::atoi(e) != 0;
     std::vector<DiskState> sth(nd);
     for (int d = 0; d < nd; ++d) sth[d] = initial_state(d);
     HIPCHK(hipMemcpyAsync((void*)std_, sth.data(), nd * sizeof(DiskState), hipMemcpyHostToDevice, stream));
@@ -1102,7 +1186,7 @@ struct Batch : BatchBase {
       plan_pos = 0;
       for (int guard = 0;; ++guard) {
         if (guard > 10000000) return set_error(FGX_ENUMERIC, "substep loop did not terminate");
-        LCHK(substep());
+        LCHK((graphs_on && !prof_on) ? substep_graphed() : substep());
         if (plan > 1) { --plan; continue; }
         LCHK(launch("summary", 0, k_summary<real>, 1, FGX_TPB, 0, D));
         HIPCHK(hipMemcpyAsync(h_summary, D.summary, (2 + FGX_SUMMARY_HIST + nd) * sizeof(int), hipMemcpyDeviceToHost,
