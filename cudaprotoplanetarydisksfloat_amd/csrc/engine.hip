@@ -732,6 +732,7 @@ struct Batch : BatchBase {
       D.sigmamed_d = sigmamed_d; D.energymed_d = energymed_d;
       LCHK(dalloc_share(&rmed_d, R, 1)); LCHK(dalloc_share(&invrmed_d, R, 1)); LCHK(dalloc_share(&invdiffrmed_d, R, 1));
       LCHK(dalloc_share(&D.vmoy, R, 1)); LCHK(dalloc_share(&cosj_d, S, 2)); LCHK(dalloc_share(&sinj_d, S, 2));
+      if (P0.evanescent) { LCHK(dalloc_share(&D.ev_d2, R, 1)); LCHK(dalloc_share(&D.ev_vt0s, R, 1)); }
       D.vref = vref; D.acc0 = acc0; D.u0 = u0; D.gq0 = gq0; D.rinf_d = rinf_d; D.invrinf_d = invrinf_d; D.rmed_d = rmed_d;
       D.invdiffrmed_d = invdiffrmed_d; D.cosj_d = cosj_d; D.sinj_d = sinj_d; D.invrmed_d = invrmed_d;
     }
@@ -1023,6 +1024,8 @@ struct Batch : BatchBase {
       LCHK(upload_vec(D.cosj + dch[d].so, h.cosj, h.nsec));
       LCHK(upload_vec(D.sinj + dch[d].so, h.sinj, h.nsec));
     }
+    // the evanescent boundary's per-ring terms, from the geometry just uploaded
+    if (D.evanescent) LCHK(launch("ev_tables", 0, k_ev_tables<real>, 1, FGX_TPB, 0, D, d));
     return FGX_OK;
   }
 

@@ -43,6 +43,10 @@ struct Dev {
   // azimuthal velocity, gq0 their pressure difference over the density sum
   const double *vref, *acc0, *u0, *gq0, *rinf_d, *invrinf_d, *rmed_d, *invrmed_d, *invdiffrmed_d, *cosj_d, *sinj_d, *cs2_d, *sigmamed_d,
       *energymed_d;
+  // EvanescentBoundary's per-ring terms of a disk whose aspect ratio does not
+  // change in time: the damping weight (damping^2 * 10) and the static part
+  // of the Keplerian relaxation target of vtheta (evanescent batches only)
+  double *ev_d2, *ev_vt0s;
   double* vmoy;  // ring-mean vtheta of the CFL condition
   double* phi;   // potential of the planets and the indirect term, FP64 field
   // self-gravity: transform work rows, kernel transforms, twiddles, ring
@@ -1351,23 +1355,60 @@ __global__ void __launch_bounds__(FGX_TPB) k_nonreflecting(Dev<real> D) {
 }
 
 // EvanescentBoundaryKernel (Kernels.cu:1076-1124)
+// EvanescentBoundary's per-ring terms (SideEuler.cu:122-141, Kernels.cu:1076-1124)
+// at time t: the damping weight d2 = damping^2 * 10 (lambda = d2 * dt / T of
+// the zone) and the static part of the relaxation target of vtheta,
+// sqrt(1/r (1 - (1 + sigma - 2f) h^2 r^2f)); zero outside the damping zones
+template <class real>
+__device__ __forceinline__ void ev_ring_terms(const Dev<real>& D, const DiskConst& c, int rb, int nr, int i, double t,
+                                              double& d2, double& vt0s) {
+  const double r = RT(rmed_d)[rb + i];
+  d2 = 0;
+  vt0s = 0;
+  if (!(r < c.ev_drmin || r > c.ev_drmax)) return;
+  double damping;
+  if (r < c.ev_drmin) {
+    damping = (r - c.ev_drmin) / (RT(rmed_d)[rb] - c.ev_drmin);
+    d2 = damping * damping * 10.0;
+  }
+  if (r > c.ev_drmax) {
+    damping = (r - c.ev_drmax) / (RT(rmed_d)[rb + nr - 1] - c.ev_drmax);
+    d2 = damping * damping * 10.0;
+  }
+  const double h = aspect_ratio_dev(c, r, t);
+  vt0s = sqrt(1.0 / r * (1.0 - (1.0 + c.sigmaslope - 2.0 * c.flaringindex) * h * h * pow(r, 2.0 * c.flaringindex)));
+}
+// the tables of disk d, once at set-up: its aspect ratio changes in time only
+// through LambdaDoubling, so with LambdaDoubling 0 the terms are those of
+// every substep; one block per disk
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_ev_tables(Dev<real> D, int d) {
+  const FGX_AS4 DiskConst& c = cref(D.dc + d);
+  for (int i = threadIdx.x; i < c.nrad; i += FGX_TPB) {
+    double d2, vt0s;
+    ev_ring_terms(D, c, c.ro, c.nrad, i, 0.0, d2, vt0s);
+    D.ev_d2[c.ro + i] = d2;
+    D.ev_vt0s[c.ro + i] = vt0s;
+  }
+}
 template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_evanescent(Dev<real> D) {
   FGX_STRIP_BEGIN {
     const double r = RT(rmed_d)[rb + i];
     if (!(r < s.ev_drmin || r > s.ev_drmax)) continue;
     const double dt = s.dt;
-    double lambda = 0, damping;
-    if (r < s.ev_drmin) {
-      damping = (r - s.ev_drmin) / (RT(rmed_d)[rb] - s.ev_drmin);
-      lambda = damping * damping * 10.0 * dt / s.ev_tin;
+    // the ring's terms: from the set-up tables (a pow, a square root and
+    // divisions per ring and launch less), or at this time with LambdaDoubling
+    double d2, vt0;
+    if (c.lambdadoubling == 0.0) {
+      d2 = RT(ev_d2)[rb + i];
+      vt0 = RT(ev_vt0s)[rb + i];
+    } else {
+      ev_ring_terms(D, c, rb, nr, i, s.time - s.time0, d2, vt0);
     }
-    if (r > s.ev_drmax) {
-      damping = (r - s.ev_drmax) / (RT(rmed_d)[rb + nr - 1] - s.ev_drmax);
-      lambda = damping * damping * 10.0 * dt / s.ev_tout;
-    }
-    const double h = aspect_ratio_dev(c, r, s.time - s.time0);
-    double vt0 = sqrt(1.0 / r * (1.0 - (1.0 + c.sigmaslope - 2.0 * c.flaringindex) * h * h * pow(r, 2.0 * c.flaringindex)));
+    // the outer zone's time scale where the zones overlap, as the outer
+    // assignment comes last in the reference
+    const double lambda = d2 * dt / (r > s.ev_drmax ? s.ev_tout : s.ev_tin);
     vt0 -= r * s.omega_frame;
     const double vr0 = -3.0 * (double)RT(visc)[rb + i] / r * (-c.sigmaslope + .5);
     // relaxation towards the FP64 initial profiles, evaluated in FP64
