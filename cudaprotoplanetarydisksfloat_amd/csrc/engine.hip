@@ -1030,6 +1030,21 @@ struct Batch : BatchBase {
   }
 
   // ------------------------------------------------------------------ step
+  // the CFL and force pass: a ring of more than 768 sectors takes one wave per
+  // 512 sectors (at most 8; cfl_waves), blocks sized for the batch's widest
+  int cfl_force(int mode) {
+    const double b = mode == 0 ? 3 : 1;
+    switch (cfl_waves(max_nsec)) {
+      case 1: return launch("cfl_force", b, k_cfl_force<real, 1>, n_rtiles, 64, 0, D, mode);
+      case 2: return launch("cfl_force", b, k_cfl_force<real, 2>, n_rtiles, 128, 0, D, mode);
+      case 3: return launch("cfl_force", b, k_cfl_force<real, 3>, n_rtiles, 192, 0, D, mode);
+      case 4: return launch("cfl_force", b, k_cfl_force<real, 4>, n_rtiles, 256, 0, D, mode);
+      case 5: return launch("cfl_force", b, k_cfl_force<real, 5>, n_rtiles, 320, 0, D, mode);
+      case 6: return launch("cfl_force", b, k_cfl_force<real, 6>, n_rtiles, 384, 0, D, mode);
+      case 7: return launch("cfl_force", b, k_cfl_force<real, 7>, n_rtiles, 448, 0, D, mode);
+      default: return launch("cfl_force", b, k_cfl_force<real, 8>, n_rtiles, 512, 0, D, mode);
+    }
+  }
   // azimuthal transport: one launch per lane width K
   // g.w: 0 = one full wave per ring, 1 = one partial wave, >= 2 = arcs
   template <int K>
@@ -1107,12 +1122,12 @@ struct Batch : BatchBase {
     // the tiling the last summary's forecast chose for this substep
     use_tiles(plan_pos < (int)fine_plan.size() && fine_plan[plan_pos]);
     ++plan_pos;
-    LCHK(launch("cfl_force", 3, k_cfl_force<real>, n_rtiles, 64, 0, D, 0));
+    LCHK(cfl_force(0));
     if (accretion) {
       // AccreteOntoPlanets needs the substep's dt and comes before the forces
       LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D, 1));
       LCHK(launch("accrete", 0, k_accrete<real>, nd, FGX_TPB, 0, D));
-      LCHK(launch("cfl_force", 1, k_cfl_force<real>, n_rtiles, 64, 0, D, 1));
+      LCHK(cfl_force(1));
       LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D, 2));
     } else {
       LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D, 0));

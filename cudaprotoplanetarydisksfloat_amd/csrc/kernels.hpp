@@ -409,8 +409,18 @@ __device__ __forceinline__ double potential_planets_at(const Dev<real>& D, const
 // start of the substep: one wave per ring, ring sums and minima by shuffles.
 // A disk whose planets accrete needs its forces after the accretion, which
 // needs the substep's dt: mode 0 skips its forces, mode 1 forms only those.
-template <class real>
-__global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
+// Waves per ring of the CFL and force pass: a ring of more than 768 sectors
+// takes one wave per 512 sectors (at most 8), each walking every wp-th group
+// of 64 sectors, so a wide ring's passes run wp times shorter.  The count is
+// the disk's own, so a disk's sums are the same alone and in any batch.
+__device__ __host__ __forceinline__ int cfl_waves(int nsec) {
+  return nsec <= 768 ? 1 : ((nsec + 511) / 512 < 8 ? (nsec + 511) / 512 : 8);
+}
+// WPR: the batch's largest count (a block of 64*WPR threads); the waves past
+// the ring's own count end at once.  The waves' sums and extrema meet in LDS
+// in wave order; with one wave every value is the one-wave pass's.
+template <class real, int WPR>
+__global__ void __launch_bounds__(64 * WPR) k_cfl_force(Dev<real> D, int mode) {
   const RingTile t = D.rtiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
   const FGX_AS4 DiskConst& c = cref(D.dc + t.d);
@@ -421,7 +431,25 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   if (mode == 1 && !c.accretes) return;
   const int ns = c.nsec, i = t.i, rb = c.ro;
   const long long fb = c.fo;
-  const int lane = (int)threadIdx.x;
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const int wp = WPR > 1 ? cfl_waves(ns) : 1;
+  if (w >= wp) return;   // a barrier below waits only for the waves still running
+  const int j0 = (int)threadIdx.x, js = 64 * wp;   // this thread's first sector and stride
+  __shared__ double xred[WPR];
+  // the block's sum (or max with MAX) of one value per wave, in wave order;
+  // every thread receives it
+  auto across = [&](double v, bool max) -> double {
+    if constexpr (WPR == 1) {
+      return v;
+    } else {
+      __syncthreads();   // the previous use of xred is over
+      if (lane == 0) xred[w] = v;
+      __syncthreads();
+      double r = xred[0];
+      for (int q = 1; q < wp; ++q) r = max ? (xred[q] > r ? xred[q] : r) : r + xred[q];
+      return r;
+    }
+  };
   // disk forces on the targets (k = 0 or k = 10 Hill cut), per ring in FP64
   // numeric health of the density at the start of the substep (mode 0): a
   // value that is not positive, or not a number, flags the disk; the first
@@ -431,7 +459,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   bool dbad = false;
   if (mode == 0 && (c.accretes || nt == 0)) {
 #pragma unroll 4
-    for (int j = lane; j < ns; j += 64) dbad |= !(s0 + (real)AT(D.dens, i, j) > (real)0);
+    for (int j = j0; j < ns; j += js) dbad |= !(s0 + (real)AT(D.dens, i, j) > (real)0);
   }
   if (mode == 1 || !c.accretes) {
     const real rmed = RT(rmed)[rb + i], surf = RT(surf)[rb + i];
@@ -441,7 +469,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
       const int kk = s.tk[q];
       double fx = 0, fy = 0;
 #pragma unroll 4
-      for (int j = lane; j < ns; j += 64) {
+      for (int j = j0; j < ns; j += js) {
         const real cm = surf * (s0 + (real)AT(D.dens, i, j));
         if (q == 0) dbad |= !(cm > (real)0);
         const real dx = rmed * RT(cosj)[c.so + j] - tx, dy = rmed * RT(sinj)[c.so + j] - ty;
@@ -454,9 +482,9 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
         fx += (double)(cm * dx * inv3 * cut);
         fy += (double)(cm * dy * inv3 * cut);
       }
-      fx = wave_sum(fx);
-      fy = wave_sum(fy);
-      if (lane == 0) { out[2 * q] = fx; out[2 * q + 1] = fy; }
+      fx = across(wave_sum(fx), false);
+      fy = across(wave_sum(fy), false);
+      if (threadIdx.x == 0) { out[2 * q] = fx; out[2 * q + 1] = fy; }
     }
   }
   if (mode == 0 && __any(dbad) && lane == 0) atomicOr(D.health + t.d, FGX_HEALTH_DENSITY);
@@ -464,9 +492,9 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   const double vr0 = RT(vref)[rb + i];
   double acc = 0;
 #pragma unroll 4
-  for (int j = lane; j < ns; j += 64) acc += (double)(real)AT(D.vtheta, i, j);
-  const double mres = wave_sum(acc) / (double)ns;   // ring mean of the residual
-  if (lane == 0) D.vmoy[rb + i] = vr0 + mres;
+  for (int j = j0; j < ns; j += js) acc += (double)(real)AT(D.vtheta, i, j);
+  const double mres = across(wave_sum(acc), false) / (double)ns;   // ring mean of the residual
+  if (threadIdx.x == 0) D.vmoy[rb + i] = vr0 + mres;
   if (i == 0) return;
   const real dxrad = RT(rsup)[rb + i] - RT(rinf)[rb + i];
   const real dxt = RT(dxtheta)[rb + i];
@@ -483,7 +511,7 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   real xmax = 0;
   bool nbad = false;
 #pragma unroll 4
-  for (int j = lane; j < ns; j += 64) {
+  for (int j = j0; j < ns; j += js) {
     const real rs = AT(D.vtheta, i, j);
     const real vr = AT(D.vrad, i, j);
     const real cs = D.adiabatic ? (real)AT(D.cs, i, j) : csr;
@@ -506,7 +534,8 @@ __global__ void __launch_bounds__(64) k_cfl_force(Dev<real> D, int mode) {
   }
   if (__any(nbad) && lane == 0) atomicOr(D.health + t.d, FGX_HEALTH_NAN);
   xmax = -wave_min(-xmax);
-  if (lane == 0) {
+  xmax = (real)across((double)xmax, true);
+  if (threadIdx.x == 0) {
     const real dt2 = (real)kCFLSECURITY / sqrt(xmax);
     D.newdt[rb + i] = dt2 < (real)1e30 ? dt2 : (real)1e30;
   }
