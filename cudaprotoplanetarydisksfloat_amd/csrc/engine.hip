@@ -1102,9 +1102,12 @@ struct Batch : BatchBase {
   // transform passes, leaving SG_Accr, SG_Acct and their ring means
   int self_gravity() {
     const double r = 1.0 / sizeof(real);
-    LCHK(launch("sg_rows", 20 * r, k_sg_rows_fwd<real>, n_rtiles, FGX_TPB, sg_shm_row, D));
-    LCHK(launch("sg_cols", 96 * r, k_sg_cols<real>, n_sgctiles, FGX_TPB, sg_shm_col, D));
-    LCHK(launch("sg_rows", 20 * r + 2, k_sg_rows_inv<real>, n_rtiles, FGX_TPB, sg_shm_row, D));
+    // as many threads as a stage has butterflies (at most 1024, at least 256)
+    auto threads = [](int butterflies) { return std::min(1024, std::max(FGX_TPB, (butterflies + 63) / 64 * 64)); };
+    const int trow = threads(max_nsec / 3), tcol = threads(max_nrad);
+    LCHK(launch("sg_rows", 20 * r, k_sg_rows_fwd<real>, n_rtiles, trow, sg_shm_row, D));
+    LCHK(launch("sg_cols", 96 * r, k_sg_cols<real>, n_sgctiles, tcol, sg_shm_col, D));
+    LCHK(launch("sg_rows", 20 * r + 2, k_sg_rows_inv<real>, n_rtiles, trow, sg_shm_row, D));
     return FGX_OK;
   }
   int boundary() {

@@ -105,9 +105,13 @@ __device__ inline cxd* lds_fft(cxd* a, cxd* b, int n, int nl, const int* fac, in
   return src;
 }
 
+// Block sizes: the passes are launched with 256 to 1024 threads, as many as a
+// stage has butterflies (a transform's LDS footprint holds a block per CU at
+// the largest grids, so its waves are all the CU has); every loop over
+// elements strides by the block size, and no value depends on it.
 // pass A: one block per ring i < NRAD
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_sg_rows_fwd(Dev<real> D) {
+__global__ void __launch_bounds__(1024) k_sg_rows_fwd(Dev<real> D) {
   extern __shared__ unsigned char fgx_smem[];
   FGX_RING_PROLOGUE
   if (i >= nr) return;
@@ -115,19 +119,19 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_rows_fwd(Dev<real> D) {
   cxd* b = a + ns;
   const double q = RT(rmed_d)[rb + i] / RT(rmed_d)[rb];
   const double sq = sqrt(q);
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     const double var = s_whole_d(D, rb, i, (real)AT(D.dens, i, j)) * sq;
     a[j] = cxd{var, var * q};
   }
   __syncthreads();
   const cxd* out = lds_fft(a, b, ns, 1, c.fac_ns, c.nf_ns, -1.0, D.sgtw + c.tw_ns);
   cxd* w = D.sgw + c.sgw_o + (long long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) w[j] = out[j];
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) w[j] = out[j];
 }
 
 // pass B: one block per pair of columns (k, NSEC-k) of a disk
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_sg_cols(Dev<real> D) {
+__global__ void __launch_bounds__(1024) k_sg_cols(Dev<real> D) {
   extern __shared__ unsigned char fgx_smem[];
   const RingTile t = D.sg_ctiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
@@ -140,7 +144,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_cols(Dev<real> D) {
   cxd* a = reinterpret_cast<cxd*>(fgx_smem);
   cxd* b = a + 2 * n;
   const cxd* w = D.sgw + c.sgw_o;
-  for (int u = threadIdx.x; u < nl * n; u += FGX_TPB) {
+  for (int u = threadIdx.x; u < nl * n; u += blockDim.x) {
     const int l = u / n, ki = u - l * n;
     a[u] = (ki < nr) ? w[(long long)ki * ns + (l ? kb : ka)] : cxd{0.0, 0.0};
   }
@@ -150,7 +154,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_cols(Dev<real> D) {
   cxd* o = (z == a) ? b : a;
   const cxd* kr = D.sgkr + c.sgk_o;
   const cxd* kt = D.sgkt + c.sgk_o;
-  for (int u = threadIdx.x; u < nl * n; u += FGX_TPB) {
+  for (int u = threadIdx.x; u < nl * n; u += blockDim.x) {
     const int l = u / n, ki = u - l * n;
     const int col = l ? kb : ka;
     const int pl = (nl == 1) ? 0 : 1 - l;
@@ -167,7 +171,7 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_cols(Dev<real> D) {
   __syncthreads();
   const cxd* res = lds_fft(o, z, n, nl, c.fac_nr, c.nf_nr, 1.0, tw);
   cxd* wo = D.sgw + c.sgw_o;
-  for (int u = threadIdx.x; u < nl * n; u += FGX_TPB) {
+  for (int u = threadIdx.x; u < nl * n; u += blockDim.x) {
     const int l = u / n, ki = u - l * n;
     if (ki < nr) wo[(long long)ki * ns + (l ? kb : ka)] = res[u];
   }
@@ -175,30 +179,42 @@ __global__ void __launch_bounds__(FGX_TPB) k_sg_cols(Dev<real> D) {
 
 // pass C: one block per ring i < NRAD
 template <class real>
-__global__ void __launch_bounds__(FGX_TPB) k_sg_rows_inv(Dev<real> D) {
+__global__ void __launch_bounds__(1024) k_sg_rows_inv(Dev<real> D) {
   extern __shared__ unsigned char fgx_smem[];
-  __shared__ double redd[FGX_TPB / 64];
+  __shared__ double redd[1024 / 64];
   FGX_RING_PROLOGUE
   if (i >= nr) return;
   cxd* a = reinterpret_cast<cxd*>(fgx_smem);
   cxd* b = a + ns;
   const cxd* w = D.sgw + c.sgw_o + (long long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) a[j] = w[j];
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) a[j] = w[j];
   __syncthreads();
   const cxd* out = lds_fft(a, b, ns, 1, c.fac_ns, c.nf_ns, 1.0, D.sgtw + c.tw_ns);
   const double q = RT(rmed_d)[rb + i] / RT(rmed_d)[rb];
   const double norm = c.sg_rstep * c.sg_tstep / ((double)(2 * nr) * (double)ns);
   const double nrr = norm / sqrt(q), ntt = norm / (q * sqrt(q));
   const double self = kG * c.sg_rstep * c.sg_tstep / c.sg_eps;
+  // the fields and the ring mean of g_r are formed by the first FGX_TPB
+  // threads, sector j by thread j mod FGX_TPB, and summed over their four
+  // waves in order, whatever the block size, so no bit depends on it
   double acc = 0;
-  for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-    const double gr = out[j].x * nrr + self * s_whole_d(D, rb, i, (real)AT(D.dens, i, j));
-    AT(D.sg_accr, i, j) = (real)gr;
-    AT(D.sg_acct, i, j) = (real)(out[j].y * ntt);
-    acc += gr;
+  if (threadIdx.x < FGX_TPB) {
+    for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
+      const double gr = out[j].x * nrr + self * s_whole_d(D, rb, i, (real)AT(D.dens, i, j));
+      AT(D.sg_accr, i, j) = (real)gr;
+      AT(D.sg_acct, i, j) = (real)(out[j].y * ntt);
+      acc += gr;
+    }
   }
-  acc = block_sum(acc, redd);
-  if (threadIdx.x == 0) D.axisg[rb + i] = acc / (double)ns;   // Make1Dprofile(2)
+  acc = wave_sum(acc);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0 && threadIdx.x < FGX_TPB) redd[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int k = 0; k < FGX_TPB / 64; ++k) r += redd[k];
+    D.axisg[rb + i] = r / (double)ns;   // Make1Dprofile(2)
+  }
 }
 
 // Update_sgvelocityKernel (Kernels.cu:1024-1041) on VradInt / VthetaInt
