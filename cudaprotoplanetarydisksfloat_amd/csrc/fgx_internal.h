@@ -59,7 +59,7 @@ double aspect_ratio(const fgx_params& P, double r, double t);
 double psys_frequency(const fgx_params& P, const fgx_planets& s);
 // self-gravity (sg.cpp): radix factorisation of a transform length (4, 2, 3,
 // 5), and the per-disk set-up: FP64 kernel transforms (interleaved re/im,
-// (2*nrad) x nsec), the ring mean of the initial radial SG acceleration, and
+// (2*nrad) x nsec stored by column, entry (i, j) at j*2*nrad + i), the ring mean of the initial radial SG acceleration, and
 // the planets' SG-corrected circular velocities (Init_planetarysys_withSG)
 #define FGX_SG_MAXFAC 24
 int sg_factor(int n, int* fac, int* nf);

@@ -124,12 +124,16 @@ int sg_host_setup(HostDisk& h, std::vector<double>& khat_r, std::vector<double>&
   }
   fft2d(kr, NR2, NS, -1);
   fft2d(kt, NR2, NS, -1);
+  // stored by column (entry (i, j) at j * 2NRAD + i): the device reads them a
+  // column pair at a time
   khat_r.resize(2 * kr.size());
   khat_t.resize(2 * kt.size());
-  for (size_t k = 0; k < kr.size(); ++k) {
-    khat_r[2 * k] = kr[k].real(); khat_r[2 * k + 1] = kr[k].imag();
-    khat_t[2 * k] = kt[k].real(); khat_t[2 * k + 1] = kt[k].imag();
-  }
+  for (int i = 0; i < NR2; ++i)
+    for (int j = 0; j < NS; ++j) {
+      const size_t k = (size_t)i * NS + j, q = (size_t)j * NR2 + i;
+      khat_r[2 * q] = kr[k].real(); khat_r[2 * q + 1] = kr[k].imag();
+      khat_t[2 * q] = kt[k].real(); khat_t[2 * q + 1] = kt[k].imag();
+    }
   // radial acceleration of the initial density (ComputeFFTKernel,
   // ComputeConvolutionKernel, ComputeSgAccKernel; Kernels.cu:949-1021)
   std::vector<cd> sr((size_t)NR2 * NS, 0.0);

@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(1024) k_sg_cols(Dev<real> D) {
     const cxd sr = cx_scale(cx_add(zz, zc), 0.5);
     const cxd d = cx_sub(zz, zc);
     const cxd st = cxd{0.5 * d.y, -0.5 * d.x};   // d / (2i)
-    const long long kk = (long long)ki * ns + col;
+    const long long kk = (long long)col * n + ki;   // the kernel transforms are stored by column
     const cxd gr = cx_mul(kr[kk], sr), gt = cx_mul(kt[kk], st);
     o[u] = cx_scale(cx_add(gr, cx_i(gt, 1.0)), -kG);
   }
