@@ -40,6 +40,7 @@ struct fgx_server {
     size_t footprint = 0;
     std::vector<int> refused;   // batches whose slot set-up refused the request
     std::vector<double> final_fields[FGX_NFIELDS];
+    std::string error;   // the library's message where status is an error
   };
   struct Slot {
     int req = -1;         // request held, -1 = free (frozen)
@@ -102,12 +103,15 @@ void worker(Server* S, Live* L, std::vector<fgx_params> ps, std::vector<fgx_plan
     if (b && status == FGX_OK) status = fgx_batch_info(b, k, &di);
     std::vector<double> keep[4];
     if (b && status == FGX_OK && S->cfg.keep_final)
-      for (int f = 0; f < 4; ++f) {
+      for (int f = 0; f < 4 && status == FGX_OK; ++f) {
         if (fields[f] == FGX_ENERGY && !r.p.adiabatic) continue;
         keep[f].resize((size_t)(r.p.nrad + 1) * r.p.nsec);
-        if (fgx_batch_read_field(b, k, fields[f], FGX_F64, keep[f].data(), keep[f].size())) status = FGX_EDEVICE;
+        status = fgx_batch_read_field(b, k, fields[f], FGX_F64, keep[f].data(), keep[f].size());
       }
+    // the message of this thread's last error (errors are kept per thread)
+    const std::string msg = status == FGX_OK ? std::string() : std::string(fgx::last_error());
     std::lock_guard<std::mutex> g(S->mu);
+    S->reqs[rid].error = msg;
     fgx_request_info& in = S->reqs[rid].info;
     in.finish = S->now();
     in.status = status;
@@ -422,10 +426,11 @@ int fgx_server_run(fgx_server* s) {
     s->cv.wait_for(lk, std::chrono::duration<double>(dt));
   }
   lk.unlock();
-  int rc = FGX_OK;
-  for (auto& r : s->reqs)
-    if (r.info.status != FGX_OK && rc == FGX_OK) rc = r.info.status;
-  return rc;
+  // the first failed request's status and message (set in its worker)
+  for (size_t k = 0; k < s->reqs.size(); ++k)
+    if (s->reqs[k].info.status != FGX_OK)
+      return fgx::set_error(s->reqs[k].info.status, "request %zu: %s", k, s->reqs[k].error.c_str());
+  return FGX_OK;
 }
 
 int fgx_server_stats_get(fgx_server* s, fgx_server_stats* out) {

@@ -8,10 +8,15 @@
 // Ns = product of the earlier radices maps x[j + r*N/R] (r < R), twiddled by
 // exp(sign*2*pi*i*r*(j mod Ns)/(Ns*R)), through an R-point DFT to
 // y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].  The device kernels use the same
-// factorisation.
+// factorisation and the same twiddle table, exp(-2*pi*i*m/n) (conjugated for
+// the inverse).  The 2-D transforms and the Green's functions of a large grid
+// (2048 x 3072 points at NRAD=1024) run over the host's cores: lines and rows
+// are independent, so the result does not depend on the thread count.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "fgx_internal.h"
@@ -55,8 +60,29 @@ void dft_small(cd* v, int R, int sign) {
   }
 }
 
-// one line of length n (contiguous in a), unnormalised, sign -1 forward
-void fft_line(cd* a, cd* b, int n, const int* fac, int nf, int sign) {
+// f(lo, hi) over [0, n) in contiguous chunks on up to 16 threads
+template <class F>
+void parallel_for(int n, F f) {
+  const int nt = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), n / 16}));
+  if (nt <= 1) { f(0, n); return; }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) th.emplace_back(f, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
+  for (std::thread& x : th) x.join();
+}
+
+// twiddle table of a length-n transform: w[m] = exp(-2*pi*i*m/n)
+std::vector<cd> twiddles(int n) {
+  std::vector<cd> w(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = 2.0 * M_PI * (double)m / (double)n;
+    w[m] = cd(std::cos(a), -std::sin(a));
+  }
+  return w;
+}
+
+// one line of length n (contiguous in a), unnormalised, sign -1 forward;
+// tw = twiddles(n)
+void fft_line(cd* a, cd* b, int n, const int* fac, int nf, int sign, const cd* tw) {
   cd* src = a;
   cd* dst = b;
   int Ns = 1;
@@ -64,11 +90,13 @@ void fft_line(cd* a, cd* b, int n, const int* fac, int nf, int sign) {
   for (int st = 0; st < nf; ++st) {
     const int R = fac[st];
     const int m = n / R;
+    const int ts = n / (Ns * R);
     for (int j = 0; j < m; ++j) {
       const int k = j % Ns;
-      for (int r = 0; r < R; ++r) {
-        const double ang = sign * 2.0 * M_PI * (double)(r * k) / (double)(Ns * R);
-        v[r] = src[j + r * m] * cd(std::cos(ang), std::sin(ang));
+      v[0] = src[j];
+      for (int r = 1; r < R; ++r) {
+        const cd w = tw[r * k * ts];   // exp(-2 pi i r k / (Ns R))
+        v[r] = src[j + r * m] * (sign > 0 ? std::conj(w) : w);
       }
       dft_small(v, R, sign);
       const int base = (j / Ns) * Ns * R + k;
@@ -80,19 +108,47 @@ void fft_line(cd* a, cd* b, int n, const int* fac, int nf, int sign) {
   if (src != a) std::memcpy(a, src, sizeof(cd) * n);
 }
 
-// in-place 2-D transform of a rows x cols array (row-major)
-void fft2d(std::vector<cd>& x, int rows, int cols, int sign) {
+// in-place 2-D transform of a rows x cols array (row-major); only rows
+// [0, rows_in) of the input are non-zero (the row pass skips the others,
+// whose transform is zero) and only rows [0, rows_out) of the output are kept
+// (the inverse row pass skips the others)
+void fft2d(std::vector<cd>& x, int rows, int cols, int sign, int rows_in, int rows_out) {
   int fr[FGX_SG_MAXFAC], fc[FGX_SG_MAXFAC], nr = 0, nc = 0;
   sg_factor(rows, fr, &nr);
   sg_factor(cols, fc, &nc);
-  std::vector<cd> line(std::max(rows, cols)), tmp(std::max(rows, cols));
-  for (int i = 0; i < rows; ++i) fft_line(&x[(size_t)i * cols], tmp.data(), cols, fc, nc, sign);
-  for (int j = 0; j < cols; ++j) {
-    for (int i = 0; i < rows; ++i) line[i] = x[(size_t)i * cols + j];
-    fft_line(line.data(), tmp.data(), rows, fr, nr, sign);
-    for (int i = 0; i < rows; ++i) x[(size_t)i * cols + j] = line[i];
+  const std::vector<cd> twr = twiddles(rows), twc = twiddles(cols);
+  // forward: rows, then columns; inverse: columns, then rows (the kept ones)
+  auto row_pass = [&](int nrows) {
+    parallel_for(nrows, [&](int lo, int hi) {
+      std::vector<cd> tmp(cols);
+      for (int i = lo; i < hi; ++i) fft_line(&x[(size_t)i * cols], tmp.data(), cols, fc, nc, sign, twc.data());
+    });
+  };
+  // columns eight at a time, so that each row's cache line is read and
+  // written once per group rather than once per column
+  constexpr int G = 8;
+  auto col_pass = [&]() {
+    parallel_for((cols + G - 1) / G, [&](int lo, int hi) {
+      std::vector<cd> line((size_t)G * rows), tmp(rows);
+      for (int g = lo; g < hi; ++g) {
+        const int j0 = g * G, nj = std::min(G, cols - j0);
+        for (int i = 0; i < rows; ++i)
+          for (int q = 0; q < nj; ++q) line[(size_t)q * rows + i] = x[(size_t)i * cols + j0 + q];
+        for (int q = 0; q < nj; ++q) fft_line(&line[(size_t)q * rows], tmp.data(), rows, fr, nr, sign, twr.data());
+        for (int i = 0; i < rows; ++i)
+          for (int q = 0; q < nj; ++q) x[(size_t)i * cols + j0 + q] = line[(size_t)q * rows + i];
+      }
+    });
+  };
+  if (sign < 0) {
+    row_pass(rows_in);
+    col_pass();
+  } else {
+    col_pass();
+    row_pass(rows_out);
   }
 }
+void fft2d(std::vector<cd>& x, int rows, int cols, int sign) { fft2d(x, rows, cols, sign, rows, rows); }
 
 }  // namespace
 
@@ -113,35 +169,46 @@ int sg_host_setup(HostDisk& h, std::vector<double>& khat_r, std::vector<double>&
   const double eps = h.sg_eps;
   // Green's functions on the log grid (compute_kernel, SgMain.cu:49-81), FP64
   std::vector<cd> kr((size_t)NR2 * NS), kt((size_t)NR2 * NS);
-  for (int i = 0; i < NR2; ++i) {
-    const double u = (i < NR) ? std::log(h.radii[i] / h.radii[0]) : -std::log(h.radii[NR2 - i] / h.radii[0]);
-    for (int j = 0; j < NS; ++j) {
-      const double th = 2.0 * M_PI * (double)j / (double)NS;
-      const double base = std::pow(eps * eps * std::exp(u) + 2.0 * (std::cosh(u) - std::cos(th)), -1.5);
-      kr[(size_t)i * NS + j] = (1.0 + eps * eps - std::cos(th) * std::exp(-u)) * base;
-      kt[(size_t)i * NS + j] = std::sin(th) * base;
+  parallel_for(NR2, [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      const double u = (i < NR) ? std::log(h.radii[i] / h.radii[0]) : -std::log(h.radii[NR2 - i] / h.radii[0]);
+      for (int j = 0; j < NS; ++j) {
+        const double th = 2.0 * M_PI * (double)j / (double)NS;
+        const double base = std::pow(eps * eps * std::exp(u) + 2.0 * (std::cosh(u) - std::cos(th)), -1.5);
+        kr[(size_t)i * NS + j] = (1.0 + eps * eps - std::cos(th) * std::exp(-u)) * base;
+        kt[(size_t)i * NS + j] = std::sin(th) * base;
+      }
     }
-  }
+  });
   fft2d(kr, NR2, NS, -1);
   fft2d(kt, NR2, NS, -1);
   // stored by column (entry (i, j) at j * 2NRAD + i): the device reads them a
   // column pair at a time
   khat_r.resize(2 * kr.size());
   khat_t.resize(2 * kt.size());
-  for (int i = 0; i < NR2; ++i)
-    for (int j = 0; j < NS; ++j) {
-      const size_t k = (size_t)i * NS + j, q = (size_t)j * NR2 + i;
-      khat_r[2 * q] = kr[k].real(); khat_r[2 * q + 1] = kr[k].imag();
-      khat_t[2 * q] = kt[k].real(); khat_t[2 * q + 1] = kt[k].imag();
+  parallel_for((NS + 7) / 8, [&](int lo, int hi) {   // eight columns at a time
+    for (int g = lo; g < hi; ++g) {
+      const int j0 = 8 * g, nj = std::min(8, NS - j0);
+      for (int i = 0; i < NR2; ++i)
+        for (int jj = 0; jj < nj; ++jj) {
+          const size_t k = (size_t)i * NS + j0 + jj, q = (size_t)(j0 + jj) * NR2 + i;
+          khat_r[2 * q] = kr[k].real(); khat_r[2 * q + 1] = kr[k].imag();
+          khat_t[2 * q] = kt[k].real(); khat_t[2 * q + 1] = kt[k].imag();
+        }
     }
+  });
   // radial acceleration of the initial density (ComputeFFTKernel,
   // ComputeConvolutionKernel, ComputeSgAccKernel; Kernels.cu:949-1021)
   std::vector<cd> sr((size_t)NR2 * NS, 0.0);
-  for (int i = 0; i < NR; ++i)
-    for (int j = 0; j < NS; ++j) sr[(size_t)i * NS + j] = h.dens[(size_t)i * NS + j] * std::sqrt(h.rmed[i] / h.rmed[0]);
-  fft2d(sr, NR2, NS, -1);
-  for (size_t k = 0; k < sr.size(); ++k) sr[k] = -kG * kr[k] * sr[k];
-  fft2d(sr, NR2, NS, +1);
+  parallel_for(NR, [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i)
+      for (int j = 0; j < NS; ++j) sr[(size_t)i * NS + j] = h.dens[(size_t)i * NS + j] * std::sqrt(h.rmed[i] / h.rmed[0]);
+  });
+  fft2d(sr, NR2, NS, -1, NR, NR2);   // rows >= NRAD of the input are zero
+  parallel_for(NR2, [&](int lo, int hi) {
+    for (size_t k = (size_t)lo * NS; k < (size_t)hi * NS; ++k) sr[k] = -kG * kr[k] * sr[k];
+  });
+  fft2d(sr, NR2, NS, +1, NR2, NR);   // rows < NRAD of the result are read
   const double norm = h.sg_rstep * h.sg_tstep / ((double)NR2 * (double)NS);
   axisg.assign(NR + 1, 0.0);
   for (int i = 0; i < NR; ++i) {

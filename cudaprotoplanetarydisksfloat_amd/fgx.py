@@ -38,6 +38,15 @@ def _load():
         raise ImportError(
             f"native library {_LIBPATH} is missing: build it with "
             "`python -m cudaprotoplanetarydisksfloat_amd.build`")
+    # One HIP runtime per process: PyTorch-ROCm brings its own copy of the
+    # HIP and HSA runtime libraries.  Loaded after them, libfgx.so shares
+    # them; loaded before them (this module imported ahead of torch), the
+    # first GPU use by torch leaves libfgx.so's runtime with no device
+    # (hipGetDeviceCount fails).  So torch, where installed, is loaded first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return ctypes.CDLL(_LIBPATH)
 
 

@@ -57,3 +57,21 @@ def test_autotune_without_device_fails_cleanly(tmp_path):
     assert not cache.exists()   # nothing recorded for a failed tuning
     with pytest.raises(FgxError):
         autotune([p], [pl], n_dt=0)
+
+
+def test_server_without_device_reports_the_workers_message():
+    """A request that fails in the server's worker thread ends with that
+    thread's status and message: fgx_server_run returns the first failed
+    request's status with the library's text for it, not an empty one."""
+    from cudaprotoplanetarydisksfloat_amd import F32, FgxError, Server, configs, device_count
+    if device_count() > 0:
+        pytest.skip("a device is present")
+    p, pl = configs.load(*configs.plain_disk(nrad=16, nsec=32))
+    srv = Server(precision=F32, hbm_budget=1 << 30, max_batch_disks=2, snapshot_every=0)
+    srv.submit(p, pl, 1, 0.0)
+    with pytest.raises(FgxError) as e:
+        srv.run()
+    assert e.value.code == -2
+    assert "request 0: no HIP device available" in str(e.value)
+    assert srv.request(0).status == -2
+    srv.close()

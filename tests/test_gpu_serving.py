@@ -103,3 +103,30 @@ def test_continuous_batching_refills_slots(gpu):
             assert np.array_equal(srv.field(rid, name), alone.field(name)), (rid, name)
         alone.close()
     srv.close()
+
+
+def test_package_imported_before_torch(gpu):
+    """The package loads PyTorch's HIP runtime before its own library, so
+    that a program importing the package first and initialising the GPU
+    through torch afterwards still reaches the device (a batch and a served
+    request), in a fresh process."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from cudaprotoplanetarydisksfloat_amd import Batch, Server, F32, configs\n"
+        "import torch\n"
+        "torch.cuda.get_device_properties(0)\n"
+        "par, cfg = configs.plain_disk(nrad=32, nsec=96)\n"
+        "p, pl = configs.load(par, cfg)\n"
+        "Batch(p, pl, F32).close()\n"
+        "srv = Server(precision=F32, hbm_budget=1 << 30, max_batch_disks=2, snapshot_every=0)\n"
+        "srv.submit(p, pl, 2, 0.0)\n"
+        "srv.run()\n"
+        "assert srv.request(0).status == 0\n"
+        "srv.close()\n"
+        "print('ok')\n" % root)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
