@@ -69,6 +69,21 @@ static int noalloc_scratch(int dev, double** out) {
                        __FILE__, __LINE__);                                                 \
   } while (0)
 
+// A kernel's dynamic-LDS limit (hipFuncAttributeMaxDynamicSharedMemorySize)
+// is process-wide, and batches of different grids may be created at the same
+// time (the serving loop's workers): it is only ever raised, under a lock, so
+// that a batch setting a smaller limit cannot undercut one that needs more.
+static std::mutex g_lds_attr_mu;
+static int raise_dynamic_lds(const void* f, size_t bytes) {
+  static std::unordered_map<const void*, size_t> limit;
+  std::lock_guard<std::mutex> lk(g_lds_attr_mu);
+  size_t& cur = limit[f];
+  if (bytes <= cur || bytes <= 48 * 1024) return FGX_OK;
+  HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  cur = bytes;
+  return FGX_OK;
+}
+
 struct BatchBase {
   virtual ~BatchBase() {}
   virtual int step(int n_dt) = 0;
@@ -854,9 +869,7 @@ struct Batch : BatchBase {
     if (theta_shmem + 1024 > lds_max)
       return set_error(FGX_EPARAM, "NSEC=%d needs %zu B of LDS per ring in this precision; the device has %zu", fb_nsec,
                        theta_shmem, lds_max);
-    if (theta_shmem > 48 * 1024)
-      HIPCHK(hipFuncSetAttribute((const void*)k_trans_theta<real>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)theta_shmem));
+    LCHK(raise_dynamic_lds((const void*)k_trans_theta<real>, theta_shmem));
     if (sg) {
       // self-gravity transforms: two rows of nsec, four columns of 2*nrad (FP64 complex)
       sg_shm_row = 2 * (size_t)max_nsec * sizeof(cxd);
@@ -868,7 +881,7 @@ struct Batch : BatchBase {
                            (const void*)k_sg_cols<real>};
       const size_t sz[3] = {sg_shm_row, sg_shm_row, sg_shm_col};
       for (int q = 0; q < 3; ++q)
-        if (sz[q] > 48 * 1024) HIPCHK(hipFuncSetAttribute(kf[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)sz[q]));
+        LCHK(raise_dynamic_lds(kf[q], sz[q]));
       // the SG field of the initial density (read_field before the first step)
       D.ignore_active = 1;
       LCHK(self_gravity());

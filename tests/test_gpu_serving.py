@@ -130,3 +130,24 @@ def test_package_imported_before_torch(gpu):
         "print('ok')\n" % root)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=110)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
+
+
+def test_batches_of_different_lds_needs(gpu):
+    """Two batches whose self-gravity column pass needs different amounts of
+    LDS (128 KB at NRAD=1024, 64 KB at NRAD=512), created one after the other
+    as the serving loop's workers may: the second creation does not lower the
+    first one's limit, and the first batch still steps, bit-identical to a
+    batch of the same disk created alone."""
+    big = configs.load(*configs.sg_disk(nrad=1024, nsec=96))
+    small = configs.load(*configs.sg_disk(nrad=512, nsec=96))
+    a = Batch(*big, F32)
+    b = Batch(*small, F32)
+    a.step(1)
+    b.step(1)
+    fa = a.field("dens")
+    a.close()
+    b.close()
+    solo = Batch(*big, F32)
+    solo.step(1)
+    assert np.array_equal(fa, solo.field("dens"))
+    solo.close()
