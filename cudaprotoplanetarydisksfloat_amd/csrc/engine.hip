@@ -288,6 +288,15 @@ struct Batch : BatchBase {
   size_t F = 0, R = 0, S = 0;
   int* h_summary = nullptr;  // pinned
   int visc_dynamic = 0, alpha = 0, need_force = 1;
+  // Launches of k_visc_profile that would rewrite what D.csmean and D.visc
+  // already hold are skipped.  Without LambdaDoubling the profile is a
+  // function of the ring means of the sound speed alone, and the sound speed
+  // changes only where k_sound_speed runs.  visc_fresh: D.visc is the profile
+  // of D.csmean; cs_dirty: D.csmean may differ from the ring means of D.cs;
+  // visc_time: some disk's profile depends on time.  Every step() call starts
+  // them over, so changes made between calls are always seen.
+  bool visc_fresh = false, cs_dirty = true, visc_time = true;
+  bool visc_skip = true;   // FGX_VISC_SKIP=0 launches every one
   bool accretion = false;       // some disk's planets accrete (StubbedPhysics)
   bool outer_source = false;    // some disk has the outer mass source (StubbedPhysics)
   size_t theta_shmem = 0;
@@ -461,6 +470,7 @@ struct Batch : BatchBase {
   int substep_graphed() {
     const Dev<real> D0 = D;
     const int pp0 = plan_pos;
+    const bool vf0 = visc_fresh, cd0 = cs_dirty;
     const HostTiles t0 = tiles_save();
     launch_mode = 1;
     rec_hash = 1469598103934665603ull;
@@ -471,8 +481,9 @@ struct Batch : BatchBase {
     if (it == graph_cache.end()) {
       const Dev<real> D1 = D;
       const int pp1 = plan_pos;
+      const bool vf1 = visc_fresh, cd1 = cs_dirty;
       const HostTiles t1 = tiles_save();
-      D = D0; plan_pos = pp0; tiles_restore(t0);
+      D = D0; plan_pos = pp0; visc_fresh = vf0; cs_dirty = cd0; tiles_restore(t0);
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       r = substep();
@@ -487,7 +498,8 @@ struct Batch : BatchBase {
       it = graph_cache.emplace(rec_hash, x).first;
       ++graph_builds;
       // the capture advanced the host state as the recording did
-      if (plan_pos != pp1 || std::memcmp(&D, &D1, sizeof D) != 0 || tiles_save().st != t1.st)
+      if (plan_pos != pp1 || visc_fresh != vf1 || cs_dirty != cd1 || std::memcmp(&D, &D1, sizeof D) != 0 ||
+          tiles_save().st != t1.st)
         return set_error(FGX_EDEVICE, "substep capture changed the host state differently from its recording");
     } else {
       ++graph_hits;
@@ -658,6 +670,7 @@ struct Batch : BatchBase {
     // fused source pass: on by default, FGX_FUSED_SOURCE=0 keeps the
     // separate kernels; FGX_SOURCE_RINGS sets the strip height
     if (const char* e = std::getenv("FGX_FUSED_SOURCE")) fused_source = std::atoi(e) != 0;
+    if (const char* e = std::getenv("FGX_VISC_SKIP")) visc_skip = std::atoi(e) != 0;
     if (tl && tl->source_rings > 0) src_rings = tl->source_rings;
     else if (const char* e = std::getenv("FGX_SOURCE_RINGS")) src_rings = std::atoi(e);
     src_rings = std::min(std::max(src_rings, 1), FGX_SRC_RINGS_MAX);   // k_source stages rs + 8 rings
@@ -1123,10 +1136,21 @@ struct Batch : BatchBase {
     LCHK(launch("sg_rows", 20 * r + 2, k_sg_rows_inv<real>, n_rtiles, trow, sg_shm_row, D));
     return FGX_OK;
   }
+  int visc_profile(double bytes, int update_csmean) {
+    if (visc_skip && !visc_time && visc_fresh && (!update_csmean || !cs_dirty)) return FGX_OK;
+    LCHK(launch("visc_profile", bytes, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, update_csmean, alpha));
+    visc_fresh = true;
+    if (update_csmean) cs_dirty = false;
+    return FGX_OK;
+  }
+  int sound_speed() {
+    cs_dirty = true;
+    return launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D);
+  }
   int boundary() {
     if (D.open_inner) LCHK(launch("open_boundary", 0, k_open_boundary<real>, nd, FGX_TPB, 0, D));
     if (D.nonreflecting) {
-      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
+      if (D.adiabatic) LCHK(sound_speed());
       LCHK(launch("nonreflecting", 0, k_nonreflecting<real>, nd, FGX_TPB, 0, D));
     }
     if (D.evanescent) LCHK(launch("evanescent", D.adiabatic ? 8 : 6, k_evanescent<real>, n_stiles, FGX_TPB, 0, D));
@@ -1149,13 +1173,12 @@ struct Batch : BatchBase {
       LCHK(launch("substep_setup", 0, k_substep_setup<real>, nd, FGX_TPB, 0, D, 0));
     }
     if (D.corotating) LCHK(launch("correct_vtheta", 2, k_correct_vtheta<real>, n_stiles, FGX_TPB, 0, D));
-    if (visc_dynamic) LCHK(launch("visc_profile", 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 0, alpha));
+    if (visc_dynamic) LCHK(visc_profile(0, 0));
     LCHK(boundary());
     if (fused_source) {
       // the viscosity profile only depends on the sound speed, which the
       // source stages do not change, so it can be formed first
-      if (D.viscous && visc_dynamic)
-        LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
+      if (D.viscous && visc_dynamic) LCHK(visc_profile(ad ? 1 : 0, 1));
       if (D.self_gravity) {
         // the self-gravity field only depends on the density, which the
         // source stages read but do not change; the pass adds it after
@@ -1177,7 +1200,7 @@ struct Batch : BatchBase {
         LCHK(launch("sg_update", 6, k_sg_update<real>, n_stiles, FGX_TPB, 0, D));
       }
       if (D.viscous) {
-        if (visc_dynamic) LCHK(launch("visc_profile", ad ? 1 : 0, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
+        if (visc_dynamic) LCHK(visc_profile(ad ? 1 : 0, 1));
         LCHK(launch("visc_tau", 6, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vradint,
                     (const real*)D.vthetaint, 0));
       }
@@ -1186,7 +1209,7 @@ struct Batch : BatchBase {
     }
     LCHK(boundary());
     if (ad) {
-      if (visc_dynamic) LCHK(launch("visc_profile", 1, k_visc_profile<real>, n_rtiles, FGX_TPB, 0, D, 1, alpha));
+      if (visc_dynamic) LCHK(visc_profile(1, 1));
       LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vrad, (const real*)D.vtheta, 1));
       LCHK(launch("substep3", 9, k_substep3<real>, n_stiles, FGX_TPB, 0, D));
     }
@@ -1210,10 +1233,14 @@ struct Batch : BatchBase {
       act0[d] = !frozen_h[d] && !health_h[d];
       n_act0 += act0[d];
     }
+    visc_fresh = false;
+    cs_dirty = true;
+    visc_time = false;
+    for (int d = 0; d < nd; ++d) visc_time |= dch[d].lambdadoubling != 0.0;
     for (int k = 0; k < n_dt; ++k) {
       LCHK(launch("begin_dt", 0, k_begin_dt<real>, (nd + 63) / 64, 64, 0, D));
       if (compact_on && n_act0 < nd) LCHK(lists_narrow(act0.data()));
-      if (D.adiabatic) LCHK(launch("sound_speed", 3, k_sound_speed<real>, n_stiles, FGX_TPB, 0, D));
+      if (D.adiabatic) LCHK(sound_speed());
       int plan = 0;
       // the first substeps of the DT run every disk that is not frozen
       fine_plan.assign(1, (double)covered_cells < fine_cells);

@@ -224,6 +224,38 @@ def test_fused_source_equals_separate_kernels(gpu, monkeypatch, case, precision)
         assert np.array_equal(a, b), (case, n, float(np.abs(a - b).max()))
 
 
+@pytest.mark.parametrize("precision", [F32, F64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["adiabatic_evanescent", "adiabatic_cooling_open", "nonreflecting_adiabatic",
+                                  "lambda_doubling"])
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_viscosity_profile_reuse(gpu, monkeypatch, case, precision, fused):
+    """The alpha viscosity profile of an adiabatic disk is re-formed only
+    where the sound speed it averages has changed (each DT's start, a
+    non-reflecting boundary) or where it depends on time (LambdaDoubling);
+    every field ends bit-identical to re-forming it at each of its places in
+    the substep, over several CFL-driven substeps per DT and two step calls."""
+    if case == "lambda_doubling":
+        par, cfg, keys = _case("adiabatic_evanescent")
+        keys = dict(keys, LAMBDADOUBLING=0.1, CAVITYRATIO=0.5, CAVITYRADIUS=1.0, CAVITYWIDTH=1.0)
+    else:
+        par, cfg, keys = _case(case)
+    p, pl = configs.load(par, cfg, **keys)
+    monkeypatch.setenv("FGX_FUSED_SOURCE", fused)
+    out = {}
+    for skip in ("1", "0"):
+        monkeypatch.setenv("FGX_VISC_SKIP", skip)
+        b = Batch(p, pl, precision)
+        b.step(2)
+        b.step(1)
+        out[skip] = ({n: b.field(n) for n in ("dens", "vrad", "vtheta", "energy")}, b.info().substeps,
+                     b.ring("viscosity"))
+        b.close()
+    assert out["1"][1] == out["0"][1] and out["1"][1] > 3, (out["1"][1], out["0"][1])
+    for n in out["0"][0]:
+        assert np.array_equal(out["1"][0][n], out["0"][0][n]), (case, n)
+    assert np.array_equal(out["1"][2], out["0"][2])
+
+
 @pytest.mark.parametrize("case", ["plain", "adiabatic_evanescent", "open"])
 def test_fine_tiling_equals_coarse(gpu, monkeypatch, case):
     """The short-strip tiling used when few disks are still active gives the
