@@ -1210,8 +1210,14 @@ struct Batch : BatchBase {
     LCHK(boundary());
     if (ad) {
       if (visc_dynamic) LCHK(visc_profile(1, 1));
-      LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vrad, (const real*)D.vtheta, 1));
-      LCHK(launch("substep3", 9, k_substep3<real>, n_stiles, FGX_TPB, 0, D));
+      if (fused_source) {
+        // the stresses stay in registers (bit-identical to the two kernels)
+        LCHK(launch("substep3", 11, k_substep3_tau<real>, n_stiles, FGX_TPB, 0, D));
+      } else {
+        LCHK(launch("visc_tau", 7, k_visc_tau<real>, n_stiles, FGX_TPB, 0, D, (const real*)D.vrad,
+                    (const real*)D.vtheta, 1));
+        LCHK(launch("substep3", 9, k_substep3<real>, n_stiles, FGX_TPB, 0, D));
+      }
     }
     if (D.advect_label)
       LCHK(launch("trans_rad", ad ? 12 : 10, k_trans_rad_strip<real, true>, n_stiles, FGX_TPB, 0, D, stiles, strip_rings));

@@ -1124,39 +1124,71 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep1(Dev<real> D) {
 // strain rates stay in registers, only the three stresses (and the velocity
 // divergence when an energy equation needs it) are stored
 template <class real>
+struct Stress { real rr, pp, rp, divv, sig; };
+// a value the compiler must take as it is: no product or sum on either side of
+// it is fused with one on the other (as if it had been stored and read back),
+// so that a kernel that keeps the stresses in registers rounds exactly like
+// the pair of kernels that passes them through memory
+template <class T>
+__device__ __forceinline__ T opaque(T x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+template <class real>
+__device__ __forceinline__ Stress<real> opaque(Stress<real> t) {
+  t.rr = opaque(t.rr); t.pp = opaque(t.pp); t.rp = opaque(t.rp); t.divv = opaque(t.divv); t.sig = opaque(t.sig);
+  return t;
+}
+// the stresses and the velocity divergence of cell (i, j) from the velocities
+// vr, vt and the density (sig: the cell's whole density)
+template <class real>
+__device__ __forceinline__ Stress<real> stress_at(const Dev<real>& D, const DiskConst& c, int rb, long long fb,
+                                                  const real* __restrict__ vr, const real* __restrict__ vt, int i,
+                                                  int j) {
+  const int ns = c.nsec;
+  const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
+  const real invdphi = (real)((double)ns / (2.0 * kPI));
+  const real irm = RT(invrmed)[rb + i];
+  const real vr0 = AT(vr, i, j), vr1 = AT(vr, i + 1, j);
+  const real vt0 = AT(vt, i, j), vtp = AT(vt, i, jp);
+  const real drr = (vr1 - vr0) * RT(invdiffrsup)[rb + i];
+  const real dpp = (vtp - vt0) * invdphi * irm + (real)0.5 * (vr1 + vr0) * irm;
+  real divv = (vr1 * RT(rsup)[rb + i] - vr0 * RT(rinf)[rb + i]) * RT(invdiffrsup)[rb + i] * irm;
+  divv += (vtp - vt0) * invdphi * irm;
+  const real nu = RT(visc)[rb + i];
+  const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
+  const real third = (real)(1.0 / 3.0);
+  Stress<real> t;
+  t.rr = (real)2 * sig * nu * (drr - third * divv);
+  t.pp = (real)2 * sig * nu * (dpp - third * divv);
+  t.divv = divv;
+  t.sig = sig;
+  if (i > 0) {
+    // the angular velocity difference between rings is formed in FP64 from
+    // the residuals and the reference profile
+    const double w0 = (RT(vref)[rb + i] + (double)vt0) * RT(invrmed_d)[rb + i];
+    const double w1 = (RT(vref)[rb + i - 1] + (double)AT(vt, i - 1, j)) * RT(invrmed_d)[rb + i - 1];
+    const real drp = (real)0.5 * ((real)(RT(rinf_d)[rb + i] * (w0 - w1) * RT(invdiffrmed_d)[rb + i]) +
+                                  (vr0 - AT(vr, i, jm)) * invdphi * RT(invrinf)[rb + i]);
+    // corner density: column 0 of ring i as the reference indexes it
+    // (Kernels.cu:876), or column j-1 with the quirk fix
+    const real sc = s_whole(D, rb, i, D.fix_taurp ? (real)AT(D.dens, i, jm) : (real)AT(D.dens, i, 0));
+    const real sm_ = s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, j)), sml = s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, jm));
+    t.rp = (real)(2.0 * 0.25) * (sig + sm_ + sc + sml) * nu * drp;
+  } else {
+    t.rp = 0;
+  }
+  return t;
+}
+template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_visc_tau(Dev<real> D, const real* __restrict__ vr,
                                                       const real* __restrict__ vt, int write_divv) {
   FGX_STRIP_BEGIN {
-    const int jp = (j + 1 == ns) ? 0 : j + 1, jm = (j == 0) ? ns - 1 : j - 1;
-    const real invdphi = (real)((double)ns / (2.0 * kPI));
-    const real irm = RT(invrmed)[rb + i];
-    const real vr0 = AT(vr, i, j), vr1 = AT(vr, i + 1, j);
-    const real vt0 = AT(vt, i, j), vtp = AT(vt, i, jp);
-    const real drr = (vr1 - vr0) * RT(invdiffrsup)[rb + i];
-    const real dpp = (vtp - vt0) * invdphi * irm + (real)0.5 * (vr1 + vr0) * irm;
-    real divv = (vr1 * RT(rsup)[rb + i] - vr0 * RT(rinf)[rb + i]) * RT(invdiffrsup)[rb + i] * irm;
-    divv += (vtp - vt0) * invdphi * irm;
-    const real nu = RT(visc)[rb + i];
-    const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
-    const real third = (real)(1.0 / 3.0);
-    AT(D.taurr, i, j) = (real)2 * sig * nu * (drr - third * divv);
-    AT(D.taupp, i, j) = (real)2 * sig * nu * (dpp - third * divv);
-    if (write_divv) AT(D.divv, i, j) = divv;
-    if (i > 0) {
-      // the angular velocity difference between rings is formed in FP64 from
-      // the residuals and the reference profile
-      const double w0 = (RT(vref)[rb + i] + (double)vt0) * RT(invrmed_d)[rb + i];
-      const double w1 = (RT(vref)[rb + i - 1] + (double)AT(vt, i - 1, j)) * RT(invrmed_d)[rb + i - 1];
-      const real drp = (real)0.5 * ((real)(RT(rinf_d)[rb + i] * (w0 - w1) * RT(invdiffrmed_d)[rb + i]) +
-                                    (vr0 - AT(vr, i, jm)) * invdphi * RT(invrinf)[rb + i]);
-      // corner density: column 0 of ring i as the reference indexes it
-      // (Kernels.cu:876), or column j-1 with the quirk fix
-      const real sc = s_whole(D, rb, i, D.fix_taurp ? (real)AT(D.dens, i, jm) : (real)AT(D.dens, i, 0));
-      const real sm_ = s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, j)), sml = s_whole(D, rb, i - 1, (real)AT(D.dens, i - 1, jm));
-      AT(D.taurp, i, j) = (real)(2.0 * 0.25) * (sig + sm_ + sc + sml) * nu * drp;
-    } else {
-      AT(D.taurp, i, j) = 0;
-    }
+    const Stress<real> t = stress_at(D, c, rb, fb, vr, vt, i, j);
+    AT(D.taurr, i, j) = t.rr;
+    AT(D.taupp, i, j) = t.pp;
+    if (write_divv) AT(D.divv, i, j) = t.divv;
+    AT(D.taurp, i, j) = t.rp;
   }
 }
 
@@ -1239,44 +1271,88 @@ __global__ void __launch_bounds__(FGX_TPB) k_substep2(Dev<real> D) {
 // ================================================================ Substep3
 // Substep3Kernel + Substep3Kernel2 (Kernels.cu:91-139)
 template <class real>
-__device__ __forceinline__ real qplus_at(const Dev<real>& D, const DiskConst& c, int rb, long long fb, int ns, int i, int j) {
-  const real nu = RT(visc)[rb + i];
-  if (nu == (real)0) return 0;
-  const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
-  const real trr = AT(D.taurr, i, j), trp = AT(D.taurp, i, j), tpp = AT(D.taupp, i, j), dv = AT(D.divv, i, j);
+__device__ __forceinline__ real qplus_of(real nu, real sig, real trr, real trp, real tpp, real dv) {
   real q = (real)0.5 / nu / sig * (trr * trr + trp * trp + tpp * tpp);
   q += (real)(2.0 / 9.0) * nu * sig * dv * dv;
   return q;
 }
 template <class real>
+__device__ __forceinline__ real qplus_at(const Dev<real>& D, const DiskConst& c, int rb, long long fb, int ns, int i, int j) {
+  const real nu = RT(visc)[rb + i];
+  if (nu == (real)0) return 0;
+  const real sig = s_whole(D, rb, i, (real)AT(D.dens, i, j));
+  return qplus_of(nu, sig, (real)AT(D.taurr, i, j), (real)AT(D.taurp, i, j), (real)AT(D.taupp, i, j),
+                  (real)AT(D.divv, i, j));
+}
+// the same from the stresses formed in place (stress_at on vrad, vtheta)
+template <class real>
+__device__ __forceinline__ real qplus_here(const Dev<real>& D, const DiskConst& c, int rb, long long fb, int i, int j) {
+  const real nu = RT(visc)[rb + i];
+  if (nu == (real)0) return 0;
+  const Stress<real> t = opaque(stress_at(D, c, rb, fb, (const real*)D.vrad, (const real*)D.vtheta, i, j));
+  return qplus_of(nu, t.sig, t.rr, t.rp, t.pp, t.divv);
+}
+// the new energy of cell (i, j) from its heating qp and divergence dvv
+template <class real>
+__device__ __forceinline__ real substep3_energy(const Dev<real>& D, const DiskConst& c, const DiskState& s, int rb,
+                                                long long fb, int i, int j, real qp, real dvv) {
+  const int ns = c.nsec;
+  const real dt = (real)s.dt;
+  // on residuals of the energy profile e0: e_new - e0 = (num - e0 * den) / den
+  const real g1 = (real)c.adiabaticindex - (real)1;
+  const real eint = AT(D.energyint, i, j);
+  const real e0 = RT(energymed)[rb + i];
+  real en;
+  if (!D.cooling) {
+    const real gd = g1 * dt * dvv;
+    en = (dt * qp + eint - e0 * gd) / ((real)1 + gd);
+  } else {
+    // the density residual is Sigma - SigmaMed
+    const real sres = AT(D.dens, i, j), sm = RT(sigmamed)[rb + i], sig = sm + sres;
+    const real tc = RT(coolingtime)[rb + i];
+    const real num = e0 * dt * sres / sm + tc * eint + dt * tc * (qp - RT(qplusmed)[rb + i] * sig / sm) -
+                     e0 * g1 * dt * tc * dvv;
+    const real den = dt + tc + g1 * dt * tc * dvv;
+    en = num / den;
+  }
+  return en;
+}
+// ring 0's heating, extrapolated from rings 1 and 2 (Kernels.cu:91-139)
+template <class real>
+__device__ __forceinline__ real qplus_ring0(const Dev<real>& D, int rb, real q1, real q2) {
+  return q1 * (real)exp((double)log(q1 / q2) * log(RT(rmed_d)[rb] / RT(rmed_d)[rb + 1]) / log(RT(rmed_d)[rb + 1] / RT(rmed_d)[rb + 2]));
+}
+template <class real>
 __global__ void __launch_bounds__(FGX_TPB) k_substep3(Dev<real> D) {
   FGX_STRIP_BEGIN {
-    const real dt = (real)s.dt;
     real qp;
     if (i > 0) qp = qplus_at(D, c, rb, fb, ns, i, j);
     else if (RT(visc)[rb + nr - 1] != (real)0) {
       const real q1 = qplus_at(D, c, rb, fb, ns, 1, j), q2 = qplus_at(D, c, rb, fb, ns, 2, j);
-      qp = q1 * (real)exp((double)log(q1 / q2) * log(RT(rmed_d)[rb] / RT(rmed_d)[rb + 1]) / log(RT(rmed_d)[rb + 1] / RT(rmed_d)[rb + 2]));
+      qp = qplus_ring0(D, rb, q1, q2);
     } else qp = 0;
     AT(D.qplus, i, j) = qp;
-    // on residuals of the energy profile e0: e_new - e0 = (num - e0 * den) / den
-    const real g1 = (real)c.adiabaticindex - (real)1;
-    const real eint = AT(D.energyint, i, j), dvv = AT(D.divv, i, j);
-    const real e0 = RT(energymed)[rb + i];
-    real en;
-    if (!D.cooling) {
-      const real gd = g1 * dt * dvv;
-      en = (dt * qp + eint - e0 * gd) / ((real)1 + gd);
-    } else {
-      // the density residual is Sigma - SigmaMed
-      const real sres = AT(D.dens, i, j), sm = RT(sigmamed)[rb + i], sig = sm + sres;
-      const real tc = RT(coolingtime)[rb + i];
-      const real num = e0 * dt * sres / sm + tc * eint + dt * tc * (qp - RT(qplusmed)[rb + i] * sig / sm) -
-                       e0 * g1 * dt * tc * dvv;
-      const real den = dt + tc + g1 * dt * tc * dvv;
-      en = num / den;
-    }
-    AT(D.energy, i, j) = en;
+    AT(D.energy, i, j) = substep3_energy(D, c, s, rb, fb, i, j, qp, (real)AT(D.divv, i, j));
+  }
+}
+// Substep3 with its viscous stresses formed in place (ViscousTerms on the
+// new velocities + Substep3, Kernels.cu:91-139, 843-880): the operations of
+// k_visc_tau and k_substep3, with the stresses kept in registers instead of
+// stored and read back; ring 0 forms rings 1 and 2's for its extrapolation
+template <class real>
+__global__ void __launch_bounds__(FGX_TPB) k_substep3_tau(Dev<real> D) {
+  FGX_STRIP_BEGIN {
+    const Stress<real> t = opaque(stress_at(D, c, rb, fb, (const real*)D.vrad, (const real*)D.vtheta, i, j));
+    AT(D.divv, i, j) = t.divv;
+    real qp;
+    if (i > 0) {
+      const real nu = RT(visc)[rb + i];
+      qp = nu == (real)0 ? (real)0 : qplus_of(nu, t.sig, t.rr, t.rp, t.pp, t.divv);
+    } else if (RT(visc)[rb + nr - 1] != (real)0) {
+      qp = qplus_ring0(D, rb, qplus_here(D, c, rb, fb, 1, j), qplus_here(D, c, rb, fb, 2, j));
+    } else qp = 0;
+    AT(D.qplus, i, j) = qp;
+    AT(D.energy, i, j) = substep3_energy(D, c, s, rb, fb, i, j, qp, t.divv);
   }
 }
 
