@@ -884,9 +884,10 @@ struct Batch : BatchBase {
                        theta_shmem, lds_max);
     LCHK(raise_dynamic_lds((const void*)k_trans_theta<real>, theta_shmem));
     if (sg) {
-      // self-gravity transforms: two rows of nsec, four columns of 2*nrad (FP64 complex)
-      sg_shm_row = 2 * (size_t)max_nsec * sizeof(cxd);
-      sg_shm_col = 8 * (size_t)max_nrad * sizeof(cxd);
+      // self-gravity transforms: two rows of nsec, four columns of 2*nrad (FP64 complex,
+      // one pad per eight elements: lds_len)
+      sg_shm_row = 2 * (size_t)lds_len(max_nsec) * sizeof(cxd);
+      sg_shm_col = 2 * (size_t)lds_len(4 * max_nrad) * sizeof(cxd);
       if (sg_shm_row + 1024 > lds_max || sg_shm_col + 1024 > lds_max)
         return set_error(FGX_EPARAM, "self-gravity transforms of NSEC=%d, 2*NRAD=%d exceed the %zu B of LDS", max_nsec,
                          2 * max_nrad, lds_max);

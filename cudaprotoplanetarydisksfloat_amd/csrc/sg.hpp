@@ -56,7 +56,15 @@ __device__ __forceinline__ void dft_small(cxd (&v)[R], double s) {
   }
 }
 
+// LDS position of element e of a transform buffer: one 16-byte pad after
+// every eight elements, so that the stores of the first stages (lanes R
+// elements apart) fall in different banks; a buffer of N elements takes
+// lds_len(N) slots
+__device__ __host__ __forceinline__ int lp(int e) { return e + (e >> 3); }
+__device__ __host__ __forceinline__ int lds_len(int n) { return n + (n >> 3) + 1; }
+
 // one radix-R Stockham stage over nl lines of length n stored back to back
+// (element positions lp())
 template <int R>
 __device__ __forceinline__ void fft_stage(const cxd* src, cxd* dst, int n, int nl, int Ns, double s,
                                           const cxd* __restrict__ tw) {
@@ -64,13 +72,12 @@ __device__ __forceinline__ void fft_stage(const cxd* src, cxd* dst, int n, int n
   const int ts = n / (Ns * R);
   for (int t = threadIdx.x; t < m * nl; t += blockDim.x) {
     const int l = t / m, j = t - l * m;
-    const cxd* sl = src + (size_t)l * n;
-    cxd* dl = dst + (size_t)l * n;
+    const int e0 = l * n;
     const int k = j % Ns;
     cxd v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      v[r] = sl[j + r * m];
+      v[r] = src[lp(e0 + j + r * m)];
       if (r > 0) {
         cxd w = tw[r * k * ts];   // exp(-2 pi i r k / (Ns R))
         if (s > 0) w.y = -w.y;
@@ -78,9 +85,9 @@ __device__ __forceinline__ void fft_stage(const cxd* src, cxd* dst, int n, int n
       }
     }
     dft_small<R>(v, s);
-    const int base = (j / Ns) * Ns * R + k;
+    const int base = e0 + (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) dl[base + r * Ns] = v[r];
+    for (int r = 0; r < R; ++r) dst[lp(base + r * Ns)] = v[r];
   }
 }
 
@@ -116,17 +123,17 @@ __global__ void __launch_bounds__(1024) k_sg_rows_fwd(Dev<real> D) {
   FGX_RING_PROLOGUE
   if (i >= nr) return;
   cxd* a = reinterpret_cast<cxd*>(fgx_smem);
-  cxd* b = a + ns;
+  cxd* b = a + lds_len(ns);
   const double q = RT(rmed_d)[rb + i] / RT(rmed_d)[rb];
   const double sq = sqrt(q);
   for (int j = threadIdx.x; j < ns; j += blockDim.x) {
     const double var = s_whole_d(D, rb, i, (real)AT(D.dens, i, j)) * sq;
-    a[j] = cxd{var, var * q};
+    a[lp(j)] = cxd{var, var * q};
   }
   __syncthreads();
   const cxd* out = lds_fft(a, b, ns, 1, c.fac_ns, c.nf_ns, -1.0, D.sgtw + c.tw_ns);
   cxd* w = D.sgw + c.sgw_o + (long long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) w[j] = out[j];
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) w[j] = out[lp(j)];
 }
 
 // pass B: one block per pair of columns (k, NSEC-k) of a disk
@@ -142,11 +149,11 @@ __global__ void __launch_bounds__(1024) k_sg_cols(Dev<real> D) {
   const int ka = t.i, kb = (ns - ka) % ns;
   const int nl = (ka == kb) ? 1 : 2;
   cxd* a = reinterpret_cast<cxd*>(fgx_smem);
-  cxd* b = a + 2 * n;
+  cxd* b = a + lds_len(2 * n);
   const cxd* w = D.sgw + c.sgw_o;
   for (int u = threadIdx.x; u < nl * n; u += blockDim.x) {
     const int l = u / n, ki = u - l * n;
-    a[u] = (ki < nr) ? w[(long long)ki * ns + (l ? kb : ka)] : cxd{0.0, 0.0};
+    a[lp(u)] = (ki < nr) ? w[(long long)ki * ns + (l ? kb : ka)] : cxd{0.0, 0.0};
   }
   __syncthreads();
   const cxd* tw = D.sgtw + c.tw_nr;
@@ -159,21 +166,21 @@ __global__ void __launch_bounds__(1024) k_sg_cols(Dev<real> D) {
     const int col = l ? kb : ka;
     const int pl = (nl == 1) ? 0 : 1 - l;
     const int kim = (ki == 0) ? 0 : n - ki;
-    const cxd zz = z[u], zp = z[pl * n + kim];
+    const cxd zz = z[lp(u)], zp = z[lp(pl * n + kim)];
     const cxd zc = cxd{zp.x, -zp.y};
     const cxd sr = cx_scale(cx_add(zz, zc), 0.5);
     const cxd d = cx_sub(zz, zc);
     const cxd st = cxd{0.5 * d.y, -0.5 * d.x};   // d / (2i)
     const long long kk = (long long)col * n + ki;   // the kernel transforms are stored by column
     const cxd gr = cx_mul(kr[kk], sr), gt = cx_mul(kt[kk], st);
-    o[u] = cx_scale(cx_add(gr, cx_i(gt, 1.0)), -kG);
+    o[lp(u)] = cx_scale(cx_add(gr, cx_i(gt, 1.0)), -kG);
   }
   __syncthreads();
   const cxd* res = lds_fft(o, z, n, nl, c.fac_nr, c.nf_nr, 1.0, tw);
   cxd* wo = D.sgw + c.sgw_o;
   for (int u = threadIdx.x; u < nl * n; u += blockDim.x) {
     const int l = u / n, ki = u - l * n;
-    if (ki < nr) wo[(long long)ki * ns + (l ? kb : ka)] = res[u];
+    if (ki < nr) wo[(long long)ki * ns + (l ? kb : ka)] = res[lp(u)];
   }
 }
 
@@ -185,9 +192,9 @@ __global__ void __launch_bounds__(1024) k_sg_rows_inv(Dev<real> D) {
   FGX_RING_PROLOGUE
   if (i >= nr) return;
   cxd* a = reinterpret_cast<cxd*>(fgx_smem);
-  cxd* b = a + ns;
+  cxd* b = a + lds_len(ns);
   const cxd* w = D.sgw + c.sgw_o + (long long)i * ns;
-  for (int j = threadIdx.x; j < ns; j += blockDim.x) a[j] = w[j];
+  for (int j = threadIdx.x; j < ns; j += blockDim.x) a[lp(j)] = w[j];
   __syncthreads();
   const cxd* out = lds_fft(a, b, ns, 1, c.fac_ns, c.nf_ns, 1.0, D.sgtw + c.tw_ns);
   const double q = RT(rmed_d)[rb + i] / RT(rmed_d)[rb];
@@ -200,9 +207,10 @@ __global__ void __launch_bounds__(1024) k_sg_rows_inv(Dev<real> D) {
   double acc = 0;
   if (threadIdx.x < FGX_TPB) {
     for (int j = threadIdx.x; j < ns; j += FGX_TPB) {
-      const double gr = out[j].x * nrr + self * s_whole_d(D, rb, i, (real)AT(D.dens, i, j));
+      const cxd o = out[lp(j)];
+      const double gr = o.x * nrr + self * s_whole_d(D, rb, i, (real)AT(D.dens, i, j));
       AT(D.sg_accr, i, j) = (real)gr;
-      AT(D.sg_acct, i, j) = (real)(out[j].y * ntt);
+      AT(D.sg_acct, i, j) = (real)(o.y * ntt);
       acc += gr;
     }
   }
