@@ -169,14 +169,22 @@ int sg_host_setup(HostDisk& h, std::vector<double>& khat_r, std::vector<double>&
   const double eps = h.sg_eps;
   // Green's functions on the log grid (compute_kernel, SgMain.cu:49-81), FP64
   std::vector<cd> kr((size_t)NR2 * NS), kt((size_t)NR2 * NS);
+  // the angle's cosine and sine per sector, the radial terms per row
+  std::vector<double> cth(NS), sth(NS);
+  for (int j = 0; j < NS; ++j) {
+    const double th = 2.0 * M_PI * (double)j / (double)NS;
+    cth[j] = std::cos(th);
+    sth[j] = std::sin(th);
+  }
   parallel_for(NR2, [&](int lo, int hi) {
     for (int i = lo; i < hi; ++i) {
       const double u = (i < NR) ? std::log(h.radii[i] / h.radii[0]) : -std::log(h.radii[NR2 - i] / h.radii[0]);
+      const double eu = eps * eps * std::exp(u), ch = std::cosh(u), emu = std::exp(-u);
       for (int j = 0; j < NS; ++j) {
-        const double th = 2.0 * M_PI * (double)j / (double)NS;
-        const double base = std::pow(eps * eps * std::exp(u) + 2.0 * (std::cosh(u) - std::cos(th)), -1.5);
-        kr[(size_t)i * NS + j] = (1.0 + eps * eps - std::cos(th) * std::exp(-u)) * base;
-        kt[(size_t)i * NS + j] = std::sin(th) * base;
+        const double x = eu + 2.0 * (ch - cth[j]);
+        const double base = 1.0 / (x * std::sqrt(x));   // x^-1.5
+        kr[(size_t)i * NS + j] = (1.0 + eps * eps - cth[j] * emu) * base;
+        kt[(size_t)i * NS + j] = sth[j] * base;
       }
     }
   });
