@@ -171,9 +171,11 @@ __device__ __forceinline__ real rad_vl(real wm, real w0, real wp, real idm, real
   return (dqp * dqm > 0) ? (real)2 * dqp * dqm * vl_rcp(dqp + dqm) : (real)0;
 }
 
-// LAB: the label is advected as an eighth quantity (density * label)
+// LAB: the label is advected as an eighth quantity (density * label).
+// FP64 without the label is held to 168 VGPRs for three waves per SIMD (it
+// fits without spilling, and the walk is latency-bound at two)
 template <class real, bool LAB>
-__global__ void __launch_bounds__(FGX_TPB) k_trans_rad_strip(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
+__global__ void __launch_bounds__(FGX_TPB, sizeof(real) == 8 && !LAB ? 3 : 1) k_trans_rad_strip(Dev<real> D, const CellTile* __restrict__ tiles, int rs) {
   constexpr int NQ = LAB ? 8 : 7;
   const CellTile t = tiles[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   if (t.d < 0) return;
