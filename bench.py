@@ -2,12 +2,15 @@
 
 One step = one AlgoGas call (one DT, with per-disk CFL substeps) for every
 disk of the batch.  value = total cell-updates (cells x substeps, summed over
-disks) / wall time of the timed steps, inputs resident in HBM.  The same
-disks advanced one at a time through the single-disk path give `plain`.
-`roofline` is the dominant kernel's algorithmic bytes per launch over its
-average device time (HIP events on the library's stream, recorded during a
-second pass over the same steps); `cpu_baseline` times the FP64 OpenMP
-oracle on a bounded sample of the same disks.
+disks) / wall time of the timed steps, inputs resident in HBM, with the disks
+run as a group of concurrent batches (`group`: how many, picked by timing 1..3
+before the timed region, and the throughput of the same disks as one batch).
+The same disks advanced one at a time through the single-disk path give
+`plain`.  `roofline` is the dominant kernel's algorithmic bytes per launch
+over its average device time (HIP events on the library's stream, recorded
+over as many steps of the disks as one batch, so that each kernel has the
+device to itself); `cpu_baseline` times the FP64 OpenMP oracle on a bounded
+sample of the same disks.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|large|plain|sg]
                     [--serve-requests N]
@@ -165,6 +168,8 @@ def main():
                     help="requests of the serving phase (p50/p99 completion time); 0 skips it")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--parts", type=int, default=0,
+                    help="concurrent batches the disks run as (fgx_group_*); 0: the fastest of 1..3")
     ap.add_argument("--no-tune", action="store_true",
                     help="run with the engine's own tiling instead of the autotuner's choice")
     ap.add_argument("--tiling-cache", default=None,
@@ -176,7 +181,7 @@ def main():
               f"by disk, which this build does not launch", file=sys.stderr)
         sys.exit(2)
     import torch
-    from cudaprotoplanetarydisksfloat_amd import F32, F64, Batch, autotune
+    from cudaprotoplanetarydisksfloat_amd import F32, F64, Batch, Group, autotune
 
     prec = F32 if args.precision == "f32" else F64
     if args.copy_mib:
@@ -201,19 +206,55 @@ def main():
         if args.tune_only:
             print(json.dumps({"tiling": tiling.as_dict(), "tuner": tuner}))
             return
-    batch = Batch([p for p, _ in loaded], [pl for _, pl in loaded], prec, tiling=tiling)
+    ps, pls = [p for p, _ in loaded], [pl for _, pl in loaded]
+    # the disks as `parts` concurrent batches (fgx_group_*, each on its own
+    # stream and host thread): the count is the fastest of 1..3 over two DTs
+    # after one, outside the timed region (every count gives the same fields)
+    cands = [args.parts] if args.parts else ([1, 2, 3] if len(loaded) > 1 else [1])
+    runner, rates = None, {}
+    for parts in cands:
+        g = Group(ps, pls, prec, parts=parts, tiling=tiling)
+        g.step(1)
+        torch.cuda.synchronize()
+        bf = _substeps(g)
+        t0 = time.perf_counter()
+        g.step(2)
+        torch.cuda.synchronize()
+        rates[parts] = _updates(g, bf) / (time.perf_counter() - t0)
+        if runner is None or rates[parts] > rates[len(runner.parts)]:
+            if runner is not None:
+                runner.close()
+            runner = g
+        else:
+            g.close()
+    group = {"parts": len(runner.parts), "candidates": {str(k): round(v) for k, v in rates.items()},
+             "disks_per_batch": [b.n for b in runner.parts]}
+    # the same disks as one batch on one stream, brought to the same DT, so
+    # that both are timed over the same DTs
+    batch = Batch(ps, pls, prec, tiling=tiling)
     desc = dict(desc, tiling=batch.tiling().as_dict())
-    batch.step(args.warmup)
+    runner.step(args.warmup)
+    batch.step(3 + args.warmup)
     torch.cuda.synchronize()
+    before = _substeps(runner)
+    t0 = time.perf_counter()
+    runner.step(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    updates = _updates(runner, before)
+    value = updates / wall
+    runner.close()
     before = _substeps(batch)
     t0 = time.perf_counter()
     batch.step(args.steps)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    updates = _updates(batch, before)
-    value = updates / wall
+    group["one_batch_value"] = _updates(batch, before) / (time.perf_counter() - t0)
+    group["over_one_batch"] = round(value / group["one_batch_value"], 3)
 
-    # second pass over as many steps with per-kernel HIP events
+    # a second pass of the one batch over as many steps with per-kernel HIP
+    # events, each kernel alone on the device (in a group, launches of
+    # different batches overlap and share the CUs, so a launch's duration
+    # would not be its own)
     batch.profile(True)
     before = _substeps(batch)
     batch.step(args.steps)
@@ -244,7 +285,7 @@ def main():
                       "unit": "G wave-instr/s", "frac": round(rate / 1e9 / VALU_PEAK_GINSTR, 4),
                       "source": traffic_src}
     binding = "valu_issue" if valu_issue and valu_issue["frac"] > hbm_frac else "hbm"
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    roofline = {"bound": "hbm", "kernel": dom, "pass": "the disks as one batch on one stream, kernels alone on the device", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(hbm_frac, 4), "traffic": traffic,
                 "traffic_source": traffic_src, "valu_issue": valu_issue, "binding": binding,
                 "alg_bytes_per_launch": round(d["alg_bytes_per_launch"]),
@@ -288,7 +329,7 @@ def main():
             "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
             "config": dict(desc, cell_updates=updates, cells=ncell), "roofline": roofline,
-            "cpu_baseline": cpu, "plain": plain, "serving": serve, "tuner": tuner}
+            "cpu_baseline": cpu, "plain": plain, "serving": serve, "tuner": tuner, "group": group}
     print(json.dumps(line))
 
 

@@ -24,7 +24,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function"]
 
 HIP_SOURCES = ["engine.hip"]
-CXX_SOURCES = ["params.cpp", "setup.cpp", "capi.cpp", "output.cpp", "sg.cpp", "server.cpp", "tune.cpp"]
+CXX_SOURCES = ["params.cpp", "setup.cpp", "capi.cpp", "output.cpp", "sg.cpp", "server.cpp", "tune.cpp", "group.cpp"]
 
 
 def _run(cmd):

@@ -932,6 +932,16 @@ struct Batch : BatchBase {
     for (int i = 0; i < hd[d].nrad; ++i)
       for (int j = 0; j < hd[d].nsec; ++j) res[(size_t)i * hd[d].nsec + j] -= vref_h[d][i];
     LCHK(upload_field(d, D.vtheta, res));
+    // the ghost ring (ring nrad) of the velocities into the other buffer of
+    // each pair as well: no substep writes that ring, and a disk that ends
+    // its DT before its batch does has its velocities copied across the pair
+    // with it (k_vel_fixup), so with both copies equal the ring reads the
+    // same whatever the batch's substep count, i.e. whatever shares the batch
+    {
+      const size_t ns = (size_t)hd[d].nsec, g0 = (size_t)hd[d].nrad * ns;
+      LCHK(upload_vec(D.vradint + dch[d].fo + g0, std::vector<double>(hd[d].vrad.begin() + g0, hd[d].vrad.end()), ns));
+      LCHK(upload_vec(D.vthetaint + dch[d].fo + g0, std::vector<double>(res.begin() + g0, res.end()), ns));
+    }
     if (D.adiabatic) {
       // the energy is kept as a residual of its ring profile EnergyMed
       std::vector<double> eres(hd[d].energy);

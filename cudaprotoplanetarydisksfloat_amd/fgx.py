@@ -174,6 +174,11 @@ _sigs = {
     "fgx_server_read_field": ([_vp, _i, _i, _i, _vp, _sz], _i),
     "fgx_server_stats_get": ([_vp, ctypes.POINTER(ServerStats)], _i),
     "fgx_server_destroy": ([_vp], _i),
+    "fgx_group_create": ([_i, _vp, ctypes.POINTER(Planets), _i, _i, ctypes.POINTER(Tiling), ctypes.POINTER(_vp)], _i),
+    "fgx_group_step": ([_vp, _i], _i), "fgx_group_parts": ([_vp, ctypes.POINTER(_i)], _i),
+    "fgx_group_batch": ([_vp, _i, ctypes.POINTER(_vp)], _i),
+    "fgx_group_disk": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_i)], _i),
+    "fgx_group_destroy": ([_vp], _i),
 }
 for _n, (_a, _r) in _sigs.items():
     _f = getattr(lib, _n)
@@ -296,7 +301,8 @@ class Batch:
 
     def close(self):
         if getattr(self, "h", None):
-            lib.fgx_batch_destroy(self.h)
+            if getattr(self, "_owned", True):   # a group's batch goes with its group
+                lib.fgx_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -400,6 +406,77 @@ class Batch:
 
 
 Disk = Batch
+
+
+class Group:
+    """One set of disks advanced as `parts` concurrent batches (fgx_group_*):
+    each batch on its own stream and host thread, a disk's fields those it
+    has in one batch of all the disks.  `parts` is the list of the batches
+    (views that the group owns); disk(d) is (batch, index) of disk d."""
+
+    def __init__(self, params, planets, precision=F32, parts=2, tiling=None):
+        n, arr, pls = _disk_arrays(params, planets)
+        if isinstance(tiling, dict):
+            tiling = Tiling(**tiling)
+        h = ctypes.c_void_p()
+        _check(lib.fgx_group_create(n, arr, pls, precision, parts, ctypes.byref(tiling) if tiling else None,
+                                    ctypes.byref(h)))
+        self.h = h
+        self.n = n
+        self.precision = precision
+        np_ = ctypes.c_int()
+        _check(lib.fgx_group_parts(h, ctypes.byref(np_)))
+        self.parts = []
+        for k in range(np_.value):
+            bh = ctypes.c_void_p()
+            _check(lib.fgx_group_batch(h, k, ctypes.byref(bh)))
+            b = Batch.__new__(Batch)   # a view: the group destroys the batch
+            b.h, b.precision, b.params, b.shape, b._owned = bh, precision, [], [], False
+            self.parts.append(b)
+        self._where = []
+        for d in range(n):
+            bh, ix = ctypes.c_void_p(), ctypes.c_int()
+            _check(lib.fgx_group_disk(h, d, ctypes.byref(bh), ctypes.byref(ix)))
+            k = next(q for q, b in enumerate(self.parts) if b.h.value == bh.value)
+            self._where.append((k, ix.value))
+            self.parts[k].params.append(params[d].copy())
+            self.parts[k].shape.append((int(params[d].get("NRAD")), int(params[d].get("NSEC"))))
+        for b in self.parts:
+            b.n = len(b.shape)
+        self.shape = [(int(p.get("NRAD")), int(p.get("NSEC"))) for p in params]
+
+    def disk(self, d):
+        """(batch view, index in it) of disk d."""
+        k, ix = self._where[d]
+        return self.parts[k], ix
+
+    def step(self, n_dt=1):
+        _check(lib.fgx_group_step(self.h, n_dt))
+
+    def info(self, disk=0):
+        b, ix = self.disk(disk)
+        return b.info(ix)
+
+    def field(self, name, disk=0, ghost=False, dtype=np.float64):
+        b, ix = self.disk(disk)
+        return b.field(name, ix, ghost, dtype)
+
+    def planets(self, disk=0):
+        b, ix = self.disk(disk)
+        return b.planets(ix)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for b in self.parts:
+                b.h = None   # the views' handles go with the group
+            lib.fgx_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_count():

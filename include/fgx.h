@@ -305,6 +305,28 @@ int fgx_batch_tiling(fgx_batch* b, fgx_tiling* out);
 int fgx_autotune(int n, const fgx_params* p, const fgx_planets* pl, int precision, int n_dt,
                  const char* cache_path, fgx_tiling* out, fgx_tune_report* report);
 
+/* ---- disk groups: one set of disks as several concurrent batches (no
+ * counterpart in the reference, which runs one disk per process,
+ * src/Main.cu:87-98).  The disks are dealt, largest first, to `parts`
+ * batches (each to the one with the fewest cells so far; a batch keeps its
+ * disks in their given order), all of the tiling t (NULL: the engine's
+ * choice per batch).  fgx_group_step advances every batch n_dt DTs, each on
+ * its own stream and host thread, so one batch's sweeps fill the CUs the
+ * other's short per-disk launches leave idle; it returns when all are done.
+ * A disk's fields are those it has in one batch of all the disks, bit for
+ * bit.  Its batch and its index there (fgx_group_disk) serve every
+ * fgx_batch_* call on it (read, write, info, planets, freeze, checkpoint);
+ * those batches belong to the group, which destroys them. */
+#define FGX_GROUP_MAX_PARTS 8
+typedef struct fgx_group fgx_group;
+int fgx_group_create(int n, const fgx_params* p, const fgx_planets* pl, int precision, int parts,
+                     const fgx_tiling* t, fgx_group** out);
+int fgx_group_step(fgx_group* g, int n_dt);
+int fgx_group_parts(fgx_group* g, int* n);
+int fgx_group_batch(fgx_group* g, int part, fgx_batch** out);
+int fgx_group_disk(fgx_group* g, int disk, fgx_batch** batch, int* index);
+int fgx_group_destroy(fgx_group* g);
+
 /* ---- serving loop (a stream of run requests; no counterpart in the reference)
  * Requests (one disk each: parameters, planets, number of DTs, arrival time
  * in seconds from the start of fgx_server_run) are served by continuous

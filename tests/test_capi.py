@@ -45,6 +45,21 @@ def test_create_without_device_fails_cleanly():
     assert e.value.code == -2
 
 
+def test_group_arguments_and_missing_device():
+    from cudaprotoplanetarydisksfloat_amd import FgxError, Group, configs, device_count
+    loaded = [configs.load(*configs.plain_disk(nrad=16 + 8 * k, nsec=32, disk_id=k)) for k in range(3)]
+    ps, pls = [p for p, _ in loaded], [pl for _, pl in loaded]
+    for parts in (0, 9):   # outside 1..FGX_GROUP_MAX_PARTS, refused before any device call
+        with pytest.raises(FgxError) as e:
+            Group(ps, pls, parts=parts)
+        assert e.value.code == -1 and "batches" in str(e.value)
+    if device_count() == 0:
+        # the first batch's creation fails; its message is the caller's last error
+        with pytest.raises(FgxError) as e:
+            Group(ps, pls, parts=2)
+        assert e.value.code == -2
+
+
 def test_autotune_without_device_fails_cleanly(tmp_path):
     from cudaprotoplanetarydisksfloat_amd import FgxError, autotune, configs, device_count
     if device_count() > 0:
