@@ -255,8 +255,10 @@ def main():
     # events, each kernel alone on the device (in a group, launches of
     # different batches overlap and share the CUs, so a launch's duration
     # would not be its own)
-    batch.profile(True)
+    # (the substep counts are read outside it: each disk query forms the
+    # disk's circumplanetary mass with a launch of its own)
     before = _substeps(batch)
+    batch.profile(True)
     batch.step(args.steps)
     torch.cuda.synchronize()
     batch.profile(False)
